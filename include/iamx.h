@@ -1,0 +1,143 @@
+/*
+ * iamx.h -- C ABI of libiamx.so, the HIP/gfx950 kernels behind libiarray.so.
+ *
+ * Plain pointers and sizes only (no HIP or torch types in any signature).
+ * The host C library (libiarray.so) is the only intended caller; each entry
+ * names the reference routine whose work it replaces.  All device work is
+ * issued on one library-owned HIP stream (default priority) and every entry
+ * returns 0 on success or a negative IAMX_E* code; iamx_last_error() gives
+ * the message.
+ */
+#ifndef IAMX_H_
+#define IAMX_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IAMX_API __attribute__((visibility("default")))
+
+enum {
+  IAMX_OK = 0,
+  IAMX_ENODEV = -1,   /* no HIP device / runtime failure at init */
+  IAMX_EHIP = -2,     /* a HIP call failed */
+  IAMX_EINVAL = -3,   /* bad argument (shape, size, pointer) */
+  IAMX_EDATA = -4,    /* a compressed stream is malformed */
+  IAMX_ENOMEM = -5,
+};
+
+/* ---- device, memory, stream ---------------------------------------------- */
+IAMX_API int iamx_init(void);
+IAMX_API const char *iamx_last_error(void);
+IAMX_API int iamx_device_info(char *name, int namelen, int *cus, char *arch, int archlen);
+IAMX_API void *iamx_malloc(size_t bytes);
+IAMX_API void iamx_free(void *dptr);
+IAMX_API int iamx_h2d(void *dst, const void *src, size_t bytes);
+IAMX_API int iamx_d2h(void *dst, const void *src, size_t bytes);
+IAMX_API int iamx_d2d(void *dst, const void *src, size_t bytes);
+IAMX_API int iamx_memset(void *dst, int value, size_t bytes);
+IAMX_API int iamx_sync(void);
+/* Scratch that lives until iamx_shutdown; grown (never shrunk) on demand.
+ * Slot s (0..7) is independent of the others. */
+IAMX_API void *iamx_workspace(int slot, size_t bytes);
+IAMX_API void iamx_shutdown(void);
+
+/* ---- profiling (HIP events around every launch on the library stream) ---- */
+IAMX_API void iamx_prof_enable(int on);
+IAMX_API void iamx_prof_reset(void);
+IAMX_API int iamx_prof_entry(int i, const char **name, int64_t *launches, double *total_ms);
+IAMX_API double iamx_prof_span_ms(void); /* first start .. last stop since reset */
+
+/* ---- Blosc2 block decode --------------------------------------------------
+ * Replaces the decode inside blosc2_getitem_ctx (src/linalg/iarray_gemm.c:259,
+ * src/linalg/iarray_gemv.c:194) and caterva_get_slice_buffer
+ * (src/iarray_container.c:704,730): LZ4 streams are decoded by one wavefront
+ * each into scratch, then every block is un-shuffled / bit-un-shuffled and
+ * written into its tile of a dense row-major destination (edge-trimmed). */
+typedef struct iamx_lz4_job_s {
+  uint64_t src;    /* device address of the LZ4 payload */
+  int64_t dst_off; /* byte offset in scratch of the decoded stream */
+  int32_t csize;   /* compressed bytes */
+  int32_t nbytes;  /* decoded bytes (the stream's neblock) */
+} iamx_lz4_job_t;
+
+#define IAMX_MAX_STREAMS 8
+enum { IAMX_SRC_CONST = 0, IAMX_SRC_ADDR = 1, IAMX_SRC_SCRATCH = 2 };
+enum { IAMX_FILTER_NONE = 0, IAMX_FILTER_SHUFFLE = 1, IAMX_FILTER_BITSHUFFLE = 2, IAMX_FILTER_BITSHUFFLE_V2 = 3 };
+
+typedef struct iamx_block_job_s {
+  uint64_t src[IAMX_MAX_STREAMS]; /* device address or scratch offset of stream s */
+  uint8_t kind[IAMX_MAX_STREAMS]; /* IAMX_SRC_* */
+  uint8_t fill[IAMX_MAX_STREAMS]; /* byte of a constant stream */
+  int32_t neblock;                /* bytes per stream */
+  int16_t nstreams;
+  int16_t typesize;
+  int32_t filter;                 /* IAMX_FILTER_* */
+  int32_t bsize;                  /* bytes of the block */
+  int32_t rows, cols;             /* block tile shape in elements (row-major) */
+  int32_t vrows, vcols;           /* valid extent inside the tile */
+  int64_t dst_off;                /* element offset of the tile origin in dst */
+  int64_t dst_ld;                 /* destination leading dimension (elements) */
+} iamx_block_job_t;
+
+IAMX_API int iamx_decode(const iamx_lz4_job_t *d_lz4, int n_lz4, const iamx_block_job_t *d_blocks, int n_blocks,
+                         void *d_scratch, void *d_dst);
+
+/* ---- GEMM / GEMV -------------------------------------------------------------
+ * Replaces the per-C-block cblas_?gemm of the gemm branch
+ * (src/linalg/matmul/gemm.c:96-102) and of iarray_opt_gemm
+ * (src/linalg/iarray_gemm.c:276-294): one launch computes the whole dense
+ * C(MxN) = A(MxK) B(KxN), row-major.  dtype 0 = float64, 1 = float32.
+ * precision: 0 plain, 1 split-bf16 (float32), 2 ozaki-int8 (float64, `slices`). */
+IAMX_API int iamx_gemm(int dtype, int precision, int slices, int64_t M, int64_t N, int64_t K, const void *A,
+                       int64_t lda, const void *B, int64_t ldb, void *C, int64_t ldc);
+
+/* Replaces the per-C-block cblas_?gemv of the gemv branch
+ * (src/linalg/matmul/gemv.c:92-98) and of iarray_opt_gemv
+ * (src/linalg/iarray_gemv.c:212-224): y(M) = A(MxK) x(K). */
+IAMX_API int iamx_gemv(int dtype, int64_t M, int64_t K, const void *A, int64_t lda, const void *x, void *y);
+
+/* GEMV fused with the block decode (the opt_gemv shape of the work,
+ * src/linalg/iarray_gemv.c:156-229): A's LZ4 streams are decoded into
+ * scratch, then each block is un-filtered on chip and multiplied with its x
+ * segment without ever writing dense A; row sums per block column land in
+ * `partial` (nbc x M) and are added in block-column order into y.
+ * d_blk_rc holds 4 int32 per block job: first row and first column of the
+ * tile in A, the tile's column index in the global block grid (partial row),
+ * and 0; nbc is the number of block columns; bs0/bs1 are the block shape
+ * (bs1 <= 1024). */
+IAMX_API int iamx_gemv_fused(int dtype, const iamx_lz4_job_t *d_lz4, int n_lz4, const iamx_block_job_t *d_blocks,
+                             const int32_t *d_blk_rc, int n_blocks, int nbc, int bs0, int bs1, const void *x,
+                             void *y, int64_t M, void *d_scratch, void *d_partial);
+
+/* ---- C write stage --------------------------------------------------------------
+ * Replaces the shuffle/bitshuffle that blosc2_compress_ctx applies to every C
+ * block before LZ4 (src/linalg/matmul/gemm.c:216-230): gathers each C block
+ * tile from the dense C (zero padding outside the valid extent) and writes the
+ * filtered block bytes chunk-major, block-major into `out`. */
+typedef struct iamx_tiling_s {
+  int64_t shape[2];
+  int64_t chunk[2];
+  int64_t block[2];
+  int64_t nchunks[2]; /* chunks per axis (extshape / chunkshape) */
+  int64_t nblocks[2]; /* blocks per chunk per axis (extchunkshape / blockshape) */
+  int32_t typesize;
+  int32_t filter; /* IAMX_FILTER_* */
+} iamx_tiling_t;
+
+IAMX_API int iamx_pack_blocks(const iamx_tiling_t *tiling, const void *d_src, int64_t src_ld, void *d_out);
+
+/* Transpose a dense row-major (rows x cols) matrix: the omatcopy of a
+ * transposed view (src/iarray_container.c:718-723). */
+IAMX_API int iamx_transpose(int typesize, int64_t rows, int64_t cols, const void *d_src, void *d_dst);
+
+/* Error word written by the decode kernels (0 = no error since last reset). */
+IAMX_API int iamx_decode_status(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

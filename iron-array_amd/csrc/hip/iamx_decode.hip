@@ -1,0 +1,344 @@
+// iamx_decode.hip -- Blosc2 block decode (LZ4 + un-shuffle / bit-un-shuffle)
+// and the C write-side block packing, for gfx950.
+//
+// Decode is two launches:
+//   k_lz4_streams   one 64-lane wavefront per LZ4 stream.  The token stream is
+//                   parsed uniformly by all lanes (every lane holds the same
+//                   ip/op, loads of one address are broadcast), literal runs
+//                   and matches are copied by the 64 lanes together.  A match
+//                   copies out[op+i] = out[op-off + (i mod off)], so every
+//                   source byte precedes op and one match never depends on
+//                   itself; stores of earlier sequences are made visible with
+//                   a workgroup-scope release/acquire only when the match
+//                   source overlaps bytes stored since the last such fence.
+//   k_unfilter      one workgroup per block: gathers each element's bytes from
+//                   its byte planes (byte shuffle) or bit planes (bitshuffle,
+//                   8x8 bit-matrix transposes), and writes the element into its
+//                   tile of the dense destination, skipping the padding outside
+//                   the valid extent.
+// Stream sources are either the compressed chunk arena (raw streams, read in
+// place), the LZ4 scratch, or a constant byte (zero / run streams, special
+// chunks).
+#include "iamx_internal.h"
+
+namespace {
+
+// Delta-swap transpose of an 8x8 bit matrix held in a uint64 (row r = byte r,
+// column c = bit c): bit 8r+c moves to bit 8c+r (Hacker's Delight, 7-3).
+__device__ __forceinline__ uint64_t transpose8(uint64_t x) {
+  uint64_t t;
+  t = (x ^ (x >> 7)) & 0x00AA00AA00AA00AAull;
+  x = x ^ t ^ (t << 7);
+  t = (x ^ (x >> 14)) & 0x0000CCCC0000CCCCull;
+  x = x ^ t ^ (t << 14);
+  t = (x ^ (x >> 28)) & 0x00000000F0F0F0F0ull;
+  x = x ^ t ^ (t << 28);
+  return x;
+}
+
+__global__ __launch_bounds__(64) void k_lz4_streams(const iamx_lz4_job_t *__restrict__ jobs, int njobs,
+                                                    uint8_t *__restrict__ scratch, int *status) {
+  const int j = blockIdx.x;
+  if (j >= njobs) return;
+  const iamx_lz4_job_t job = jobs[j];
+  const uint8_t *__restrict__ src = reinterpret_cast<const uint8_t *>(job.src);
+  uint8_t *dst = scratch + job.dst_off;
+  const int lane = threadIdx.x;
+  const int32_t iend = job.csize, oend = job.nbytes;
+  int32_t ip = 0, op = 0, fenced = 0;
+  int bad = 0;
+  for (;;) {
+    if (ip >= iend) { bad = 1; break; }
+    const uint32_t token = src[ip++];
+    int32_t lit = (int32_t)(token >> 4);
+    if (lit == 15) {
+      uint32_t b;
+      do {
+        if (ip >= iend) { bad = 2; break; }
+        b = src[ip++];
+        lit += (int32_t)b;
+      } while (b == 255);
+      if (bad) break;
+    }
+    if (lit > iend - ip || lit > oend - op) { bad = 3; break; }
+    for (int32_t i = lane; i < lit; i += 256) {
+      dst[op + i] = src[ip + i];
+      if (i + 64 < lit) dst[op + i + 64] = src[ip + i + 64];
+      if (i + 128 < lit) dst[op + i + 128] = src[ip + i + 128];
+      if (i + 192 < lit) dst[op + i + 192] = src[ip + i + 192];
+    }
+    ip += lit;
+    op += lit;
+    if (ip == iend) break;  // last sequence: literals only
+    if (iend - ip < 2) { bad = 4; break; }
+    const int32_t off = (int32_t)src[ip] | ((int32_t)src[ip + 1] << 8);
+    ip += 2;
+    int32_t ml = (int32_t)(token & 15u);
+    if (ml == 15) {
+      uint32_t b;
+      do {
+        if (ip >= iend) { bad = 5; break; }
+        b = src[ip++];
+        ml += (int32_t)b;
+      } while (b == 255);
+      if (bad) break;
+    }
+    ml += 4;
+    if (off == 0 || off > op || ml > oend - op) { bad = 6; break; }
+    const int32_t mstart = op - off;
+    const int32_t span = ml < off ? ml : off;
+    if (mstart + span > fenced) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      fenced = op;
+    }
+    if (off >= ml) {
+      for (int32_t i = lane; i < ml; i += 64) dst[op + i] = dst[mstart + i];
+    } else {
+      for (int32_t i = lane; i < ml; i += 64) dst[op + i] = dst[mstart + (i % off)];
+    }
+    op += ml;
+  }
+  if (!bad && op != oend) bad = 7;
+  if (bad && lane == 0) atomicOr(status, 1 << bad);
+}
+
+struct Planes {
+  const uint8_t *ptr[IAMX_MAX_STREAMS];
+  uint8_t fill[IAMX_MAX_STREAMS];
+  uint8_t is_const[IAMX_MAX_STREAMS];
+};
+
+// Byte q of the block's concatenated streams.
+__device__ __forceinline__ uint8_t block_byte(const Planes &P, int32_t neblock, int32_t q) {
+  const int s = q / neblock;
+  const int32_t o = q - s * neblock;
+  return P.is_const[s] ? P.fill[s] : P.ptr[s][o];
+}
+
+template <typename T>
+__device__ __forceinline__ void put_elem(uint8_t *dst, int64_t idx, uint64_t v) {
+  reinterpret_cast<T *>(dst)[idx] = (T)v;
+}
+
+__device__ __forceinline__ void store_elem(uint8_t *dst, int ts, int64_t idx, uint64_t v) {
+  switch (ts) {
+    case 8: put_elem<uint64_t>(dst, idx, v); break;
+    case 4: put_elem<uint32_t>(dst, idx, v); break;
+    case 2: put_elem<uint16_t>(dst, idx, v); break;
+    default: put_elem<uint8_t>(dst, idx, v); break;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_unfilter(const iamx_block_job_t *__restrict__ jobs, int njobs,
+                                                  const uint8_t *__restrict__ scratch, uint8_t *__restrict__ dst) {
+  __shared__ iamx_block_job_t J;
+  __shared__ Planes P;
+  const int bj = blockIdx.x;
+  if (bj >= njobs) return;
+  if (threadIdx.x == 0) {
+    J = jobs[bj];
+    for (int s = 0; s < IAMX_MAX_STREAMS; s++) {
+      P.is_const[s] = (s >= J.nstreams) || J.kind[s] == IAMX_SRC_CONST;
+      P.fill[s] = s < J.nstreams ? J.fill[s] : 0;
+      P.ptr[s] = J.kind[s] == IAMX_SRC_ADDR      ? reinterpret_cast<const uint8_t *>(J.src[s])
+                 : J.kind[s] == IAMX_SRC_SCRATCH ? scratch + J.src[s]
+                                                 : nullptr;
+    }
+  }
+  __syncthreads();
+  const int ts = J.typesize;
+  const int32_t n = J.bsize / ts;  // elements in the block
+  const int32_t cols = J.cols;
+  const int64_t ld = J.dst_ld;
+  const int64_t base = J.dst_off;
+  if (J.filter == IAMX_FILTER_BITSHUFFLE || J.filter == IAMX_FILTER_BITSHUFFLE_V2) {
+    const int32_t n8 = J.filter == IAMX_FILTER_BITSHUFFLE ? n - n % 8 : n;
+    const int32_t rowlen = n8 / 8;
+    const int32_t ngroups = rowlen + (n - n8);  // 8-element groups, then tail elements one by one
+    for (int32_t g = threadIdx.x; g < ngroups; g += blockDim.x) {
+      uint64_t elems[8];
+      int cnt;
+      int32_t e0;
+      if (g < rowlen) {
+        for (int i = 0; i < 8; i++) elems[i] = 0;
+        for (int j = 0; j < ts; j++) {
+          uint64_t x = 0;
+          for (int k = 0; k < 8; k++) {
+            const int32_t q = (8 * j + k) * rowlen + g;
+            const uint8_t b = J.nstreams == 1 ? (P.is_const[0] ? P.fill[0] : P.ptr[0][q]) : block_byte(P, J.neblock, q);
+            x |= (uint64_t)b << (8 * k);
+          }
+          const uint64_t y = transpose8(x);
+          for (int i = 0; i < 8; i++) elems[i] |= ((y >> (8 * i)) & 0xffull) << (8 * j);
+        }
+        cnt = 8;
+        e0 = 8 * g;
+      } else {
+        const int32_t e = n8 + (g - rowlen);
+        uint64_t v = 0;
+        for (int j = 0; j < ts; j++) v |= (uint64_t)block_byte(P, J.neblock, e * ts + j) << (8 * j);
+        elems[0] = v;
+        cnt = 1;
+        e0 = e;
+      }
+      int32_t r = e0 / cols, c = e0 - r * cols;
+      for (int i = 0; i < cnt; i++) {
+        if (r < J.vrows && c < J.vcols) store_elem(dst, ts, base + (int64_t)r * ld + c, elems[i]);
+        if (++c == cols) { c = 0; ++r; }
+      }
+    }
+    return;
+  }
+  // byte shuffle (planes) or no filter
+  const bool shuffled = J.filter == IAMX_FILTER_SHUFFLE && ts > 1;
+  const uint8_t *plane[8];
+  uint8_t pconst[8], pfill[8];
+  if (shuffled) {
+    for (int j = 0; j < ts; j++) {
+      if (J.nstreams == ts) {
+        plane[j] = P.ptr[j]; pconst[j] = P.is_const[j]; pfill[j] = P.fill[j];
+      } else {  // one stream holds all planes back to back
+        plane[j] = P.ptr[0] ? P.ptr[0] + (int64_t)j * n : nullptr; pconst[j] = P.is_const[0]; pfill[j] = P.fill[0];
+      }
+    }
+  }
+  for (int32_t r = threadIdx.x / 64; r < J.vrows; r += blockDim.x / 64) {
+    for (int32_t c = threadIdx.x % 64; c < J.vcols; c += 64) {
+      const int32_t e = r * cols + c;
+      uint64_t v = 0;
+      if (shuffled) {
+        for (int j = 0; j < ts; j++) v |= (uint64_t)(pconst[j] ? pfill[j] : plane[j][e]) << (8 * j);
+      } else {
+        for (int j = 0; j < ts; j++) v |= (uint64_t)block_byte(P, J.neblock, e * ts + j) << (8 * j);
+      }
+      store_elem(dst, ts, base + (int64_t)r * ld + c, v);
+    }
+  }
+}
+
+__device__ __forceinline__ uint64_t load_elem(const uint8_t *src, int ts, int64_t idx) {
+  switch (ts) {
+    case 8: return reinterpret_cast<const uint64_t *>(src)[idx];
+    case 4: return reinterpret_cast<const uint32_t *>(src)[idx];
+    case 2: return reinterpret_cast<const uint16_t *>(src)[idx];
+    default: return src[idx];
+  }
+}
+
+// One workgroup per C block: gather the tile (zeros outside the valid extent),
+// apply the filter, write the block's bytes at (chunk * nblocks + block) * bsize.
+__global__ __launch_bounds__(256) void k_pack(iamx_tiling_t t, const uint8_t *__restrict__ src, int64_t ld,
+                                              uint8_t *__restrict__ out) {
+  const int64_t nbpc = t.nblocks[0] * t.nblocks[1];
+  const int64_t id = blockIdx.x;
+  const int64_t nc = id / nbpc, nb = id % nbpc;
+  const int64_t ci0 = nc / t.nchunks[1], ci1 = nc % t.nchunks[1];
+  const int64_t bi0 = nb / t.nblocks[1], bi1 = nb % t.nblocks[1];
+  const int ts = t.typesize;
+  const int32_t rows = (int32_t)t.block[0], cols = (int32_t)t.block[1];
+  const int32_t n = rows * cols;
+  const int64_t r_in = bi0 * t.block[0], c_in = bi1 * t.block[1];
+  const int64_t gr0 = ci0 * t.chunk[0] + r_in, gc0 = ci1 * t.chunk[1] + c_in;
+  int64_t vr = t.chunk[0] - r_in;
+  if (t.shape[0] - gr0 < vr) vr = t.shape[0] - gr0;
+  int64_t vc = t.chunk[1] - c_in;
+  if (t.shape[1] - gc0 < vc) vc = t.shape[1] - gc0;
+  uint8_t *o = out + id * (int64_t)n * ts;
+  auto elem = [&](int32_t e) -> uint64_t {
+    const int32_t r = e / cols, c = e - (e / cols) * cols;
+    if (r >= vr || c >= vc) return 0;
+    return load_elem(src, ts, (gr0 + r) * ld + gc0 + c);
+  };
+  if (t.filter == IAMX_FILTER_BITSHUFFLE) {
+    const int32_t n8 = n - n % 8, rowlen = n8 / 8;
+    for (int32_t g = threadIdx.x; g < rowlen; g += blockDim.x) {
+      uint64_t v[8];
+      for (int i = 0; i < 8; i++) v[i] = elem(8 * g + i);
+      for (int j = 0; j < ts; j++) {
+        uint64_t x = 0;
+        for (int i = 0; i < 8; i++) x |= ((v[i] >> (8 * j)) & 0xffull) << (8 * i);
+        const uint64_t y = transpose8(x);
+        for (int k = 0; k < 8; k++) o[(int64_t)(8 * j + k) * rowlen + g] = (uint8_t)(y >> (8 * k));
+      }
+    }
+    for (int32_t e = n8 + threadIdx.x; e < n; e += blockDim.x) {
+      const uint64_t v = elem(e);
+      for (int j = 0; j < ts; j++) o[(int64_t)e * ts + j] = (uint8_t)(v >> (8 * j));
+    }
+    return;
+  }
+  const bool shuffled = t.filter == IAMX_FILTER_SHUFFLE && ts > 1;
+  for (int32_t e = threadIdx.x; e < n; e += blockDim.x) {
+    const uint64_t v = elem(e);
+    if (shuffled) {
+      for (int j = 0; j < ts; j++) o[(int64_t)j * n + e] = (uint8_t)(v >> (8 * j));
+    } else {
+      for (int j = 0; j < ts; j++) o[(int64_t)e * ts + j] = (uint8_t)(v >> (8 * j));
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_transpose(int64_t rows, int64_t cols, const T *__restrict__ src,
+                                                   T *__restrict__ dst) {
+  __shared__ T tile[32][33];
+  const int64_t r0 = (int64_t)blockIdx.y * 32, c0 = (int64_t)blockIdx.x * 32;
+  const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;  // 32 x 8
+  for (int i = ty; i < 32; i += 8) {
+    const int64_t r = r0 + i, c = c0 + tx;
+    if (r < rows && c < cols) tile[i][tx] = src[r * cols + c];
+  }
+  __syncthreads();
+  for (int i = ty; i < 32; i += 8) {
+    const int64_t c = c0 + i, r = r0 + tx;  // dst is cols x rows
+    if (r < rows && c < cols) dst[c * rows + r] = tile[tx][i];
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+IAMX_API int iamx_decode(const iamx_lz4_job_t *d_lz4, int n_lz4, const iamx_block_job_t *d_blocks, int n_blocks,
+                         void *d_scratch, void *d_dst) {
+  int rc = iamx::ensure_init();
+  if (rc) return rc;
+  if (n_lz4 < 0 || n_blocks < 0) return iamx::set_error(IAMX_EINVAL, "iamx_decode: negative job count");
+  if (n_lz4 > 0)
+    IAMX_LAUNCH("lz4_streams", k_lz4_streams, dim3(n_lz4), dim3(64), 0, d_lz4, n_lz4, (uint8_t *)d_scratch,
+                iamx::status_word());
+  if (n_blocks > 0)
+    IAMX_LAUNCH("unfilter", k_unfilter, dim3(n_blocks), dim3(256), 0, d_blocks, n_blocks,
+                (const uint8_t *)d_scratch, (uint8_t *)d_dst);
+  return IAMX_OK;
+}
+
+IAMX_API int iamx_pack_blocks(const iamx_tiling_t *t, const void *d_src, int64_t src_ld, void *d_out) {
+  int rc = iamx::ensure_init();
+  if (rc) return rc;
+  if (!t || t->typesize < 1 || t->typesize > 8)
+    return iamx::set_error(IAMX_EINVAL, "iamx_pack_blocks: bad tiling");
+  const int64_t nblk = t->nchunks[0] * t->nchunks[1] * t->nblocks[0] * t->nblocks[1];
+  if (nblk <= 0 || nblk > INT32_MAX) return iamx::set_error(IAMX_EINVAL, "iamx_pack_blocks: bad block count");
+  IAMX_LAUNCH("pack_blocks", k_pack, dim3((unsigned)nblk), dim3(256), 0, *t, (const uint8_t *)d_src, src_ld,
+              (uint8_t *)d_out);
+  return IAMX_OK;
+}
+
+IAMX_API int iamx_transpose(int typesize, int64_t rows, int64_t cols, const void *d_src, void *d_dst) {
+  int rc = iamx::ensure_init();
+  if (rc) return rc;
+  dim3 grid((unsigned)((cols + 31) / 32), (unsigned)((rows + 31) / 32));
+  if (typesize == 8)
+    IAMX_LAUNCH("transpose", k_transpose<uint64_t>, grid, dim3(256), 0, rows, cols, (const uint64_t *)d_src,
+                (uint64_t *)d_dst);
+  else if (typesize == 4)
+    IAMX_LAUNCH("transpose", k_transpose<uint32_t>, grid, dim3(256), 0, rows, cols, (const uint32_t *)d_src,
+                (uint32_t *)d_dst);
+  else
+    return iamx::set_error(IAMX_EINVAL, "iamx_transpose: typesize %d", typesize);
+  return IAMX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,189 @@
+// iamx_gemm.hip -- plain FP64 / FP32 GEMM on CDNA4 matrix cores, and GEMV.
+//
+// GEMM: C(MxN) = A(MxK) B(KxN), row-major, any M/N/K (edges predicated).
+// 256 threads = 4 wavefronts in a 2x2 arrangement; each wavefront owns a
+// 64x64 sub-tile made of 4x4 MFMA 16x16 tiles.  Operand tiles are staged in
+// LDS k-major (A transposed on the way in) so that every MFMA operand load is
+// 16 consecutive elements per k; the next K-slab is fetched into registers
+// while the current one feeds the matrix cores (one barrier per slab).
+//   float64: v_mfma_f64_16x16x4_f64   float32: v_mfma_f32_16x16x4_f32
+// MFMA operand layout (16x16x4, one block): lane l supplies A[l%16][l/16] and
+// B[l/16][l%16]; it receives D[4*(l/16)+v][l%16], v = 0..3.
+#include "iamx_internal.h"
+
+namespace {
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  typedef f64x4 acc_t;
+  static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mfma<float> {
+  typedef f32x4 acc_t;
+  static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+constexpr int BM = 128, BN = 128, BK = 16, PAD = 4;
+
+// Map the launch index onto output tiles so that the 8 XCDs (workgroups are
+// dealt round-robin to XCDs) each get a contiguous band of tile rows: tiles
+// that share A rows then share one L2.
+__device__ __forceinline__ void tile_coords(int &tm, int &tn, int tiles_m, int tiles_n) {
+  const int nt = tiles_m * tiles_n;
+  int id = blockIdx.x;
+  if (nt % 8 == 0) {
+    const int per = nt / 8;
+    id = (id % 8) * per + id / 8;
+  }
+  tm = id / tiles_n;
+  tn = id % tiles_n;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm(int64_t M, int64_t N, int64_t K, const T *__restrict__ A, int64_t lda,
+                                              const T *__restrict__ B, int64_t ldb, T *__restrict__ C, int64_t ldc,
+                                              int tiles_m, int tiles_n) {
+  __shared__ T As[2][BK][BM + PAD];
+  __shared__ T Bs[2][BK][BN + PAD];
+  typedef typename Mfma<T>::acc_t acc_t;
+  int tm, tn;
+  tile_coords(tm, tn, tiles_m, tiles_n);
+  const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+
+  // global->LDS mapping: A: row tid/2, k (tid%2)*8..+8 ; B: k tid/16, col (tid%16)*8..+8
+  const int a_r = tid >> 1, a_k = (tid & 1) * 8;
+  const int b_k = tid >> 4, b_c = (tid & 15) * 8;
+  T ra[8], rb[8];
+  auto gload = [&](int64_t k0) {
+    const int64_t gr = m0 + a_r;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const int64_t gk = k0 + a_k + i;
+      ra[i] = (gr < M && gk < K) ? A[gr * lda + gk] : (T)0;
+    }
+    const int64_t gk = k0 + b_k;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const int64_t gc = n0 + b_c + i;
+      rb[i] = (gk < K && gc < N) ? B[gk * ldb + gc] : (T)0;
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) As[buf][a_k + i][a_r] = ra[i];
+#pragma unroll
+    for (int i = 0; i < 8; i++) Bs[buf][b_k][b_c + i] = rb[i];
+  };
+
+  acc_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = acc_t{0, 0, 0, 0};
+
+  const int64_t nk = (K + BK - 1) / BK;
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  for (int64_t kt = 0; kt < nk; kt++) {
+    const int cur = (int)(kt & 1);
+    if (kt + 1 < nk) gload((kt + 1) * BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      T a[4], b[4];
+      const int k = kk + (lane >> 4);
+#pragma unroll
+      for (int i = 0; i < 4; i++) a[i] = As[cur][k][wr * 64 + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < 4; j++) b[j] = Bs[cur][k][wc * 64 + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = Mfma<T>::mma(a[i], b[j], acc[i][j]);
+    }
+    if (kt + 1 < nk) sstore(cur ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int64_t col = n0 + wc * 64 + j * 16 + (lane & 15);
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int64_t row = m0 + wr * 64 + i * 16 + 4 * (lane >> 4) + v;
+        if (row < M && col < N) C[row * ldc + col] = acc[i][j][v];
+      }
+    }
+}
+
+// y = A x : one wavefront per row, lanes stride over k, tree reduction.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemv(int64_t M, int64_t K, const T *__restrict__ A, int64_t lda,
+                                              const T *__restrict__ x, T *__restrict__ y) {
+  const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= M) return;
+  const T *a = A + row * lda;
+  T s = 0;
+  for (int64_t k = lane; k < K; k += 64) s += a[k] * x[k];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (lane == 0) y[row] = s;
+}
+
+}  // namespace
+
+extern "C" {
+
+IAMX_API int iamx_gemm(int dtype, int precision, int slices, int64_t M, int64_t N, int64_t K, const void *A,
+                       int64_t lda, const void *B, int64_t ldb, void *C, int64_t ldc) {
+  int rc = iamx::ensure_init();
+  if (rc) return rc;
+  (void)slices;
+  if (M <= 0 || N <= 0 || K < 0) return iamx::set_error(IAMX_EINVAL, "iamx_gemm: bad shape %lld x %lld x %lld",
+                                                        (long long)M, (long long)N, (long long)K);
+  if (precision != 0) return iamx::set_error(IAMX_EINVAL, "iamx_gemm: precision %d not available", precision);
+  const int64_t tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
+  if (tiles_m * tiles_n > INT32_MAX) return iamx::set_error(IAMX_EINVAL, "iamx_gemm: too many tiles");
+  dim3 grid((unsigned)(tiles_m * tiles_n));
+  if (dtype == 0)
+    IAMX_LAUNCH("gemm_f64_mfma", k_gemm<double>, grid, dim3(256), 0, M, N, K, (const double *)A, lda,
+                (const double *)B, ldb, (double *)C, ldc, (int)tiles_m, (int)tiles_n);
+  else if (dtype == 1)
+    IAMX_LAUNCH("gemm_f32_mfma", k_gemm<float>, grid, dim3(256), 0, M, N, K, (const float *)A, lda,
+                (const float *)B, ldb, (float *)C, ldc, (int)tiles_m, (int)tiles_n);
+  else
+    return iamx::set_error(IAMX_EINVAL, "iamx_gemm: dtype %d", dtype);
+  return IAMX_OK;
+}
+
+IAMX_API int iamx_gemv(int dtype, int64_t M, int64_t K, const void *A, int64_t lda, const void *x, void *y) {
+  int rc = iamx::ensure_init();
+  if (rc) return rc;
+  if (M <= 0 || K < 0) return iamx::set_error(IAMX_EINVAL, "iamx_gemv: bad shape");
+  dim3 grid((unsigned)((M + 3) / 4));
+  if (dtype == 0)
+    IAMX_LAUNCH("gemv_f64", k_gemv<double>, grid, dim3(256), 0, M, K, (const double *)A, lda, (const double *)x,
+                (double *)y);
+  else if (dtype == 1)
+    IAMX_LAUNCH("gemv_f32", k_gemv<float>, grid, dim3(256), 0, M, K, (const float *)A, lda, (const float *)x,
+                (float *)y);
+  else
+    return iamx::set_error(IAMX_EINVAL, "iamx_gemv: dtype %d", dtype);
+  return IAMX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,44 @@
+// iamx_internal.h -- shared pieces of the HIP translation units of libiamx.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../../include/iamx.h"
+
+namespace iamx {
+
+extern hipStream_t g_stream;
+extern int g_num_cus;
+
+int set_error(int code, const char *fmt, ...);
+int check_hip(hipError_t e, const char *what);
+int ensure_init();
+
+// Device word the decode kernels OR their error bits into.
+int *status_word();
+
+// Profiling: brackets one launch with HIP events when enabled.
+struct ProfScope {
+  int slot = -1;
+  explicit ProfScope(const char *name);
+  ~ProfScope();
+};
+
+}  // namespace iamx
+
+#define IAMX_HIP_CHECK(expr)                                   \
+  do {                                                         \
+    hipError_t _e = (expr);                                    \
+    if (_e != hipSuccess) return iamx::check_hip(_e, #expr);   \
+  } while (0)
+
+// Launch `kernel` on the library stream inside a profiling scope and check
+// the launch status.
+#define IAMX_LAUNCH(name, kernel, grid, block, shmem, ...)                          \
+  do {                                                                              \
+    iamx::ProfScope _ps(name);                                                      \
+    hipLaunchKernelGGL(kernel, grid, block, shmem, iamx::g_stream, __VA_ARGS__);    \
+    hipError_t _e = hipGetLastError();                                              \
+    if (_e != hipSuccess) return iamx::check_hip(_e, name);                         \
+  } while (0)

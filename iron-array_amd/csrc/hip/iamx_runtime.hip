@@ -1,0 +1,280 @@
+// iamx_runtime.hip -- device selection, memory, workspaces, profiling.
+//
+// One process drives one GPU (device 0 of what HIP exposes).  All kernels run
+// on a single non-blocking stream created at default priority; synchronous
+// copies are stream-ordered copies followed by a stream sync.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "iamx_internal.h"
+
+namespace iamx {
+
+hipStream_t g_stream = nullptr;
+int g_num_cus = 0;
+static int g_init_state = 0;  // 0 not tried, 1 ok, <0 failed
+static char g_err[512] = "";
+static std::mutex g_mu;
+static int *g_status = nullptr;
+
+int set_error(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+int check_hip(hipError_t e, const char *what) {
+  return set_error(IAMX_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+int ensure_init() {
+  if (g_init_state == 1) return IAMX_OK;
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (g_init_state == 1) return IAMX_OK;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n < 1) {
+    g_init_state = -1;
+    return set_error(IAMX_ENODEV, "no HIP device available (%s)",
+                     e != hipSuccess ? hipGetErrorString(e) : "0 devices");
+  }
+  if ((e = hipSetDevice(0)) != hipSuccess) return check_hip(e, "hipSetDevice");
+  hipDeviceProp_t p;
+  if ((e = hipGetDeviceProperties(&p, 0)) != hipSuccess) return check_hip(e, "hipGetDeviceProperties");
+  g_num_cus = p.multiProcessorCount;
+  if ((e = hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking)) != hipSuccess)
+    return check_hip(e, "hipStreamCreateWithFlags");
+  if ((e = hipMalloc(&g_status, 256)) != hipSuccess) return check_hip(e, "hipMalloc(status)");
+  if ((e = hipMemset(g_status, 0, 256)) != hipSuccess) return check_hip(e, "hipMemset(status)");
+  if ((e = hipDeviceSynchronize()) != hipSuccess) return check_hip(e, "hipDeviceSynchronize");
+  g_init_state = 1;
+  return IAMX_OK;
+}
+
+int *status_word() { return g_status; }
+
+// ---- profiling -------------------------------------------------------------------
+struct ProfEntry {
+  std::string name;
+  int64_t launches = 0;
+  double total_ms = 0.0;
+};
+struct Pending {
+  int entry;
+  hipEvent_t a, b;
+};
+static bool g_prof_on = false;
+static std::vector<ProfEntry> g_prof;
+static std::vector<Pending> g_pending;
+static std::vector<hipEvent_t> g_event_pool;
+static hipEvent_t g_first = nullptr, g_last = nullptr;
+static double g_span_ms = 0.0;
+
+static hipEvent_t get_event() {
+  if (!g_event_pool.empty()) {
+    hipEvent_t e = g_event_pool.back();
+    g_event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+ProfScope::ProfScope(const char *name) {
+  if (!g_prof_on) return;
+  int idx = -1;
+  for (size_t i = 0; i < g_prof.size(); i++)
+    if (g_prof[i].name == name) idx = (int)i;
+  if (idx < 0) {
+    g_prof.push_back(ProfEntry{name});
+    idx = (int)g_prof.size() - 1;
+  }
+  Pending p{idx, get_event(), get_event()};
+  if (!p.a || !p.b) return;
+  (void)hipEventRecord(p.a, g_stream);
+  if (!g_first) g_first = p.a;
+  g_pending.push_back(p);
+  slot = (int)g_pending.size() - 1;
+}
+
+ProfScope::~ProfScope() {
+  if (slot < 0) return;
+  (void)hipEventRecord(g_pending[slot].b, g_stream);
+  g_last = g_pending[slot].b;
+}
+
+static void prof_collect() {
+  if (g_pending.empty()) return;
+  (void)hipStreamSynchronize(g_stream);
+  for (auto &p : g_pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      g_prof[p.entry].launches++;
+      g_prof[p.entry].total_ms += ms;
+    }
+  }
+  if (g_first && g_last) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, g_first, g_last) == hipSuccess) g_span_ms += ms;
+  }
+  for (auto &p : g_pending) {
+    g_event_pool.push_back(p.a);
+    g_event_pool.push_back(p.b);
+  }
+  g_pending.clear();
+  g_first = g_last = nullptr;
+}
+
+// ---- workspaces ---------------------------------------------------------------------
+struct Workspace {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+static Workspace g_ws[8];
+
+}  // namespace iamx
+
+using namespace iamx;
+
+extern "C" {
+
+IAMX_API int iamx_init(void) { return ensure_init(); }
+
+IAMX_API const char *iamx_last_error(void) { return g_err; }
+
+IAMX_API int iamx_device_info(char *name, int namelen, int *cus, char *arch, int archlen) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  hipDeviceProp_t p;
+  IAMX_HIP_CHECK(hipGetDeviceProperties(&p, 0));
+  if (name && namelen > 0) snprintf(name, (size_t)namelen, "%s", p.name);
+  if (arch && archlen > 0) snprintf(arch, (size_t)archlen, "%s", p.gcnArchName);
+  if (cus) *cus = p.multiProcessorCount;
+  return IAMX_OK;
+}
+
+IAMX_API void *iamx_malloc(size_t bytes) {
+  if (ensure_init()) return nullptr;
+  void *p = nullptr;
+  if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) {
+    set_error(IAMX_ENOMEM, "hipMalloc(%zu) failed", bytes);
+    return nullptr;
+  }
+  return p;
+}
+
+IAMX_API void iamx_free(void *p) {
+  if (p && g_init_state == 1) (void)hipFree(p);
+}
+
+IAMX_API int iamx_h2d(void *dst, const void *src, size_t bytes) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (!bytes) return IAMX_OK;
+  IAMX_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream));
+  IAMX_HIP_CHECK(hipStreamSynchronize(g_stream));
+  return IAMX_OK;
+}
+
+IAMX_API int iamx_d2h(void *dst, const void *src, size_t bytes) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (!bytes) return IAMX_OK;
+  IAMX_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
+  IAMX_HIP_CHECK(hipStreamSynchronize(g_stream));
+  return IAMX_OK;
+}
+
+IAMX_API int iamx_d2d(void *dst, const void *src, size_t bytes) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (!bytes) return IAMX_OK;
+  IAMX_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_stream));
+  return IAMX_OK;
+}
+
+IAMX_API int iamx_memset(void *dst, int value, size_t bytes) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (!bytes) return IAMX_OK;
+  IAMX_HIP_CHECK(hipMemsetAsync(dst, value, bytes, g_stream));
+  return IAMX_OK;
+}
+
+IAMX_API int iamx_sync(void) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  IAMX_HIP_CHECK(hipStreamSynchronize(g_stream));
+  return IAMX_OK;
+}
+
+IAMX_API void *iamx_workspace(int slot, size_t bytes) {
+  if (slot < 0 || slot >= 8 || ensure_init()) return nullptr;
+  Workspace &w = g_ws[slot];
+  if (w.bytes >= bytes && w.ptr) return w.ptr;
+  if (w.ptr) {
+    (void)hipStreamSynchronize(g_stream);
+    (void)hipFree(w.ptr);
+    w.ptr = nullptr;
+    w.bytes = 0;
+  }
+  size_t want = bytes + bytes / 8 + 4096;
+  if (hipMalloc(&w.ptr, want) != hipSuccess) {
+    w.ptr = nullptr;
+    set_error(IAMX_ENOMEM, "workspace %d: hipMalloc(%zu) failed", slot, want);
+    return nullptr;
+  }
+  w.bytes = want;
+  return w.ptr;
+}
+
+IAMX_API void iamx_shutdown(void) {
+  if (g_init_state != 1) return;
+  (void)hipStreamSynchronize(g_stream);
+  for (auto &w : g_ws) {
+    if (w.ptr) (void)hipFree(w.ptr);
+    w = Workspace{};
+  }
+}
+
+IAMX_API void iamx_prof_enable(int on) { g_prof_on = on != 0; }
+
+IAMX_API void iamx_prof_reset(void) {
+  prof_collect();
+  g_prof.clear();
+  g_span_ms = 0.0;
+}
+
+IAMX_API int iamx_prof_entry(int i, const char **name, int64_t *launches, double *total_ms) {
+  prof_collect();
+  if (i < 0 || i >= (int)g_prof.size()) return 0;
+  if (name) *name = g_prof[i].name.c_str();
+  if (launches) *launches = g_prof[i].launches;
+  if (total_ms) *total_ms = g_prof[i].total_ms;
+  return 1;
+}
+
+IAMX_API double iamx_prof_span_ms(void) {
+  prof_collect();
+  return g_span_ms;
+}
+
+IAMX_API int iamx_decode_status(void) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  int v = 0;
+  IAMX_HIP_CHECK(hipMemcpyAsync(&v, g_status, sizeof(int), hipMemcpyDeviceToHost, g_stream));
+  IAMX_HIP_CHECK(hipStreamSynchronize(g_stream));
+  if (v) IAMX_HIP_CHECK(hipMemsetAsync(g_status, 0, sizeof(int), g_stream));
+  return v;
+}
+
+}  // extern "C"

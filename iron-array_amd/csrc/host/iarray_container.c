@@ -1,0 +1,895 @@
+/*
+ * iarray_container.c -- containers of libiarray.so: Caterva-style two-level
+ * tiling, constructors, read-back, views, and the device mirror + decode plan
+ * the GPU kernels consume.
+ *
+ * Tiling (restated from the reference's index math, src/iarray_zproxy.c:46-85,
+ * src/linalg/matmul/gemm.c:25-69): extshape = ceil(shape/chunk)*chunk,
+ * extchunkshape = ceil(chunk/block)*block; chunks are numbered row-major over
+ * extshape/chunkshape, blocks row-major inside a chunk over
+ * extchunkshape/blockshape, each block is a row-major blockshape tile and the
+ * chunk's uncompressed content is its blocks back to back
+ * (chunk bytes = prod(extchunkshape)*itemsize, blocksize =
+ * prod(blockshape)*itemsize).  Element (c_nd*chunkshape + b_nd*blockshape + i)
+ * is valid when it lies inside both the chunk's chunkshape and the array
+ * shape; everything else is padding, written as zeros here and never read.
+ *
+ * Entry points mirror src/iarray_constructor.c:16-305 (empty, from_buffer,
+ * to_buffer), src/constructors/iarray_constructor_arange.c:40 and
+ * src/constructors/iarray_constructor_linspace.c:117 (value = start +
+ * flat_index * step, step = (stop-start)/(n-1) for linspace),
+ * src/iarray_container.c:500-541 / 628-736 (get_slice_buffer, transposed
+ * views), src/iarray_container.c:1200-1287 (info, almost_equal) and
+ * src/linalg/transpose.c:18-45.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "iarray_private.h"
+
+#define FAIL_NULL(p) \
+  do {               \
+    if (!(p)) return iax_set_error(INA_ERROR(INA_ERR_NULL_POINTER), "null argument: %s", #p); \
+  } while (0)
+
+static inline int32_t rd32(const uint8_t *p) {
+  int32_t v;
+  memcpy(&v, p, 4);
+  return v;
+}
+
+int64_t iax_dtype_size(iarray_data_type_t t) {
+  switch (t) {
+    case IARRAY_DATA_TYPE_DOUBLE: case IARRAY_DATA_TYPE_INT64: case IARRAY_DATA_TYPE_UINT64: return 8;
+    case IARRAY_DATA_TYPE_FLOAT: case IARRAY_DATA_TYPE_INT32: case IARRAY_DATA_TYPE_UINT32: return 4;
+    case IARRAY_DATA_TYPE_INT16: case IARRAY_DATA_TYPE_UINT16: return 2;
+    case IARRAY_DATA_TYPE_INT8: case IARRAY_DATA_TYPE_UINT8: case IARRAY_DATA_TYPE_BOOL: return 1;
+    default: return -1;
+  }
+}
+
+static int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+static ina_rc_t check_config(const iarray_config_t *cfg) {
+  if (cfg->compression_codec != IARRAY_COMPRESSION_LZ4 && cfg->compression_codec != IARRAY_COMPRESSION_LZ4HC)
+    return iax_set_error(INA_ERROR(INA_ERR_NOT_SUPPORTED | IARRAY_ES_BLOSC),
+                         "only the LZ4 codecs are on the GPU decode path (codec %d)", cfg->compression_codec);
+  if (cfg->filter_flags & (IARRAY_COMP_DELTA | IARRAY_COMP_TRUNC_PREC))
+    return iax_set_error(INA_ERROR(INA_ERR_NOT_SUPPORTED | IARRAY_ES_BLOSC), "DELTA/TRUNC_PREC filters unsupported");
+  if (cfg->compression_level < 0 || cfg->compression_level > 9)
+    return iax_set_error(INA_ERROR(INA_ERR_INVALID_ARGUMENT | IARRAY_ES_BLOSC), "clevel %d", cfg->compression_level);
+  return INA_SUCCESS;
+}
+
+ina_rc_t iax_container_new(iarray_context_t *ctx, iarray_dtshape_t *dtshape, iarray_storage_t *storage,
+                           iarray_container_t **cp) {
+  FAIL_NULL(ctx);
+  FAIL_NULL(dtshape);
+  FAIL_NULL(storage);
+  FAIL_NULL(cp);
+  if (dtshape->ndim < 1 || dtshape->ndim > 2)
+    return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_NDIM), "ndim %d: the matmul path handles 1-D and 2-D arrays",
+                         dtshape->ndim);
+  const int64_t is = iax_dtype_size(dtshape->dtype);
+  if (is < 0) return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_DTYPE), "dtype %d", dtshape->dtype);
+  if (storage->urlpath)
+    return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_STORAGE), "on-disk frames (urlpath) are not supported");
+  ina_rc_t rc = check_config(&ctx->cfg);
+  if (INA_FAILED(rc)) return rc;
+  for (int i = 0; i < dtshape->ndim; i++) {
+    if (dtshape->shape[i] < 0) return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_SHAPE), "negative shape");
+    if (storage->chunkshape[i] < 1)
+      return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_CHUNKSHAPE), "chunkshape[%d] = %lld", i,
+                           (long long)storage->chunkshape[i]);
+    if (storage->blockshape[i] < 1 || storage->blockshape[i] > storage->chunkshape[i])
+      return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_BLOCKSHAPE), "blockshape[%d] = %lld", i,
+                           (long long)storage->blockshape[i]);
+  }
+  iarray_container_t *c = (iarray_container_t *)calloc(1, sizeof(*c));
+  if (!c) return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+  c->dtshape = *dtshape;
+  c->dtshape.dtype_size = (int32_t)is;
+  c->storage = *storage;
+  c->storage.urlpath = NULL;
+  c->cfg = ctx->cfg;
+  c->itemsize = (int)is;
+  c->ndim = dtshape->ndim;
+  for (int i = 0; i < 2; i++) {
+    const int one = (c->ndim == 1 && i == 0);
+    const int s = c->ndim == 2 ? i : 0;
+    c->shape[i] = one ? 1 : dtshape->shape[s];
+    c->chunk[i] = one ? 1 : storage->chunkshape[s];
+    c->block[i] = one ? 1 : storage->blockshape[s];
+    c->extshape[i] = cdiv(c->shape[i] > 0 ? c->shape[i] : 1, c->chunk[i]) * c->chunk[i];
+    c->extchunk[i] = cdiv(c->chunk[i], c->block[i]) * c->block[i];
+    c->nchunks_ax[i] = c->extshape[i] / c->chunk[i];
+    c->nblocks_ax[i] = c->extchunk[i] / c->block[i];
+  }
+  c->nchunks = c->nchunks_ax[0] * c->nchunks_ax[1];
+  c->chunk_nbytes = c->extchunk[0] * c->extchunk[1] * is;
+  const int64_t bsz = c->block[0] * c->block[1] * is;
+  if (c->chunk_nbytes > INT32_MAX || bsz > INT32_MAX)
+    return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_CHUNKSHAPE), "chunks are limited to 2 GB (Blosc2 int32 sizes)");
+  c->blocksize = (int32_t)bsz;
+  c->chunks = (uint8_t **)calloc((size_t)c->nchunks, sizeof(uint8_t *));
+  c->csizes = (int32_t *)calloc((size_t)c->nchunks, sizeof(int32_t));
+  c->arena_off = (int64_t *)calloc((size_t)c->nchunks, sizeof(int64_t));
+  if (!c->chunks || !c->csizes || !c->arena_off) {
+    free(c->chunks); free(c->csizes); free(c->arena_off); free(c);
+    return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+  }
+  *cp = c;
+  return INA_SUCCESS;
+}
+
+static void free_plan(iax_plan_t *p) {
+  iamx_free(p->d_lz4);
+  iamx_free(p->d_blocks);
+  iamx_free(p->d_blk_rc);
+  memset(p, 0, sizeof(*p));
+}
+
+static void drop_device(iarray_container_t *c) {
+  if (c->plan_valid) free_plan(&c->plan);
+  c->plan_valid = 0;
+  iamx_free(c->d_arena);
+  c->d_arena = NULL;
+  c->device_valid = 0;
+}
+
+INA_API(void) iarray_container_free(iarray_context_t *ctx, iarray_container_t **cp) {
+  (void)ctx;
+  if (!cp || !*cp) return;
+  iarray_container_t *c = *cp;
+  if (!c->viewed) {
+    for (int64_t i = 0; i < c->nchunks; i++) free(c->chunks[i]);
+    drop_device(c);
+    free(c->chunks);
+    free(c->csizes);
+    free(c->arena_off);
+  }
+  free(c);
+  *cp = NULL;
+}
+
+INA_API(ina_rc_t) iarray_empty(iarray_context_t *ctx, iarray_dtshape_t *dtshape, iarray_storage_t *storage,
+                               iarray_container_t **container) {
+  return iax_container_new(ctx, dtshape, storage, container);
+}
+
+/* a special-value chunk: 32-byte header, value kind in bits 4..6 of byte 31 */
+static uint8_t *zero_chunk(const iarray_container_t *c, int32_t *csize) {
+  uint8_t *h = (uint8_t *)calloc(1, 32);
+  if (!h) return NULL;
+  const int32_t nb = (int32_t)c->chunk_nbytes, bs = c->blocksize, cb = 32;
+  h[0] = 5; h[1] = 1; h[2] = 0x05 | 0x10 | (1 << 5); h[3] = (uint8_t)c->itemsize;
+  memcpy(h + 4, &nb, 4); memcpy(h + 8, &bs, 4); memcpy(h + 12, &cb, 4);
+  h[31] = 1 << 4;
+  *csize = 32;
+  return h;
+}
+
+INA_API(ina_rc_t) iarray_zeros(iarray_context_t *ctx, iarray_dtshape_t *dtshape, iarray_storage_t *storage,
+                               iarray_container_t **container) {
+  ina_rc_t rc = iax_container_new(ctx, dtshape, storage, container);
+  if (INA_FAILED(rc)) return rc;
+  iarray_container_t *c = *container;
+  for (int64_t i = 0; i < c->nchunks; i++)
+    if (!(c->chunks[i] = zero_chunk(c, &c->csizes[i]))) return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+  return INA_SUCCESS;
+}
+
+/* ---- host encode of a row-major buffer --------------------------------------------------- */
+typedef struct {
+  iarray_container_t *c;
+  const uint8_t *buf;
+  uint8_t **scratch;
+  int err;
+} enc_ctx_t;
+
+static void encode_chunk_task(void *vctx, int64_t nc, int worker) {
+  enc_ctx_t *x = (enc_ctx_t *)vctx;
+  iarray_container_t *c = x->c;
+  const int is = c->itemsize;
+  uint8_t *chunk = x->scratch[worker];
+  memset(chunk, 0, (size_t)c->chunk_nbytes);
+  const int64_t ci0 = nc / c->nchunks_ax[1], ci1 = nc % c->nchunks_ax[1];
+  for (int64_t bi0 = 0; bi0 < c->nblocks_ax[0]; bi0++)
+    for (int64_t bi1 = 0; bi1 < c->nblocks_ax[1]; bi1++) {
+      uint8_t *blk = chunk + (bi0 * c->nblocks_ax[1] + bi1) * (int64_t)c->blocksize;
+      const int64_t r_in = bi0 * c->block[0], c_in = bi1 * c->block[1];
+      const int64_t row0 = ci0 * c->chunk[0] + r_in, col0 = ci1 * c->chunk[1] + c_in;
+      int64_t vr = c->block[0], vc = c->block[1];
+      if (c->chunk[0] - r_in < vr) vr = c->chunk[0] - r_in;
+      if (c->shape[0] - row0 < vr) vr = c->shape[0] - row0;
+      if (c->chunk[1] - c_in < vc) vc = c->chunk[1] - c_in;
+      if (c->shape[1] - col0 < vc) vc = c->shape[1] - col0;
+      if (vr <= 0 || vc <= 0) continue; /* all-padding block: stays zero */
+      for (int64_t r = 0; r < vr; r++)
+        memcpy(blk + r * c->block[1] * is, x->buf + ((row0 + r) * c->shape[1] + col0) * is, (size_t)(vc * is));
+    }
+  if (c->chunks[nc]) free(c->chunks[nc]);
+  c->chunks[nc] = NULL;
+  if (iax_encode_chunk(chunk, (int32_t)c->chunk_nbytes, c->blocksize, is, &c->cfg, 0, &c->chunks[nc],
+                       &c->csizes[nc]) != 0)
+    x->err = 1;
+}
+
+static ina_rc_t encode_buffer(iarray_container_t *c, const void *buf) {
+  const int nt = iax_nthreads(&c->cfg);
+  enc_ctx_t x = {c, (const uint8_t *)buf, NULL, 0};
+  x.scratch = (uint8_t **)calloc((size_t)nt, sizeof(uint8_t *));
+  if (!x.scratch) return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+  for (int i = 0; i < nt; i++)
+    if (!(x.scratch[i] = (uint8_t *)malloc((size_t)c->chunk_nbytes))) x.err = 1;
+  if (!x.err) iax_parallel_for(nt, c->nchunks, encode_chunk_task, &x);
+  for (int i = 0; i < nt; i++) free(x.scratch[i]);
+  free(x.scratch);
+  drop_device(c);
+  if (x.err) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "chunk encode failed");
+  return INA_SUCCESS;
+}
+
+INA_API(ina_rc_t) iarray_from_buffer(iarray_context_t *ctx, iarray_dtshape_t *dtshape, void *buffer, int64_t buflen,
+                                     iarray_storage_t *storage, iarray_container_t **container) {
+  FAIL_NULL(buffer);
+  ina_rc_t rc = iax_container_new(ctx, dtshape, storage, container);
+  if (INA_FAILED(rc)) return rc;
+  iarray_container_t *c = *container;
+  if (c->shape[0] * c->shape[1] * c->itemsize > buflen)
+    return iax_set_error(INA_ERROR(IARRAY_ERR_TOO_SMALL_BUFFER), "The size of the buffer is not enough");
+  return encode_buffer(c, buffer);
+}
+
+static ina_rc_t from_ramp(iarray_context_t *ctx, iarray_dtshape_t *dtshape, double start, double step,
+                          iarray_storage_t *storage, iarray_container_t **container) {
+  int64_t n = 1;
+  for (int i = 0; i < dtshape->ndim; i++) n *= dtshape->shape[i];
+  const int64_t is = iax_dtype_size(dtshape->dtype);
+  if (is < 0) return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_DTYPE), "dtype %d", dtshape->dtype);
+  void *buf = malloc((size_t)(n > 0 ? n : 1) * (size_t)is);
+  if (!buf) return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+  for (int64_t i = 0; i < n; i++) {
+    const double v = start + (double)i * step;
+    switch (dtshape->dtype) {
+      case IARRAY_DATA_TYPE_DOUBLE: ((double *)buf)[i] = v; break;
+      case IARRAY_DATA_TYPE_FLOAT: ((float *)buf)[i] = (float)v; break;
+      case IARRAY_DATA_TYPE_INT64: ((int64_t *)buf)[i] = (int64_t)v; break;
+      case IARRAY_DATA_TYPE_INT32: ((int32_t *)buf)[i] = (int32_t)v; break;
+      case IARRAY_DATA_TYPE_INT16: ((int16_t *)buf)[i] = (int16_t)v; break;
+      case IARRAY_DATA_TYPE_INT8: ((int8_t *)buf)[i] = (int8_t)v; break;
+      case IARRAY_DATA_TYPE_UINT64: ((uint64_t *)buf)[i] = (uint64_t)v; break;
+      case IARRAY_DATA_TYPE_UINT32: ((uint32_t *)buf)[i] = (uint32_t)v; break;
+      case IARRAY_DATA_TYPE_UINT16: ((uint16_t *)buf)[i] = (uint16_t)v; break;
+      case IARRAY_DATA_TYPE_UINT8: ((uint8_t *)buf)[i] = (uint8_t)v; break;
+      case IARRAY_DATA_TYPE_BOOL: ((uint8_t *)buf)[i] = v != 0.0; break;
+      default: break;
+    }
+  }
+  ina_rc_t rc = iarray_from_buffer(ctx, dtshape, buf, n * is, storage, container);
+  free(buf);
+  return rc;
+}
+
+INA_API(ina_rc_t) iarray_arange(iarray_context_t *ctx, iarray_dtshape_t *dtshape, double start, double step,
+                                iarray_storage_t *storage, iarray_container_t **container) {
+  FAIL_NULL(dtshape);
+  return from_ramp(ctx, dtshape, start, step, storage, container);
+}
+
+INA_API(ina_rc_t) iarray_linspace(iarray_context_t *ctx, iarray_dtshape_t *dtshape, double start, double stop,
+                                  iarray_storage_t *storage, iarray_container_t **container) {
+  FAIL_NULL(dtshape);
+  int64_t n = 1;
+  for (int i = 0; i < dtshape->ndim; i++) n *= dtshape->shape[i];
+  return from_ramp(ctx, dtshape, start, (stop - start) / ((double)n - 1), storage, container);
+}
+
+/* ---- device mirror ------------------------------------------------------------------------ */
+ina_rc_t iax_ensure_device(iarray_container_t *c) {
+  if (c->viewed) return iax_ensure_device(c->viewed);
+  if (c->device_valid) return INA_SUCCESS;
+  int64_t total = 0;
+  for (int64_t i = 0; i < c->nchunks; i++) {
+    c->arena_off[i] = total;
+    total += (c->chunks[i] ? ((c->csizes[i] + 15) & ~15) : 0);
+  }
+  iamx_free(c->d_arena);
+  c->d_arena = (uint8_t *)iamx_malloc((size_t)(total > 0 ? total : 16));
+  if (!c->d_arena) return iax_dev_error(iamx_init() ? iamx_init() : IAMX_ENOMEM, "device arena");
+  uint8_t *host = (uint8_t *)malloc((size_t)(total > 0 ? total : 16));
+  if (!host) return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+  for (int64_t i = 0; i < c->nchunks; i++)
+    if (c->chunks[i]) memcpy(host + c->arena_off[i], c->chunks[i], (size_t)c->csizes[i]);
+  int rc = iamx_h2d(c->d_arena, host, (size_t)total);
+  free(host);
+  if (rc) return iax_dev_error(rc, "upload chunks");
+  c->device_valid = 1;
+  return INA_SUCCESS;
+}
+
+/* ---- decode plan ------------------------------------------------------------------------------ */
+typedef struct {
+  iamx_lz4_job_t *lz4;
+  int64_t n_lz4, cap_lz4;
+  iamx_block_job_t *blk;
+  int32_t *rc4;
+  int64_t n_blk, cap_blk;
+  int64_t scratch;
+} jobs_t;
+
+static int push_lz4(jobs_t *j, iamx_lz4_job_t v) {
+  if (j->n_lz4 == j->cap_lz4) {
+    j->cap_lz4 = j->cap_lz4 ? 2 * j->cap_lz4 : 1024;
+    void *p = realloc(j->lz4, (size_t)j->cap_lz4 * sizeof(v));
+    if (!p) return -1;
+    j->lz4 = (iamx_lz4_job_t *)p;
+  }
+  j->lz4[j->n_lz4++] = v;
+  return 0;
+}
+
+static int push_blk(jobs_t *j, const iamx_block_job_t *v, int32_t row0, int32_t col0, int32_t bcol) {
+  if (j->n_blk == j->cap_blk) {
+    j->cap_blk = j->cap_blk ? 2 * j->cap_blk : 1024;
+    void *p = realloc(j->blk, (size_t)j->cap_blk * sizeof(*v));
+    void *q = realloc(j->rc4, (size_t)j->cap_blk * 4 * sizeof(int32_t));
+    if (!p || !q) return -1;
+    j->blk = (iamx_block_job_t *)p;
+    j->rc4 = (int32_t *)q;
+  }
+  j->rc4[4 * j->n_blk] = row0;
+  j->rc4[4 * j->n_blk + 1] = col0;
+  j->rc4[4 * j->n_blk + 2] = bcol;
+  j->rc4[4 * j->n_blk + 3] = 0;
+  j->blk[j->n_blk++] = *v;
+  return 0;
+}
+
+/* Stream table of block `nb` of one chunk (host copy `ch`, device copy at
+ * `dev`).  Fills the stream fields of *J (typesize/filter/neblock/nstreams/
+ * bsize/src/kind/fill); LZ4 streams get a scratch slot and an LZ4 job. */
+static ina_rc_t block_streams(const uint8_t *ch, int32_t csize, uint64_t dev, const iax_header_t *h, int nb,
+                              int itemsize, jobs_t *jobs, iamx_block_job_t *J) {
+  const int ts = h->typesize;
+  const int leftover = (nb == h->nblocks - 1) && h->leftover;
+  const int32_t bsize = leftover ? h->leftover : h->blocksize;
+  if (ts != itemsize) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "chunk typesize %d != %d", ts, itemsize);
+  if (bsize % ts) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "block size not a multiple of typesize");
+  J->typesize = (int16_t)ts;
+  J->bsize = bsize;
+  if (h->special) {
+    uint8_t val[8] = {0};
+    if (h->special == 2) {
+      if (ts == 4) { const uint32_t v = 0x7fc00000u; memcpy(val, &v, 4); }
+      else if (ts == 8) { const uint64_t v = 0x7ff8000000000000ull; memcpy(val, &v, 8); }
+    } else if (h->special == 3) {
+      if (csize < h->header_len + ts) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "short value chunk");
+      memcpy(val, ch + h->header_len, (size_t)ts);
+    }
+    J->nstreams = (int16_t)ts;
+    J->neblock = bsize / ts;
+    J->filter = ts > 1 ? IAMX_FILTER_SHUFFLE : IAMX_FILTER_NONE;
+    for (int s = 0; s < ts; s++) { J->kind[s] = IAMX_SRC_CONST; J->fill[s] = val[s]; }
+    return INA_SUCCESS;
+  }
+  if (h->memcpyed) {
+    if ((int64_t)h->header_len + (int64_t)nb * h->blocksize + bsize > csize)
+      return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "short memcpyed chunk");
+    J->nstreams = 1;
+    J->neblock = bsize;
+    J->filter = IAMX_FILTER_NONE;
+    J->kind[0] = IAMX_SRC_ADDR;
+    J->src[0] = dev + (uint64_t)h->header_len + (uint64_t)nb * (uint64_t)h->blocksize;
+    return INA_SUCCESS;
+  }
+  /* filters: only (bit)shuffle in any slot; TRUNC_PREC (4) is a no-op on decode */
+  int filter = IAMX_FILTER_NONE;
+  for (int i = 0; i < 6; i++) {
+    const int f = h->filters[i];
+    if (f == 0 || f == 4) continue;
+    if (filter != IAMX_FILTER_NONE) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "stacked filters");
+    if (f == 1 && h->filters_meta[i] == 0) filter = IAMX_FILTER_SHUFFLE;
+    else if (f == 2) filter = h->version >= 3 ? IAMX_FILTER_BITSHUFFLE : IAMX_FILTER_BITSHUFFLE_V2;
+    else return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "unsupported filter %d", f);
+  }
+  if (filter == IAMX_FILTER_BITSHUFFLE_V2 && (bsize / ts) % 8) filter = IAMX_FILTER_NONE;
+  if (h->blosc2_flags & 0x1) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "dictionaries unsupported");
+  const int nstreams = (!h->dont_split && !leftover) ? ts : 1;
+  if (nstreams > IAMX_MAX_STREAMS) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "too many streams");
+  const int32_t neblock = bsize / nstreams;
+  J->nstreams = (int16_t)nstreams;
+  J->neblock = neblock;
+  J->filter = filter;
+  const int32_t bstart = rd32(ch + h->header_len + 4 * nb);
+  if (bstart < h->header_len || bstart >= csize) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "bad bstart");
+  int64_t p = bstart;
+  for (int s = 0; s < nstreams; s++) {
+    if (p + 4 > csize) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "truncated block");
+    const int32_t cs = rd32(ch + p);
+    p += 4;
+    if (cs == 0) {
+      J->kind[s] = IAMX_SRC_CONST;
+      J->fill[s] = 0;
+    } else if (cs < 0) {
+      if (p + 1 > csize || !(ch[p] & 0x1) || cs < -255)
+        return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "bad run stream");
+      J->kind[s] = IAMX_SRC_CONST;
+      J->fill[s] = (uint8_t)(-cs);
+      p += 1;
+    } else if (cs == neblock) {
+      if (p + cs > csize) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "truncated raw stream");
+      J->kind[s] = IAMX_SRC_ADDR;
+      J->src[s] = dev + (uint64_t)p;
+      p += cs;
+    } else {
+      if (p + cs > csize) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "truncated LZ4 stream");
+      if (h->compformat != 1) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "codec format %d", h->compformat);
+      iamx_lz4_job_t lj = {dev + (uint64_t)p, jobs->scratch, cs, neblock};
+      if (push_lz4(jobs, lj)) return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+      J->kind[s] = IAMX_SRC_SCRATCH;
+      J->src[s] = (uint64_t)jobs->scratch;
+      jobs->scratch += (neblock + 15) & ~15;
+      p += cs;
+    }
+  }
+  return INA_SUCCESS;
+}
+
+static ina_rc_t upload_plan(jobs_t *j, iax_plan_t *plan) {
+  memset(plan, 0, sizeof(*plan));
+  plan->n_lz4 = (int)j->n_lz4;
+  plan->n_blocks = (int)j->n_blk;
+  plan->scratch_bytes = j->scratch;
+  if (j->n_lz4) {
+    plan->d_lz4 = (iamx_lz4_job_t *)iamx_malloc((size_t)j->n_lz4 * sizeof(iamx_lz4_job_t));
+    if (!plan->d_lz4) return iax_dev_error(IAMX_ENOMEM, "plan");
+    int rc = iamx_h2d(plan->d_lz4, j->lz4, (size_t)j->n_lz4 * sizeof(iamx_lz4_job_t));
+    if (rc) return iax_dev_error(rc, "plan upload");
+  }
+  if (j->n_blk) {
+    plan->d_blocks = (iamx_block_job_t *)iamx_malloc((size_t)j->n_blk * sizeof(iamx_block_job_t));
+    plan->d_blk_rc = (int32_t *)iamx_malloc((size_t)j->n_blk * 4 * sizeof(int32_t));
+    if (!plan->d_blocks || !plan->d_blk_rc) return iax_dev_error(IAMX_ENOMEM, "plan");
+    int rc = iamx_h2d(plan->d_blocks, j->blk, (size_t)j->n_blk * sizeof(iamx_block_job_t));
+    if (!rc) rc = iamx_h2d(plan->d_blk_rc, j->rc4, (size_t)j->n_blk * 4 * sizeof(int32_t));
+    if (rc) return iax_dev_error(rc, "plan upload");
+  }
+  return INA_SUCCESS;
+}
+
+static void free_jobs(jobs_t *j) {
+  free(j->lz4);
+  free(j->blk);
+  free(j->rc4);
+}
+
+ina_rc_t iax_ensure_plan(iarray_container_t *c) {
+  if (c->viewed) return iax_ensure_plan(c->viewed);
+  if (c->plan_valid) return INA_SUCCESS;
+  ina_rc_t rc = iax_ensure_device(c);
+  if (INA_FAILED(rc)) return rc;
+  jobs_t jobs;
+  memset(&jobs, 0, sizeof(jobs));
+  const int64_t ld = c->shape[1];
+  for (int64_t nc = 0; nc < c->nchunks && INA_SUCCEED(rc); nc++) {
+    const int64_t ci0 = nc / c->nchunks_ax[1], ci1 = nc % c->nchunks_ax[1];
+    iax_header_t h;
+    const uint8_t *ch = c->chunks[nc];
+    if (ch) {
+      if (iax_parse_header(ch, c->csizes[nc], &h) || h.nbytes != c->chunk_nbytes || h.blocksize != c->blocksize ||
+          h.nblocks != c->nblocks_ax[0] * c->nblocks_ax[1]) {
+        rc = iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "chunk %lld does not match the tiling", (long long)nc);
+        break;
+      }
+    }
+    for (int64_t bi0 = 0; bi0 < c->nblocks_ax[0] && INA_SUCCEED(rc); bi0++)
+      for (int64_t bi1 = 0; bi1 < c->nblocks_ax[1] && INA_SUCCEED(rc); bi1++) {
+        const int64_t r_in = bi0 * c->block[0], c_in = bi1 * c->block[1];
+        const int64_t row0 = ci0 * c->chunk[0] + r_in, col0 = ci1 * c->chunk[1] + c_in;
+        int64_t vr = c->block[0], vc = c->block[1];
+        if (c->chunk[0] - r_in < vr) vr = c->chunk[0] - r_in;
+        if (c->shape[0] - row0 < vr) vr = c->shape[0] - row0;
+        if (c->chunk[1] - c_in < vc) vc = c->chunk[1] - c_in;
+        if (c->shape[1] - col0 < vc) vc = c->shape[1] - col0;
+        if (vr <= 0 || vc <= 0) continue; /* an all-padding block holds no data */
+        iamx_block_job_t J;
+        memset(&J, 0, sizeof(J));
+        const int nb = (int)(bi0 * c->nblocks_ax[1] + bi1);
+        if (ch) {
+          rc = block_streams(ch, c->csizes[nc], (uint64_t)(uintptr_t)(c->d_arena + c->arena_off[nc]), &h, nb,
+                             c->itemsize, &jobs, &J);
+          if (INA_FAILED(rc)) break;
+        } else { /* never-written chunk of an empty container reads as zeros */
+          J.typesize = (int16_t)c->itemsize;
+          J.bsize = c->blocksize;
+          J.nstreams = 1;
+          J.neblock = c->blocksize;
+          J.filter = IAMX_FILTER_NONE;
+          J.kind[0] = IAMX_SRC_CONST;
+        }
+        J.rows = (int32_t)c->block[0];
+        J.cols = (int32_t)c->block[1];
+        J.vrows = (int32_t)vr;
+        J.vcols = (int32_t)vc;
+        J.dst_off = row0 * ld + col0;
+        J.dst_ld = ld;
+        if (push_blk(&jobs, &J, (int32_t)row0, (int32_t)col0, (int32_t)(ci1 * c->nblocks_ax[1] + bi1)))
+          rc = INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+      }
+  }
+  if (INA_SUCCEED(rc)) rc = upload_plan(&jobs, &c->plan);
+  c->plan.nbcols = c->nchunks_ax[1] * c->nblocks_ax[1];
+  free_jobs(&jobs);
+  if (INA_FAILED(rc)) {
+    free_plan(&c->plan);
+    return rc;
+  }
+  c->plan_valid = 1;
+  return INA_SUCCESS;
+}
+
+ina_rc_t iax_decode_dense(iarray_container_t *c, void *d_dst, int scratch_slot) {
+  ina_rc_t rc = iax_ensure_plan(c);
+  if (INA_FAILED(rc)) return rc;
+  void *scratch = iamx_workspace(scratch_slot, (size_t)c->plan.scratch_bytes + 16);
+  if (!scratch) return iax_dev_error(IAMX_ENOMEM, "decode scratch");
+  int e = iamx_decode(c->plan.d_lz4, c->plan.n_lz4, c->plan.d_blocks, c->plan.n_blocks, scratch, d_dst);
+  if (e) return iax_dev_error(e, "decode");
+  return INA_SUCCESS;
+}
+
+ina_rc_t iax_operand(iarray_container_t *c, int slot, int tmp_slot, void **d_out) {
+  iarray_container_t *base = c->viewed ? c->viewed : c;
+  const size_t bytes = (size_t)(base->shape[0] * base->shape[1] * base->itemsize);
+  void *dst = iamx_workspace(slot, bytes + 16);
+  if (!dst) return iax_dev_error(iamx_init() ? iamx_init() : IAMX_ENOMEM, "operand buffer");
+  if (!c->transposed) {
+    ina_rc_t rc = iax_decode_dense(base, dst, 0);
+    if (INA_FAILED(rc)) return rc;
+    *d_out = dst;
+    return INA_SUCCESS;
+  }
+  void *tmp = iamx_workspace(tmp_slot, bytes + 16);
+  if (!tmp) return iax_dev_error(IAMX_ENOMEM, "transpose buffer");
+  ina_rc_t rc = iax_decode_dense(base, tmp, 0);
+  if (INA_FAILED(rc)) return rc;
+  int e = iamx_transpose(base->itemsize, base->shape[0], base->shape[1], tmp, dst);
+  if (e) return iax_dev_error(e, "transpose");
+  *d_out = dst;
+  return INA_SUCCESS;
+}
+
+static ina_rc_t check_decode_status(void) {
+  int st = iamx_decode_status();
+  if (st < 0) return iax_dev_error(st, "decode status");
+  if (st) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "malformed LZ4 stream (device flags 0x%x)", st);
+  return INA_SUCCESS;
+}
+
+/* ---- write stage ------------------------------------------------------------------------------- */
+typedef struct {
+  iarray_container_t *c;
+  const uint8_t *packed;
+  int filtered;
+  int err;
+} wctx_t;
+
+static void write_chunk_task(void *vctx, int64_t nc, int worker) {
+  (void)worker;
+  wctx_t *x = (wctx_t *)vctx;
+  iarray_container_t *c = x->c;
+  free(c->chunks[nc]);
+  c->chunks[nc] = NULL;
+  if (iax_encode_chunk(x->packed + nc * c->chunk_nbytes, (int32_t)c->chunk_nbytes, c->blocksize, c->itemsize,
+                       &c->cfg, x->filtered, &c->chunks[nc], &c->csizes[nc]) != 0)
+    x->err = 1;
+}
+
+ina_rc_t iax_write_from_dense(iarray_container_t *c, const void *d_dense) {
+  iamx_tiling_t t;
+  memset(&t, 0, sizeof(t));
+  for (int i = 0; i < 2; i++) {
+    t.shape[i] = c->shape[i];
+    t.chunk[i] = c->chunk[i];
+    t.block[i] = c->block[i];
+    t.nchunks[i] = c->nchunks_ax[i];
+    t.nblocks[i] = c->nblocks_ax[i];
+  }
+  t.typesize = c->itemsize;
+  const int filtered = c->cfg.compression_level > 0;
+  t.filter = filtered ? iax_filter_code(&c->cfg) : IAMX_FILTER_NONE;
+  const size_t total = (size_t)(c->nchunks * c->chunk_nbytes);
+  void *d_packed = iamx_workspace(5, total);
+  if (!d_packed) return iax_dev_error(IAMX_ENOMEM, "pack buffer");
+  int e = iamx_pack_blocks(&t, d_dense, c->shape[1], d_packed);
+  if (e) return iax_dev_error(e, "pack");
+  uint8_t *host = (uint8_t *)malloc(total);
+  if (!host) return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+  e = iamx_d2h(host, d_packed, total);
+  if (e) {
+    free(host);
+    return iax_dev_error(e, "download C blocks");
+  }
+  const double t0 = iax_now();
+  wctx_t x = {c, host, filtered, 0};
+  iax_parallel_for(iax_nthreads(&c->cfg), c->nchunks, write_chunk_task, &x);
+  iax_last_encode_s = iax_now() - t0;
+  free(host);
+  drop_device(c);
+  if (x.err) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "chunk encode failed");
+  return INA_SUCCESS;
+}
+
+/* ---- read back ----------------------------------------------------------------------------------- */
+static int64_t nitems(const iarray_container_t *c) {
+  int64_t n = 1;
+  for (int i = 0; i < c->dtshape.ndim; i++) n *= c->dtshape.shape[i];
+  return n;
+}
+
+INA_API(ina_rc_t) iarray_to_buffer(iarray_context_t *ctx, iarray_container_t *c, void *buffer, int64_t buflen) {
+  FAIL_NULL(c);
+  const int64_t n = nitems(c);
+  if (n == 0) return INA_SUCCESS;
+  FAIL_NULL(ctx);
+  FAIL_NULL(buffer);
+  if (n * c->itemsize > buflen)
+    return iax_set_error(INA_ERROR(IARRAY_ERR_TOO_SMALL_BUFFER), "The buffer size is not enough");
+  void *d = NULL;
+  ina_rc_t rc = iax_operand(c, 2, 6, &d);
+  if (INA_FAILED(rc)) return rc;
+  int e = iamx_d2h(buffer, d, (size_t)(n * c->itemsize));
+  if (e) return iax_dev_error(e, "download");
+  return check_decode_status();
+}
+
+INA_API(ina_rc_t) iarray_get_slice_buffer(iarray_context_t *ctx, iarray_container_t *c, const int64_t *start,
+                                          const int64_t *stop, void *buffer, int64_t buflen) {
+  FAIL_NULL(ctx);
+  FAIL_NULL(c);
+  FAIL_NULL(start);
+  FAIL_NULL(stop);
+  FAIL_NULL(buffer);
+  const int nd = c->dtshape.ndim;
+  int64_t s[2] = {0, 0}, e[2] = {1, 1}, lshape[2] = {1, 1};
+  for (int i = 0; i < nd; i++) {
+    const int ax = nd == 2 ? i : 1;
+    const int64_t dim = c->dtshape.shape[i];
+    lshape[ax] = dim;
+    s[ax] = start[i] < 0 ? start[i] + dim : start[i];
+    e[ax] = stop[i] < 0 ? stop[i] + dim : stop[i];
+    if (s[ax] < 0 || e[ax] > dim || s[ax] > e[ax])
+      return iax_set_error(INA_ERROR(INA_ERR_INVALID_ARGUMENT), "Start is bigger than stop (axis %d)", i);
+  }
+  const int64_t rows = e[0] - s[0], cols = e[1] - s[1];
+  if (rows * cols * c->itemsize > buflen)
+    return iax_set_error(INA_ERROR(IARRAY_ERR_TOO_SMALL_BUFFER), "The buffer size is not enough");
+  if (rows == 0 || cols == 0) return INA_SUCCESS;
+  void *d = NULL;
+  ina_rc_t rc = iax_operand(c, 2, 6, &d);
+  if (INA_FAILED(rc)) return rc;
+  const int is = c->itemsize;
+  if (cols == lshape[1]) {
+    int err = iamx_d2h(buffer, (uint8_t *)d + s[0] * lshape[1] * is, (size_t)(rows * cols * is));
+    if (err) return iax_dev_error(err, "download");
+  } else {
+    uint8_t *tmp = (uint8_t *)malloc((size_t)(rows * lshape[1] * is));
+    if (!tmp) return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+    int err = iamx_d2h(tmp, (uint8_t *)d + s[0] * lshape[1] * is, (size_t)(rows * lshape[1] * is));
+    if (err) {
+      free(tmp);
+      return iax_dev_error(err, "download");
+    }
+    for (int64_t r = 0; r < rows; r++)
+      memcpy((uint8_t *)buffer + r * cols * is, tmp + (r * lshape[1] + s[1]) * is, (size_t)(cols * is));
+    free(tmp);
+  }
+  return check_decode_status();
+}
+
+/* ---- info ------------------------------------------------------------------------------------------ */
+INA_API(ina_rc_t) iarray_container_info(iarray_container_t *c, int64_t *nbytes, int64_t *cbytes) {
+  FAIL_NULL(c);
+  iarray_container_t *b = c->viewed ? c->viewed : c;
+  int64_t cb = 0;
+  for (int64_t i = 0; i < b->nchunks; i++) cb += b->csizes[i];
+  if (nbytes) *nbytes = nitems(c) * c->itemsize;
+  if (cbytes) *cbytes = cb;
+  return INA_SUCCESS;
+}
+
+INA_API(ina_rc_t) iarray_get_dtshape(iarray_context_t *ctx, iarray_container_t *c, iarray_dtshape_t *dtshape) {
+  (void)ctx;
+  FAIL_NULL(c);
+  FAIL_NULL(dtshape);
+  *dtshape = c->dtshape;
+  return INA_SUCCESS;
+}
+
+INA_API(ina_rc_t) iarray_get_storage(iarray_context_t *ctx, iarray_container_t *c, iarray_storage_t *storage) {
+  (void)ctx;
+  FAIL_NULL(c);
+  FAIL_NULL(storage);
+  *storage = c->storage;
+  return INA_SUCCESS;
+}
+
+INA_API(ina_rc_t) iarray_is_view(iarray_context_t *ctx, iarray_container_t *c, bool *view) {
+  (void)ctx;
+  FAIL_NULL(c);
+  FAIL_NULL(view);
+  *view = c->viewed != NULL;
+  return INA_SUCCESS;
+}
+
+INA_API(ina_rc_t) iarray_container_almost_equal(iarray_container_t *a, iarray_container_t *b, double tol) {
+  FAIL_NULL(a);
+  FAIL_NULL(b);
+  if (a->dtshape.dtype != b->dtshape.dtype) return INA_ERROR(IARRAY_ERR_INVALID_DTYPE);
+  if (a->dtshape.ndim != b->dtshape.ndim) return INA_ERROR(IARRAY_ERR_INVALID_NDIM);
+  for (int i = 0; i < a->dtshape.ndim; i++)
+    if (a->dtshape.shape[i] != b->dtshape.shape[i]) return INA_ERROR(IARRAY_ERR_INVALID_SHAPE);
+  const int64_t n = nitems(a);
+  const int is = a->itemsize;
+  uint8_t *ba = (uint8_t *)malloc((size_t)(n * is + 1)), *bb = (uint8_t *)malloc((size_t)(n * is + 1));
+  iarray_context_t ctx;
+  ctx.cfg = a->cfg;
+  ina_rc_t rc = (!ba || !bb) ? INA_ERROR(INA_ERR_OUT_OF_MEMORY) : iarray_to_buffer(&ctx, a, ba, n * is);
+  if (INA_SUCCEED(rc)) rc = iarray_to_buffer(&ctx, b, bb, n * is);
+  for (int64_t i = 0; i < n && INA_SUCCEED(rc); i++) {
+    double x, y;
+    if (a->dtshape.dtype == IARRAY_DATA_TYPE_DOUBLE) { x = ((double *)ba)[i]; y = ((double *)bb)[i]; }
+    else if (a->dtshape.dtype == IARRAY_DATA_TYPE_FLOAT) { x = ((float *)ba)[i]; y = ((float *)bb)[i]; }
+    else { rc = INA_ERROR(IARRAY_ERR_INVALID_DTYPE); break; }
+    /* the reference divides by the signed value of a (src/iarray_container.c:1255-1257) */
+    if ((x - y) / x > tol) rc = iax_set_error(INA_ERROR(IARRAY_ERR_ASSERTION_FAILED), "values differ at %lld", (long long)i);
+  }
+  free(ba);
+  free(bb);
+  return rc;
+}
+
+INA_API(ina_rc_t) iarray_linalg_transpose(iarray_context_t *ctx, iarray_container_t *a, iarray_container_t **b) {
+  FAIL_NULL(ctx);
+  FAIL_NULL(a);
+  FAIL_NULL(b);
+  if (a->viewed) return iax_set_error(INA_ERROR(INA_ERR_NOT_SUPPORTED), "Transposing views is not supported yet");
+  if (a->dtshape.ndim != 2)
+    return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_NDIM), "It is only supported for two dimensions");
+  iarray_container_t *v = (iarray_container_t *)calloc(1, sizeof(*v));
+  if (!v) return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+  *v = *a;
+  v->dtshape.shape[0] = a->dtshape.shape[1];
+  v->dtshape.shape[1] = a->dtshape.shape[0];
+  v->viewed = a;
+  v->transposed = true;
+  v->plan_valid = 0;
+  memset(&v->plan, 0, sizeof(v->plan));
+  *b = v;
+  return INA_SUCCESS;
+}
+
+/* ---- raw chunk access -------------------------------------------------------------------------------- */
+INA_API(ina_rc_t) iarray_container_nchunks(iarray_container_t *c, int64_t *nchunks) {
+  FAIL_NULL(c);
+  FAIL_NULL(nchunks);
+  *nchunks = (c->viewed ? c->viewed : c)->nchunks;
+  return INA_SUCCESS;
+}
+
+INA_API(ina_rc_t) iarray_container_chunk(iarray_container_t *c, int64_t nchunk, const uint8_t **chunk,
+                                         int32_t *csize) {
+  FAIL_NULL(c);
+  FAIL_NULL(chunk);
+  FAIL_NULL(csize);
+  iarray_container_t *b = c->viewed ? c->viewed : c;
+  if (nchunk < 0 || nchunk >= b->nchunks) return INA_ERROR(INA_ERR_INVALID_ARGUMENT);
+  *chunk = b->chunks[nchunk];
+  *csize = b->csizes[nchunk];
+  return INA_SUCCESS;
+}
+
+INA_API(ina_rc_t) iarray_from_chunks(iarray_context_t *ctx, iarray_dtshape_t *dtshape, iarray_storage_t *storage,
+                                     int64_t nchunks, const uint8_t *const *chunks, const int32_t *csizes,
+                                     iarray_container_t **container) {
+  FAIL_NULL(chunks);
+  FAIL_NULL(csizes);
+  ina_rc_t rc = iax_container_new(ctx, dtshape, storage, container);
+  if (INA_FAILED(rc)) return rc;
+  iarray_container_t *c = *container;
+  if (nchunks != c->nchunks) {
+    iarray_container_free(ctx, container);
+    return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_CHUNKSHAPE), "expected %lld chunks", (long long)c->nchunks);
+  }
+  for (int64_t i = 0; i < nchunks; i++) {
+    c->chunks[i] = (uint8_t *)malloc((size_t)csizes[i]);
+    if (!c->chunks[i]) return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+    memcpy(c->chunks[i], chunks[i], (size_t)csizes[i]);
+    c->csizes[i] = csizes[i];
+  }
+  return INA_SUCCESS;
+}
+
+INA_API(ina_rc_t) iarray_decode_chunks_flat(iarray_context_t *ctx, int64_t nchunks, const uint8_t *const *chunks,
+                                            const int32_t *csizes, void *out, int64_t outlen) {
+  FAIL_NULL(ctx);
+  FAIL_NULL(chunks);
+  FAIL_NULL(csizes);
+  FAIL_NULL(out);
+  /* one arena with every chunk; each block becomes a one-row tile at its flat offset */
+  int64_t total = 0, nbytes_total = 0;
+  int64_t *off = (int64_t *)malloc((size_t)nchunks * sizeof(int64_t));
+  if (!off) return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+  int ts0 = 0;
+  for (int64_t i = 0; i < nchunks; i++) {
+    iax_header_t h;
+    if (iax_parse_header(chunks[i], csizes[i], &h)) {
+      free(off);
+      return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "chunk %lld: bad header", (long long)i);
+    }
+    if (!ts0) ts0 = h.typesize;
+    if (h.typesize != ts0 || h.nbytes % h.typesize) {
+      free(off);
+      return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "chunks must share one typesize");
+    }
+    off[i] = total;
+    total += (csizes[i] + 15) & ~15;
+    nbytes_total += h.nbytes;
+  }
+  if (nbytes_total > outlen) {
+    free(off);
+    return iax_set_error(INA_ERROR(IARRAY_ERR_TOO_SMALL_BUFFER), "output too small");
+  }
+  uint8_t *d_arena = (uint8_t *)iamx_malloc((size_t)(total + 16));
+  uint8_t *d_out = (uint8_t *)iamx_malloc((size_t)(nbytes_total + 16));
+  uint8_t *host = (uint8_t *)malloc((size_t)(total + 16));
+  ina_rc_t rc = INA_SUCCESS;
+  if (!d_arena || !d_out) rc = iax_dev_error(iamx_init() ? iamx_init() : IAMX_ENOMEM, "flat decode buffers");
+  if (!host) rc = INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+  jobs_t jobs;
+  memset(&jobs, 0, sizeof(jobs));
+  if (INA_SUCCEED(rc)) {
+    for (int64_t i = 0; i < nchunks; i++) memcpy(host + off[i], chunks[i], (size_t)csizes[i]);
+    int e = iamx_h2d(d_arena, host, (size_t)total);
+    if (e) rc = iax_dev_error(e, "upload");
+  }
+  int64_t elem0 = 0;
+  for (int64_t i = 0; i < nchunks && INA_SUCCEED(rc); i++) {
+    iax_header_t h;
+    iax_parse_header(chunks[i], csizes[i], &h);
+    for (int nb = 0; nb < h.nblocks && INA_SUCCEED(rc); nb++) {
+      iamx_block_job_t J;
+      memset(&J, 0, sizeof(J));
+      rc = block_streams(chunks[i], csizes[i], (uint64_t)(uintptr_t)(d_arena + off[i]), &h, nb, h.typesize, &jobs, &J);
+      if (INA_FAILED(rc)) break;
+      const int32_t n = J.bsize / J.typesize;
+      J.rows = 1;
+      J.cols = n;
+      J.vrows = 1;
+      J.vcols = n;
+      J.dst_off = elem0 + (int64_t)nb * (h.blocksize / h.typesize);
+      J.dst_ld = n;
+      if (push_blk(&jobs, &J, 0, 0, 0)) rc = INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+    }
+    elem0 += h.nbytes / h.typesize;
+  }
+  iax_plan_t plan;
+  memset(&plan, 0, sizeof(plan));
+  if (INA_SUCCEED(rc)) rc = upload_plan(&jobs, &plan);
+  if (INA_SUCCEED(rc)) {
+    void *scratch = iamx_workspace(0, (size_t)plan.scratch_bytes + 16);
+    int e = scratch ? iamx_decode(plan.d_lz4, plan.n_lz4, plan.d_blocks, plan.n_blocks, scratch, d_out) : IAMX_ENOMEM;
+    if (!e) e = iamx_d2h(out, d_out, (size_t)nbytes_total);
+    if (e) rc = iax_dev_error(e, "flat decode");
+    if (INA_SUCCEED(rc)) rc = check_decode_status();
+  }
+  free_plan(&plan);
+  free_jobs(&jobs);
+  free(host);
+  free(off);
+  iamx_free(d_arena);
+  iamx_free(d_out);
+  return rc;
+}

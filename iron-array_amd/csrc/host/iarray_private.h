@@ -1,0 +1,111 @@
+/*
+ * iarray_private.h -- host-side types of libiarray.so (not installed).
+ *
+ * A container is a Caterva-style two-level tiled array (chunks of blocks)
+ * whose chunks are Blosc2 chunks (LZ4 + shuffle/bitshuffle).  The compressed
+ * chunks live in host memory (the super-chunk) and are mirrored once into one
+ * device arena; the decode plan (per-block stream table) is built from the
+ * chunk headers and kept on the device next to it.
+ * Mirrors the roles of iarray_context_s / iarray_container_s
+ * (src/iarray_private.h:79-117).
+ */
+#ifndef IAMX_IARRAY_PRIVATE_H_
+#define IAMX_IARRAY_PRIVATE_H_
+
+#include <stdint.h>
+
+#include "../../../include/iamx.h"
+#include "../../../include/iarray.h"
+
+struct iarray_context_s {
+  iarray_config_t cfg;
+};
+
+/* Decode plan of a container: device job tables for iamx_decode. */
+typedef struct iax_plan_s {
+  iamx_lz4_job_t *d_lz4;
+  int n_lz4;
+  iamx_block_job_t *d_blocks;
+  int n_blocks;
+  int32_t *d_blk_rc; /* (row0, col0, block column, 0) per block job, for the fused GEMV */
+  int64_t scratch_bytes;
+  int64_t nbcols; /* block columns of the global block grid */
+  int all_zero;   /* every block is a zero block */
+} iax_plan_t;
+
+struct iarray_container_s {
+  iarray_dtshape_t dtshape; /* logical shape (a transposed view swaps it) */
+  iarray_storage_t storage;
+  iarray_config_t cfg;      /* codec settings the chunks are written with */
+  int itemsize;
+  int ndim;                 /* underlying ndim (1 or 2) */
+  /* tiling, normalised to 2-D: a 1-D array is (1, n) */
+  int64_t shape[2], chunk[2], block[2], extshape[2], extchunk[2];
+  int64_t nchunks_ax[2], nblocks_ax[2];
+  int64_t nchunks;
+  int64_t chunk_nbytes; /* prod(extchunkshape) * itemsize */
+  int32_t blocksize;    /* prod(blockshape) * itemsize */
+  /* host super-chunk */
+  uint8_t **chunks;
+  int32_t *csizes;
+  /* device mirror */
+  uint8_t *d_arena;
+  int64_t *arena_off;
+  int device_valid;
+  iax_plan_t plan;
+  int plan_valid;
+  /* views */
+  iarray_container_t *viewed;
+  bool transposed;
+};
+
+/* ---- codec (blosc_codec.c) ---------------------------------------------------- */
+typedef struct {
+  int version, flags, typesize, header_len, memcpyed, dont_split, compformat, special;
+  int32_t nbytes, blocksize, cbytes, leftover;
+  int nblocks;
+  uint8_t filters[6], filters_meta[6];
+  int blosc2_flags;
+} iax_header_t;
+
+int iax_parse_header(const uint8_t *chunk, int32_t csize, iax_header_t *h);
+int iax_lz4_compress(const uint8_t *src, int n, uint8_t *dst, int cap, int accel, uint32_t *hashtab);
+void iax_shuffle(int ts, int32_t bsize, const uint8_t *src, uint8_t *dst);
+void iax_bitshuffle(int ts, int32_t bsize, const uint8_t *src, uint8_t *dst);
+/* Encode one chunk.  `data` holds nbytes of chunk content; when `filtered` is
+ * non-zero every block of `data` already went through the chunk's filter (the
+ * GPU pack stage did it).  Returns a malloc'ed chunk in *out and its size. */
+int iax_encode_chunk(const uint8_t *data, int32_t nbytes, int32_t blocksize, int typesize,
+                     const iarray_config_t *cfg, int filtered, uint8_t **out, int32_t *csize);
+int iax_filter_code(const iarray_config_t *cfg);  /* IAMX_FILTER_* the config asks for */
+int iax_split(const iarray_config_t *cfg, int typesize, int32_t blocksize);
+
+/* ---- host parallel-for (threadpool.c) ----------------------------------------------- */
+typedef void (*iax_task_fn)(void *ctx, int64_t i, int worker);
+int iax_nthreads(const iarray_config_t *cfg);
+void iax_parallel_for(int nthreads, int64_t n, iax_task_fn fn, void *ctx);
+double iax_now(void);
+
+/* ---- containers (iarray_container.c) ---------------------------------------------------- */
+ina_rc_t iax_container_new(iarray_context_t *ctx, iarray_dtshape_t *dtshape, iarray_storage_t *storage,
+                           iarray_container_t **c);
+ina_rc_t iax_ensure_device(iarray_container_t *c);
+ina_rc_t iax_ensure_plan(iarray_container_t *c);
+/* Decode the whole (base) container into a dense row-major device buffer of
+ * shape (shape0, shape1) with leading dimension shape1. */
+ina_rc_t iax_decode_dense(iarray_container_t *c, void *d_dst, int scratch_slot);
+/* Decode a container or transposed view as an operand: dense logical-shape
+ * row-major device buffer in workspace slot `slot` (uses `tmp_slot` for views). */
+ina_rc_t iax_operand(iarray_container_t *c, int slot, int tmp_slot, void **d_out);
+/* Write C from a dense device buffer (logical shape, ld = shape1): GPU pack +
+ * host LZ4 encode; fills c->chunks. */
+ina_rc_t iax_write_from_dense(iarray_container_t *c, const void *d_dense);
+int64_t iax_dtype_size(iarray_data_type_t t);
+
+ina_rc_t iax_set_error(ina_rc_t rc, const char *fmt, ...);
+ina_rc_t iax_dev_error(int code, const char *what);
+
+extern double iax_last_encode_s;
+extern double iax_last_device_ms;
+
+#endif

@@ -173,7 +173,7 @@ static void make_tiling(int ndim, int itemsize, const int64_t *shape, const int6
                         const int64_t *block, tiling_t *t) {
   t->itemsize = itemsize;
   for (int i = 0; i < 2; i++) {
-    int s = ndim == 2 ? i : 1;
+    int s = ndim == 2 ? i : 0;
     t->shape[i] = (ndim == 1 && i == 0) ? 1 : shape[s];
     t->chunk[i] = (ndim == 1 && i == 0) ? 1 : chunk[s];
     t->block[i] = (ndim == 1 && i == 0) ? 1 : block[s];

@@ -1,0 +1,192 @@
+"""GPU parity of the block decode stage (bit-exact).
+
+* the reference's Blosc2 frame fixtures (written by c-blosc2) decode on the GPU
+  to the SHA-256s of tests/golden/fixture_hashes.json;
+* chunks from c-blosc 1.21 (byte shuffle, bitshuffle, ragged element counts,
+  format-2 rules) decode on the GPU to their inputs, when that library exists;
+* every tiling / dtype / filter / split mode the host writes round-trips
+  through the GPU decoder bit-exactly, checked against the oracle decoder;
+* a corrupted LZ4 stream is reported as IARRAY_ERR_BLOSC_FAILED (the kernel
+  bounds-checks every read and write, so no fault is possible).
+"""
+import ctypes
+import glob
+import hashlib
+import json
+import os
+import struct
+
+import numpy as np
+import pytest
+
+import pyiarray as ia
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+HASHES = json.load(open(os.path.join(GOLD, "fixture_hashes.json")))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return ia.Context(clevel=5, filters=ia.COMP_SHUFFLE)
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLD, "frames", "*.iarray"))), ids=os.path.basename)
+def test_fixture_frames_decode_on_gpu(ctx, orc, path):
+    print(f"case: fixture {os.path.basename(path)}", flush=True)
+    chunks, nbytes = orc.frame_chunks(open(path, "rb").read())
+    out = ia.decode_chunks_flat(ctx, chunks, nbytes)
+    assert hashlib.sha256(out).hexdigest() == HASHES[os.path.basename(path)]["sha256"]
+
+
+def test_fixture_frame_as_container(ctx, orc):
+    """The same frame, wrapped as a 1-D container with the generator's tiling
+    (shape 1e6, chunks 1e5, blocks 1e4; tests/generator.py:23-38)."""
+    path = os.path.join(GOLD, "frames", "test_rand_float32.iarray")
+    chunks, nbytes = orc.frame_chunks(open(path, "rb").read())
+    c = ia.Container.from_chunks(ctx, np.float32, (1000000,), (100000,), (10000,), chunks)
+    x = c.to_numpy()
+    assert hashlib.sha256(x.tobytes()).hexdigest() == HASHES["test_rand_float32.iarray"]["sha256"]
+    assert np.array_equal(c.slice((123456,), (223457,)), x[123456:223457])
+
+
+def _libblosc():
+    try:
+        b = ctypes.CDLL("/opt/conda/lib/libblosc.so.1")
+    except OSError:
+        return None
+    b.blosc_init()
+    b.blosc_compress.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_void_p,
+                                 ctypes.c_void_p, ctypes.c_size_t]
+    b.blosc_set_blocksize.argtypes = [ctypes.c_size_t]
+    b.blosc_set_compressor.argtypes = [ctypes.c_char_p]
+    b.blosc_set_compressor(b"lz4")
+    return b
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shuf", [1, 2], ids=["shuffle", "bitshuffle"])
+def test_cblosc1_chunks_decode_on_gpu(ctx, dtype, shuf):
+    b = _libblosc()
+    if b is None:
+        pytest.skip("c-blosc 1.21 not present")
+    rng = np.random.default_rng(5)
+    for n in (4096, 4100, 62500):
+        for x in (rng.random(n), np.sin(np.arange(n) * 1e-3) + rng.normal(0, 1e-3, n), np.arange(n) % 9):
+            x = x.astype(dtype)
+            for bs in (0, 2048, 8000):
+                print(f"case: cblosc1 {np.dtype(dtype).name} shuf={shuf} n={n} bs={bs}", flush=True)
+                b.blosc_set_blocksize(bs)
+                dst = ctypes.create_string_buffer(x.nbytes + 16)
+                cs = b.blosc_compress(5, shuf, x.itemsize, x.nbytes, x.ctypes.data, dst, x.nbytes + 16)
+                assert cs > 0
+                assert ia.decode_chunks_flat(ctx, [dst.raw[:cs]], x.nbytes) == x.tobytes()
+
+
+CASES = [
+    (np.float64, (50, 37), (20, 15), (7, 6), ia.COMP_SHUFFLE, 5, ia.AUTO_SPLIT),
+    (np.float32, (100, 170), (27, 30), (12, 7), ia.COMP_SHUFFLE, 5, ia.AUTO_SPLIT),
+    (np.float32, (64, 64), (64, 64), (16, 16), ia.COMP_BITSHUFFLE, 5, ia.AUTO_SPLIT),
+    (np.float32, (61, 47), (30, 20), (9, 13), ia.COMP_BITSHUFFLE, 9, ia.AUTO_SPLIT),
+    (np.float64, (33, 65), (33, 65), (11, 13), ia.COMP_SHUFFLE, 1, ia.NEVER_SPLIT),
+    (np.float64, (40, 40), (20, 20), (10, 10), 0, 5, ia.ALWAYS_SPLIT),
+    (np.float32, (40, 40), (20, 20), (10, 10), ia.COMP_SHUFFLE, 0, ia.AUTO_SPLIT),
+    (np.int32, (1000,), (300,), (64,), ia.COMP_SHUFFLE, 5, ia.AUTO_SPLIT),
+    (np.float64, (777,), (256,), (100,), ia.COMP_BITSHUFFLE, 3, ia.AUTO_SPLIT),
+    (np.float32, (500, 500), (250, 250), (250, 250), ia.COMP_BITSHUFFLE, 5, ia.AUTO_SPLIT),
+    (np.float64, (300, 260), (128, 128), (64, 64), ia.COMP_SHUFFLE, 5, ia.AUTO_SPLIT),
+]
+
+
+def _data(dtype, shape, kind):
+    rng = np.random.default_rng(sum(shape) * 31 + len(kind))
+    n = int(np.prod(shape))
+    if kind == "smooth":
+        x = np.sin(np.arange(n) * 1e-2) + rng.normal(0, 1e-3, n)
+    elif kind == "random":
+        x = rng.standard_normal(n) * 100
+    elif kind == "zeros":
+        x = np.zeros(n)
+    else:
+        x = np.arange(n) % 7
+    return x.astype(dtype).reshape(shape)
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"{np.dtype(c[0]).name}-{c[1]}-{c[2]}-{c[3]}-f{c[4]}-l{c[5]}-s{c[6]}"
+                                              for c in CASES])
+@pytest.mark.parametrize("kind", ["smooth", "random", "zeros", "ramp"])
+def test_roundtrip_gpu_decode_bit_exact(orc, case, kind):
+    dtype, shape, chunks, blocks, filt, clevel, split = case
+    print(f"case: roundtrip {case} {kind}", flush=True)
+    ctx = ia.Context(clevel=clevel, filters=filt, splitmode=split)
+    x = _data(dtype, shape, kind)
+    c = ia.Container.from_numpy(ctx, x, chunks, blocks)
+    got = c.to_numpy()
+    assert got.tobytes() == x.tobytes()
+    ref = orc.Array(c.chunks(), shape, chunks, blocks, x.itemsize).to_dense(dtype=dtype)
+    assert ref.tobytes() == got.tobytes()
+
+
+def test_zeros_and_empty_read_as_zero(ctx):
+    z = ia.Container.zeros(ctx, np.float64, (70, 90), (32, 40), (16, 20))
+    assert not z.to_numpy().any()
+    e = ia.Container.empty(ctx, np.float32, (70, 90), (32, 40), (16, 20))
+    assert not e.to_numpy().any()
+
+
+def test_transposed_view_reads_transposed(ctx):
+    x = np.arange(170 * 100, dtype=np.float64).reshape(170, 100)
+    c = ia.Container.from_numpy(ctx, x, (20, 30), (7, 11))
+    t = c.transpose()
+    assert np.array_equal(t.to_numpy(), x.T)
+    assert np.array_equal(t.slice((3, 5), (50, 160)), x.T[3:50, 5:160])
+
+
+def test_get_slice_buffer(ctx):
+    x = np.random.default_rng(3).standard_normal((257, 129)).astype(np.float32)
+    c = ia.Container.from_numpy(ctx, x, (100, 64), (25, 16))
+    for st, sp in [((0, 0), (257, 129)), ((10, 5), (200, 100)), ((-50, -20), (-1, -1)), ((5, 0), (6, 129))]:
+        s = [(a if a >= 0 else a + d) for a, d in zip(st, x.shape)]
+        e = [(b if b >= 0 else b + d) for b, d in zip(sp, x.shape)]
+        assert np.array_equal(c.slice(st, sp), x[s[0]:e[0], s[1]:e[1]])
+
+
+def _corrupt_offset(chunk: bytes) -> bytes:
+    """Point the first LZ4 match of the first LZ4 stream far before the
+    stream start (an offset larger than the bytes produced so far)."""
+    b = bytearray(chunk)
+    nblocks = -(-struct.unpack("<i", b[4:8])[0] // struct.unpack("<i", b[8:12])[0])
+    ts = b[3]
+    neb = struct.unpack("<i", b[8:12])[0] // ts
+    p = struct.unpack("<i", b[32:36])[0]
+    for _ in range(ts * nblocks):
+        cs = struct.unpack("<i", b[p:p + 4])[0]
+        if 0 < cs < neb:
+            q = p + 4
+            tok = b[q]
+            lit = tok >> 4
+            q += 1
+            if lit == 15:
+                while b[q] == 255:
+                    lit += 255
+                    q += 1
+                lit += b[q]
+                q += 1
+            q += lit
+            b[q] = 0xFF
+            b[q + 1] = 0xFF
+            return bytes(b)
+        p += 4 + max(cs, 0)
+    raise AssertionError("no LZ4 stream found")
+
+
+def test_corrupted_stream_reports_error(ctx):
+    x = (np.arange(4096) % 5).astype(np.float32)
+    c = ia.Container.from_numpy(ctx, x, (4096,), (1024,))
+    bad = _corrupt_offset(c.chunks()[0])
+    with pytest.raises(ia.IArrayError) as e:
+        ia.decode_chunks_flat(ctx, [bad], x.nbytes)
+    assert e.value.rc == ia.ERR_BLOSC_FAILED
+    # the library keeps working afterwards
+    assert ia.decode_chunks_flat(ctx, c.chunks(), x.nbytes) == x.tobytes()

@@ -1,0 +1,176 @@
+"""GPU parity of iarray_linalg_matmul / iarray_opt_gemm / iarray_opt_gemv.
+
+Oracle: oracle/ (gemm_blosc, gemv_blosc, opt_gemm, opt_gemv restated on the
+CPU, plain-C BLAS), run on the same compressed containers.  Bars:
+  * the reference's own test fixtures (tests/test_linalg_gemm.c:180-271,
+    test_linalg_gemv.c:200-222, test_opt_gemv.c:179-225,
+    test_linalg_gemm_transpose.c:193-215) with their generators (arange /
+    linspace) and tolerance, elementwise |z-o|/|z| <= 1e-14 (float64) or
+    1e-5 (float32) (tests/test_linalg_gemm.c:128-147);
+  * integer-valued operands, where every summation order is exact: bit-exact;
+  * determinism: the same call twice gives the same bits.
+"""
+import numpy as np
+import pytest
+
+import pyiarray as ia
+
+pytestmark = pytest.mark.gpu
+
+TOL = {np.dtype(np.float64): 1e-14, np.dtype(np.float32): 1e-5}
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return ia.Context(clevel=5, filters=ia.COMP_SHUFFLE)
+
+
+def _oracle(orc, c):
+    return orc.Array(c.chunks(), c.shape, c._cs, c._bs, c.dtype.itemsize)
+
+
+def _mk(ctx, gen, dtype, shape, cs, bs, **kw):
+    if gen == "arange":
+        c = ia.Container.arange(ctx, dtype, shape, cs, bs, **kw)
+    else:
+        c = ia.Container.linspace(ctx, dtype, shape, cs, bs, **kw)
+    c._cs, c._bs = cs, bs
+    return c
+
+
+def _rel_ok(z, o, tol):
+    z = z.astype(np.float64)
+    o = o.astype(np.float64)
+    nz = z != 0
+    assert np.array_equal(z[~nz], o[~nz]) or np.allclose(z[~nz], o[~nz], atol=0)
+    rel = np.abs((z[nz] - o[nz]) / z[nz])
+    assert rel.max(initial=0) <= tol, f"max rel err {rel.max():.3e} > {tol}"
+
+
+GEMM_FIXTURES = [
+    # name, dtype, x(shape, cs, bs), y(shape, cs, bs), z(cs, bs)   -- test_linalg_gemm.c
+    ("f_schunk_schunk", np.float32, ((100, 170), (27, 30), (12, 7)), ((170, 21), (20, 11), (10, 5)), ((40, 3), (20, 3))),
+    ("d_schunk_schunk", np.float64, ((300, 670), (28, 300), (11, 31)), ((670, 210), (20, 45), (7, 12)),
+     ((43, 11), (13, 5))),
+    ("f_schunk_schunk2", np.float32, ((100, 200), (10, 30), (10, 10)), ((200, 150), (25, 30), (15, 12)),
+     ((20, 30), (11, 6))),
+    ("d_schunk_schunk2", np.float64, ((100, 200), (10, 30), (7, 7)), ((200, 150), (25, 30), (5, 11)),
+     ((20, 30), (17, 7))),
+]
+
+
+@pytest.mark.parametrize("fx", GEMM_FIXTURES, ids=[f[0] for f in GEMM_FIXTURES])
+def test_matmul_reference_gemm_fixtures(ctx, orc, fx):
+    name, dt, (xs, xc, xb), (ys, yc, yb), (zc, zb) = fx
+    print(f"case: gemm fixture {name}", flush=True)
+    x = _mk(ctx, "arange", dt, xs, xc, xb)
+    y = _mk(ctx, "arange", dt, ys, yc, yb)
+    z = ia.matmul(x, y, zc, zb)
+    got = z.to_numpy()
+    o = orc.gemm_blosc(_oracle(orc, x), _oracle(orc, y), zc, zb)
+    _rel_ok(got, o, TOL[np.dtype(dt)])
+    # the oracle decoder reads the C chunks the GPU path wrote
+    oz = orc.Array(z.chunks(), z.shape, zc, zb, z.dtype.itemsize)
+    assert oz.to_dense().tobytes() == got.tobytes()
+
+
+def test_matmul_transposed_fixture(ctx, orc):
+    """test_linalg_gemm_transpose.c:193-215: x and y are transposed views."""
+    xt = _mk(ctx, "arange", np.float64, (170, 100), (20, 30), (10, 12))
+    yt = _mk(ctx, "arange", np.float64, (21, 170), (11, 20), (5, 10))
+    x, y = xt.transpose(), yt.transpose()
+    z = ia.matmul(x, y, (40, 3), (20, 3))
+    xd = orc.Array(xt.chunks(), (170, 100), (20, 30), (10, 12), 8, transposed=True)
+    yd = orc.Array(yt.chunks(), (21, 170), (11, 20), (5, 10), 8, transposed=True)
+    o = orc.gemm_blosc(xd, yd, (40, 3), (20, 3))
+    _rel_ok(z.to_numpy(), o, 1e-14)
+
+
+GEMV_FIXTURES = [
+    # test_linalg_gemv.c:200-222 style: x(shape, cs, bs), y(shape, cs, bs), z(cs, bs)
+    ("gemv_f", np.float32, ((650, 900), (200, 140), (20, 14)), ((900,), (421,), (55,)), ((155,), (47,))),
+    ("gemv_d", np.float64, ((650, 900), (200, 140), (20, 14)), ((900,), (421,), (55,)), ((155,), (47,))),
+]
+
+
+@pytest.mark.parametrize("fx", GEMV_FIXTURES, ids=[f[0] for f in GEMV_FIXTURES])
+def test_matmul_gemv_fixtures(ctx, orc, fx):
+    name, dt, (xs, xc, xb), (ys, yc, yb), (zc, zb) = fx
+    print(f"case: gemv fixture {name}", flush=True)
+    x = _mk(ctx, "linspace", dt, xs, xc, xb, start=0.0, stop=10.0)
+    y = _mk(ctx, "linspace", dt, ys, yc, yb, start=0.0, stop=10.0)
+    z = ia.matmul(x, y, zc, zb)
+    o = orc.gemv_blosc(_oracle(orc, x), _oracle(orc, y), zc, zb)
+    _rel_ok(z.to_numpy(), o, TOL[np.dtype(dt)])
+
+
+OPT_GEMV = [
+    # test_opt_gemv.c:179-225
+    ("opt_gemv_f", np.float32, ((6000, 4000), (2000, 1000), (200, 200)), ((4000,), (1000,), (200,)),
+     ((2000,), (200,))),
+    ("opt_gemv_d", np.float64, ((8000, 2000), (400, 300), (200, 150)), ((2000,), (300,), (150,)),
+     ((400,), (200,))),
+]
+
+
+@pytest.mark.parametrize("fx", OPT_GEMV, ids=[f[0] for f in OPT_GEMV])
+def test_opt_gemv_fixtures(ctx, orc, fx):
+    name, dt, (xs, xc, xb), (ys, yc, yb), (zc, zb) = fx
+    print(f"case: opt_gemv fixture {name}", flush=True)
+    x = _mk(ctx, "linspace", dt, xs, xc, xb, start=0.0, stop=10.0)
+    y = _mk(ctx, "linspace", dt, ys, yc, yb, start=0.0, stop=10.0)
+    z = ia.opt_gemv(x, y, zc, zb)
+    o = orc.opt_gemv(_oracle(orc, x), _oracle(orc, y), zc, zb, nthreads=8)
+    _rel_ok(z.to_numpy(), o, TOL[np.dtype(dt)])
+
+
+def test_opt_gemm_matches_oracle(ctx, orc):
+    x = _mk(ctx, "arange", np.float64, (300, 250), (100, 90), (50, 30))
+    y = _mk(ctx, "arange", np.float64, (250, 170), (90, 60), (30, 20))
+    z = ia.opt_gemm(x, y, (100, 60), (50, 20))
+    o = orc.opt_gemm(_oracle(orc, x), _oracle(orc, y), (100, 60), (50, 20))
+    _rel_ok(z.to_numpy(), o, 1e-14)
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("shape", [(129, 257, 65), (300, 200, 500), (64, 64, 64), (1, 300, 7)])
+def test_integer_operands_bit_exact(ctx, dt, shape):
+    M, K, N = shape
+    rng = np.random.default_rng(M * 7 + K)
+    a = rng.integers(-8, 9, (M, K)).astype(dt)
+    b = rng.integers(-8, 9, (K, N)).astype(dt)
+    ca = ia.Container.from_numpy(ctx, a, (min(M, 100), min(K, 96)), (min(M, 50), min(K, 32)))
+    cb = ia.Container.from_numpy(ctx, b, (min(K, 96), min(N, 80)), (min(K, 32), min(N, 40)))
+    cc = (min(M, 64), min(N, 64))
+    z = ia.matmul(ca, cb, cc, (min(M, 32), min(N, 16)))
+    want = (a.astype(np.float64) @ b.astype(np.float64)).astype(dt)
+    assert np.array_equal(z.to_numpy(), want)
+    x = rng.integers(-8, 9, (K,)).astype(dt)
+    cx = ia.Container.from_numpy(ctx, x, (min(K, 96),), (min(K, 32),))
+    y = ia.matmul(ca, cx, (min(M, 64),), (min(M, 32),))
+    assert np.array_equal(y.to_numpy(), (a.astype(np.float64) @ x.astype(np.float64)).astype(dt))
+
+
+def test_bitshuffled_gemv_bit_exact(ctx):
+    """The headline GEMV layout (float32, LZ4 + bitshuffle, 250x250 blocks,
+    62500 elements per block so every block has a 4-element tail)."""
+    bctx = ia.Context(clevel=5, filters=ia.COMP_BITSHUFFLE)
+    rng = np.random.default_rng(11)
+    a = rng.integers(-4, 5, (1000, 750)).astype(np.float32)
+    x = rng.integers(-4, 5, (750,)).astype(np.float32)
+    ca = ia.Container.from_numpy(bctx, a, (500, 500), (250, 250))
+    cx = ia.Container.from_numpy(bctx, x, (500,), (250,))
+    y = ia.matmul(ca, cx, (500,), (250,))
+    assert np.array_equal(y.to_numpy(), (a.astype(np.float64) @ x.astype(np.float64)).astype(np.float32))
+
+
+def test_matmul_deterministic(ctx):
+    rng = np.random.default_rng(2)
+    a = rng.standard_normal((400, 300))
+    b = rng.standard_normal((300, 350))
+    ca = ia.Container.from_numpy(ctx, a, (200, 150), (50, 50))
+    cb = ia.Container.from_numpy(ctx, b, (150, 175), (50, 25))
+    z1 = ia.matmul(ca, cb, (200, 175), (50, 25)).to_numpy()
+    z2 = ia.matmul(ca, cb, (200, 175), (50, 25)).to_numpy()
+    assert z1.tobytes() == z2.tobytes()
+    assert np.allclose(z1, a @ b, rtol=1e-12, atol=1e-12)
