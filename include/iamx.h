@@ -134,6 +134,40 @@ IAMX_API int iamx_pack_blocks(const iamx_tiling_t *tiling, const void *d_src, in
  * transposed view (src/iarray_container.c:718-723). */
 IAMX_API int iamx_transpose(int typesize, int64_t rows, int64_t cols, const void *d_src, void *d_dst);
 
+/* ---- many products in one pass ------------------------------------------------------
+ * Replaces the reference's one-call-per-product loop (each call re-decoding its
+ * panels, src/linalg/matmul.c:19-68): all tiles of all products are pulled
+ * from one device work queue by a persistent kernel. Offsets are in elements
+ * of the dtype from base_a / base_b / base_c (dense row-major, ld = K, N, N). */
+typedef struct iamx_batch_prod_s {
+  int64_t M, N, K;
+  int64_t a_off, b_off, c_off;
+} iamx_batch_prod_t;
+
+/* d_tiles: 3 int32 per tile (product, tile row, tile column), 128x128 tiles. */
+IAMX_API int iamx_gemm_batch(int dtype, const iamx_batch_prod_t *d_prods, const int32_t *d_tiles, int ntiles,
+                             const void *base_a, const void *base_b, void *base_c);
+
+typedef struct iamx_batch_pack_s {
+  iamx_tiling_t tiling;
+  int64_t c_off_bytes; /* product's dense C inside base_c */
+  int64_t out_off;     /* product's packed blocks inside out */
+} iamx_batch_pack_t;
+
+/* d_blocks: 2 int32 per C block (product, block index inside the product). */
+IAMX_API int iamx_pack_batch(const iamx_batch_pack_t *d_packs, const int32_t *d_blocks, int nblocks,
+                             const void *base_c, void *d_out);
+
+/* ---- HIP graphs of the library stream ------------------------------------------------
+ * Everything the library launches between begin and end is recorded (not
+ * run) and instantiated into an executable graph; launching it replays the
+ * same kernels on the same buffers.  Profiling events are not recorded while
+ * capturing; a replay is timed as one "graph_launch" entry. */
+IAMX_API int iamx_capture_begin(void);
+IAMX_API int iamx_capture_end(void **graph_exec);
+IAMX_API int iamx_graph_launch(void *graph_exec);
+IAMX_API void iamx_graph_destroy(void *graph_exec);
+
 /* Error word written by the decode kernels (0 = no error since last reset). */
 IAMX_API int iamx_decode_status(void);
 

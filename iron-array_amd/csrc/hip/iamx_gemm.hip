@@ -148,6 +148,11 @@ __global__ __launch_bounds__(256) void k_gemv(int64_t M, int64_t K, const T *__r
 
 extern "C" {
 
+int iamx_gemm_split_bf16(int64_t M, int64_t N, int64_t K, const float *A, int64_t lda, const float *B, int64_t ldb,
+                         float *C, int64_t ldc);
+int iamx_gemm_ozaki_i8(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B, int64_t ldb,
+                       double *C, int64_t ldc, int S);
+
 IAMX_API int iamx_gemm(int dtype, int precision, int slices, int64_t M, int64_t N, int64_t K, const void *A,
                        int64_t lda, const void *B, int64_t ldb, void *C, int64_t ldc) {
   int rc = iamx::ensure_init();
@@ -155,6 +160,15 @@ IAMX_API int iamx_gemm(int dtype, int precision, int slices, int64_t M, int64_t 
   (void)slices;
   if (M <= 0 || N <= 0 || K < 0) return iamx::set_error(IAMX_EINVAL, "iamx_gemm: bad shape %lld x %lld x %lld",
                                                         (long long)M, (long long)N, (long long)K);
+  if (precision == 1) {
+    if (dtype != 1) return iamx::set_error(IAMX_EINVAL, "iamx_gemm: split-bf16 needs float32");
+    return iamx_gemm_split_bf16(M, N, K, (const float *)A, lda, (const float *)B, ldb, (float *)C, ldc);
+  }
+  if (precision == 2) {
+    if (dtype != 0) return iamx::set_error(IAMX_EINVAL, "iamx_gemm: ozaki-int8 needs float64");
+    return iamx_gemm_ozaki_i8(M, N, K, (const double *)A, lda, (const double *)B, ldb, (double *)C, ldc,
+                              slices > 0 ? slices : 7);
+  }
   if (precision != 0) return iamx::set_error(IAMX_EINVAL, "iamx_gemm: precision %d not available", precision);
   const int64_t tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
   if (tiles_m * tiles_n > INT32_MAX) return iamx::set_error(IAMX_EINVAL, "iamx_gemm: too many tiles");

@@ -1,0 +1,373 @@
+// iamx_mixed.hip -- the two mixed-precision GEMM forms on CDNA4 matrix cores.
+//
+// Both run one "k-segmented" GEMM: C = sum over segments g of A_g . B_g^T,
+// where every operand segment is a row-major byte matrix (rows x Kb bytes,
+// K contiguous; B is stored transposed, N x Kb) and a segment table says
+// which A and B planes each K-range pairs up.  128x128 output tile per
+// workgroup, 4 wavefronts of 64x64 (4x4 MFMA 16x16 tiles), 128 operand bytes
+// of K per LDS stage (two MFMA k-steps), next stage prefetched to registers.
+//
+// SPLIT_BF16 (float32 operands): x = hi + lo with hi = bf16(x), lo = bf16(x-hi)
+//   (round to nearest even); segments (hi,hi), (hi,lo), (lo,hi) accumulate in
+//   one FP32 accumulator on v_mfma_f32_16x16x32_bf16.
+// OZAKI_INT8 (float64 operands): row i of A is scaled by 2^-e_i (e_i from the
+//   largest |A_ik|), column j of B by 2^-f_j, and each scaled value in (-1,1) is
+//   cut into S signed 7-bit digits d_1..d_S (truncation), so
+//   A_ik = 2^e_i sum_s d_s 2^-7s (+ residual).  Level L = s+t collects the
+//   slice products D_s E_t (s,t <= S, L <= S+1) on v_mfma_i32_16x16x64_i8 with
+//   exact INT32 sums (K*S*127^2 < 2^31 is checked); at the end of each level
+//   the INT32 tile is added into an FP64 accumulator with weight 2^-7L, and
+//   the epilogue applies 2^(e_i+f_j).  FP64 arithmetic appears only in that
+//   recombination and in the exponent scaling.
+#include <math.h>
+#include <string.h>
+
+#include "iamx_internal.h"
+
+namespace {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int TM = 128, TN = 128, KB = 128, ROWB = KB + 16;  // LDS row: 128 data bytes + 16 pad
+constexpr int MAX_SEGS = 64;
+
+struct Seg {
+  int64_t a_off;  // byte offset of the A plane
+  int64_t b_off;  // byte offset of the B plane
+  int32_t level;  // Ozaki level (s+t); 0 for bf16
+  int32_t level_end;
+};
+
+struct SegArgs {
+  const uint8_t *A;  // planes: row-major rows x Kb bytes, row stride lda bytes
+  const uint8_t *B;
+  int64_t lda, ldb;  // bytes
+  int64_t Kb;        // bytes of K per segment (multiple of KB)
+  int64_t M, N;      // logical output size
+  void *C;
+  int64_t ldc;       // elements
+  const int32_t *ea;  // Ozaki row exponents (M), column exponents (N)
+  const int32_t *fb;
+  int nseg;
+  int tiles_n;
+  Seg seg[MAX_SEGS];
+};
+
+// ---- bf16 split ------------------------------------------------------------------
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// Writes hi/lo planes (rows_p x cols_p, zero padded).  trans = 0: plane[r][c] =
+// X[r][c]; trans = 1: plane[r][c] = X[c][r] (B stored transposed).
+__global__ __launch_bounds__(256) void k_split_bf16(const float *__restrict__ X, int64_t xr, int64_t xc, int64_t ldx,
+                                                    int trans, uint16_t *__restrict__ hi, uint16_t *__restrict__ lo,
+                                                    int64_t rows_p, int64_t cols_p) {
+  __shared__ float t[32][33];
+  const int64_t r0 = (int64_t)blockIdx.y * 32, c0 = (int64_t)blockIdx.x * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  if (trans) {
+    // plane rows are X columns: stage X[c0.., r0..] through LDS
+    for (int i = ty; i < 32; i += 8) {
+      const int64_t xrow = c0 + i, xcol = r0 + tx;
+      t[i][tx] = (xrow < xr && xcol < xc) ? X[xrow * ldx + xcol] : 0.f;
+    }
+    __syncthreads();
+    for (int i = ty; i < 32; i += 8) {
+      const int64_t r = r0 + i, c = c0 + tx;
+      if (r < rows_p && c < cols_p) {
+        const float v = t[tx][i];
+        const uint16_t h = f2bf(v);
+        hi[r * cols_p + c] = h;
+        lo[r * cols_p + c] = f2bf(v - bf2f(h));
+      }
+    }
+  } else {
+    for (int i = ty; i < 32; i += 8) {
+      const int64_t r = r0 + i, c = c0 + tx;
+      if (r < rows_p && c < cols_p) {
+        const float v = (r < xr && c < xc) ? X[r * ldx + c] : 0.f;
+        const uint16_t h = f2bf(v);
+        hi[r * cols_p + c] = h;
+        lo[r * cols_p + c] = f2bf(v - bf2f(h));
+      }
+    }
+  }
+}
+
+// ---- Ozaki prep -------------------------------------------------------------------
+// exponent e with |x| * 2^-e < 1 for the largest |x| (0 for an all-zero line)
+__device__ __forceinline__ int exp_of(double m) { return m > 0.0 ? ilogb(m) + 1 : 0; }
+
+__global__ __launch_bounds__(256) void k_row_exp(const double *__restrict__ X, int64_t rows, int64_t cols,
+                                                 int64_t ld, int32_t *__restrict__ e) {
+  const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (r >= rows) return;
+  double m = 0.0;
+  for (int64_t c = lane; c < cols; c += 64) m = fmax(m, fabs(X[r * ld + c]));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if (lane == 0) e[r] = exp_of(m);
+}
+
+__global__ __launch_bounds__(256) void k_col_exp(const double *__restrict__ X, int64_t rows, int64_t cols,
+                                                 int64_t ld, int32_t *__restrict__ e) {
+  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= cols) return;
+  double m = 0.0;
+  for (int64_t r = 0; r < rows; r++) m = fmax(m, fabs(X[r * ld + c]));
+  e[c] = exp_of(m);
+}
+
+// S digit planes, each rows_p x cols_p int8.  trans as in k_split_bf16; the
+// scale exponent is per plane row (e[r]).
+__global__ __launch_bounds__(256) void k_slice_i8(const double *__restrict__ X, int64_t xr, int64_t xc, int64_t ldx,
+                                                  int trans, const int32_t *__restrict__ e, int S,
+                                                  int8_t *__restrict__ planes, int64_t rows_p, int64_t cols_p) {
+  __shared__ double t[32][33];
+  const int64_t r0 = (int64_t)blockIdx.y * 32, c0 = (int64_t)blockIdx.x * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  if (trans) {
+    for (int i = ty; i < 32; i += 8) {
+      const int64_t xrow = c0 + i, xcol = r0 + tx;
+      t[i][tx] = (xrow < xr && xcol < xc) ? X[xrow * ldx + xcol] : 0.0;
+    }
+    __syncthreads();
+  }
+  for (int i = ty; i < 32; i += 8) {
+    const int64_t r = r0 + i, c = c0 + tx;
+    if (r >= rows_p || c >= cols_p) continue;
+    double v;
+    if (trans) v = t[tx][i];
+    else v = (r < xr && c < xc) ? X[r * ldx + c] : 0.0;
+    const int er = r < (trans ? xc : xr) ? e[r] : 0;
+    double q = ldexp(v, -er);  // |q| < 1, exact
+    for (int s = 0; s < S; s++) {
+      q *= 128.0;                 // exact
+      const double d = trunc(q);  // |d| <= 127
+      q -= d;                     // exact
+      planes[(int64_t)s * rows_p * cols_p + r * cols_p + c] = (int8_t)d;
+    }
+  }
+}
+
+// ---- the segmented GEMM -----------------------------------------------------------------
+template <int MODE>
+struct Acc;
+template <>
+struct Acc<0> {
+  typedef f32x4 t;
+};
+template <>
+struct Acc<1> {
+  typedef i32x4 t;
+};
+
+template <int MODE>
+__device__ __forceinline__ typename Acc<MODE>::t mma(const i32x4 &a, const i32x4 &b, typename Acc<MODE>::t c);
+template <>
+__device__ __forceinline__ f32x4 mma<0>(const i32x4 &a, const i32x4 &b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                 0, 0);
+}
+template <>
+__device__ __forceinline__ i32x4 mma<1>(const i32x4 &a, const i32x4 &b, i32x4 c) {
+  return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_seg_gemm(const SegArgs P) {
+  __shared__ __attribute__((aligned(16))) uint8_t As[2][TM * ROWB];
+  __shared__ __attribute__((aligned(16))) uint8_t Bs[2][TN * ROWB];
+  typedef typename Acc<MODE>::t acc_t;
+  const int tm = blockIdx.x / P.tiles_n, tn = blockIdx.x % P.tiles_n;
+  const int64_t m0 = (int64_t)tm * TM, n0 = (int64_t)tn * TN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  // global -> LDS: 128 rows x 128 bytes per operand = 1024 x 16B; 4 per thread
+  int lr[4], lc[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int idx = tid + 256 * i;
+    lr[i] = idx >> 3;
+    lc[i] = (idx & 7) * 16;
+  }
+  i32x4 ra[4], rb[4];
+  const int64_t ktiles = P.Kb / KB;
+  const int64_t total = (int64_t)P.nseg * ktiles;
+  auto gload = [&](int64_t it) {
+    const Seg &s = P.seg[it / ktiles];
+    const int64_t kb = (it % ktiles) * KB;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      ra[i] = *reinterpret_cast<const i32x4 *>(P.A + s.a_off + (m0 + lr[i]) * P.lda + kb + lc[i]);
+      rb[i] = *reinterpret_cast<const i32x4 *>(P.B + s.b_off + (n0 + lr[i]) * P.ldb + kb + lc[i]);
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      *reinterpret_cast<i32x4 *>(&As[buf][lr[i] * ROWB + lc[i]]) = ra[i];
+      *reinterpret_cast<i32x4 *>(&Bs[buf][lr[i] * ROWB + lc[i]]) = rb[i];
+    }
+  };
+  acc_t acc[4][4];
+  double acc64[4][4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      acc[i][j] = acc_t{0, 0, 0, 0};
+      if (MODE == 1)
+#pragma unroll
+        for (int v = 0; v < 4; v++) acc64[i][j][v] = 0.0;
+    }
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  for (int64_t it = 0; it < total; it++) {
+    const int cur = (int)(it & 1);
+    if (it + 1 < total) gload(it + 1);
+#pragma unroll
+    for (int kk = 0; kk < KB; kk += 64) {
+      i32x4 a[4], b[4];
+      const int kbyte = kk + (lane >> 4) * 16;
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+        a[i] = *reinterpret_cast<const i32x4 *>(&As[cur][(wr * 64 + i * 16 + (lane & 15)) * ROWB + kbyte]);
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        b[j] = *reinterpret_cast<const i32x4 *>(&Bs[cur][(wc * 64 + j * 16 + (lane & 15)) * ROWB + kbyte]);
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = mma<MODE>(a[i], b[j], acc[i][j]);
+    }
+    if (MODE == 1) {
+      const Seg &s = P.seg[it / ktiles];
+      if (s.level_end && (it % ktiles) == ktiles - 1) {
+        const double w = ldexp(1.0, -7 * s.level);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) acc64[i][j][v] += w * (double)acc[i][j][v];
+            acc[i][j] = acc_t{0, 0, 0, 0};
+          }
+      }
+    }
+    if (it + 1 < total) sstore(cur ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int64_t col = n0 + wc * 64 + j * 16 + (lane & 15);
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int64_t row = m0 + wr * 64 + i * 16 + 4 * (lane >> 4) + v;
+        if (row < P.M && col < P.N) {
+          if (MODE == 0) reinterpret_cast<float *>(P.C)[row * P.ldc + col] = (float)acc[i][j][v];
+          else reinterpret_cast<double *>(P.C)[row * P.ldc + col] = ldexp(acc64[i][j][v], P.ea[row] + P.fb[col]);
+        }
+      }
+    }
+}
+
+int64_t rup(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
+
+}  // namespace
+
+extern "C" {
+
+// float32: C = A B through the bf16 hi/lo split (3 products).
+int iamx_gemm_split_bf16(int64_t M, int64_t N, int64_t K, const float *A, int64_t lda, const float *B, int64_t ldb,
+                         float *C, int64_t ldc) {
+  const int64_t Mp = rup(M, TM), Np = rup(N, TN), Kp = rup(K, KB / 2);
+  const size_t a_bytes = (size_t)(Mp * Kp * 2), b_bytes = (size_t)(Np * Kp * 2);
+  uint8_t *ws = (uint8_t *)iamx_workspace(1, 2 * a_bytes + 2 * b_bytes + 256);
+  if (!ws) return iamx::set_error(IAMX_ENOMEM, "split-bf16 workspace");
+  uint16_t *ahi = (uint16_t *)ws, *alo = (uint16_t *)(ws + a_bytes);
+  uint16_t *bhi = (uint16_t *)(ws + 2 * a_bytes), *blo = (uint16_t *)(ws + 2 * a_bytes + b_bytes);
+  IAMX_LAUNCH("split_bf16_A", k_split_bf16, dim3((unsigned)(Kp / 32), (unsigned)(Mp / 32)), dim3(256), 0, A, M, K,
+              lda, 0, ahi, alo, Mp, Kp);
+  IAMX_LAUNCH("split_bf16_B", k_split_bf16, dim3((unsigned)(Kp / 32), (unsigned)(Np / 32)), dim3(256), 0, B, K, N,
+              ldb, 1, bhi, blo, Np, Kp);
+  SegArgs P;
+  memset(&P, 0, sizeof(P));
+  P.A = ws;
+  P.B = ws + 2 * a_bytes;
+  P.lda = Kp * 2;
+  P.ldb = Kp * 2;
+  P.Kb = Kp * 2;
+  P.M = M;
+  P.N = N;
+  P.C = C;
+  P.ldc = ldc;
+  P.tiles_n = (int)(Np / TN);
+  P.nseg = 3;  // small products first: lo*hi, hi*lo, then hi*hi
+  P.seg[0] = Seg{(int64_t)a_bytes, 0, 0, 0};
+  P.seg[1] = Seg{0, (int64_t)b_bytes, 0, 0};
+  P.seg[2] = Seg{0, 0, 0, 0};
+  IAMX_LAUNCH("gemm_split_bf16_mfma", k_seg_gemm<0>, dim3((unsigned)((Mp / TM) * (Np / TN))), dim3(256), 0, P);
+  return IAMX_OK;
+}
+
+// float64: C = A B through Ozaki INT8 slicing with S slices.
+int iamx_gemm_ozaki_i8(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B, int64_t ldb,
+                       double *C, int64_t ldc, int S) {
+  if (S < 2 || S > 8) return iamx::set_error(IAMX_EINVAL, "ozaki: slices %d outside 2..8", S);
+  if ((double)K * S * 127.0 * 127.0 >= 2147483647.0)
+    return iamx::set_error(IAMX_EINVAL, "ozaki: K=%lld too large for exact INT32 level sums", (long long)K);
+  const int64_t Mp = rup(M, TM), Np = rup(N, TN), Kp = rup(K, KB);
+  const size_t a_bytes = (size_t)(S * Mp * Kp), b_bytes = (size_t)(S * Np * Kp);
+  uint8_t *ws = (uint8_t *)iamx_workspace(1, a_bytes + b_bytes + (size_t)(Mp + Np) * 4 + 256);
+  if (!ws) return iamx::set_error(IAMX_ENOMEM, "ozaki workspace");
+  int8_t *da = (int8_t *)ws, *db = (int8_t *)(ws + a_bytes);
+  int32_t *ea = (int32_t *)(ws + a_bytes + b_bytes), *fb = ea + Mp;
+  IAMX_LAUNCH("ozaki_row_exp", k_row_exp, dim3((unsigned)((M + 3) / 4)), dim3(256), 0, A, M, K, lda, ea);
+  IAMX_LAUNCH("ozaki_col_exp", k_col_exp, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, B, K, N, ldb, fb);
+  IAMX_LAUNCH("ozaki_slice_A", k_slice_i8, dim3((unsigned)(Kp / 32), (unsigned)(Mp / 32)), dim3(256), 0, A, M, K,
+              lda, 0, ea, S, da, Mp, Kp);
+  IAMX_LAUNCH("ozaki_slice_B", k_slice_i8, dim3((unsigned)(Kp / 32), (unsigned)(Np / 32)), dim3(256), 0, B, K, N,
+              ldb, 1, fb, S, db, Np, Kp);
+  SegArgs P;
+  memset(&P, 0, sizeof(P));
+  P.A = (const uint8_t *)da;
+  P.B = (const uint8_t *)db;
+  P.lda = Kp;
+  P.ldb = Kp;
+  P.Kb = Kp;
+  P.M = M;
+  P.N = N;
+  P.C = C;
+  P.ldc = ldc;
+  P.ea = ea;
+  P.fb = fb;
+  P.tiles_n = (int)(Np / TN);
+  int n = 0;
+  // smallest levels last would lose them in FP64 rounding order; add from the
+  // least significant level (S+1) up to the most significant (2)
+  for (int L = S + 1; L >= 2; L--) {
+    for (int s = 1; s <= S; s++) {
+      const int t = L - s;
+      if (t < 1 || t > S) continue;
+      P.seg[n++] = Seg{(int64_t)(s - 1) * Mp * Kp, (int64_t)(t - 1) * Np * Kp, L, 0};
+    }
+    P.seg[n - 1].level_end = 1;
+  }
+  P.nseg = n;
+  IAMX_LAUNCH("gemm_ozaki_i8_mfma", k_seg_gemm<1>, dim3((unsigned)((Mp / TM) * (Np / TN))), dim3(256), 0, P);
+  return IAMX_OK;
+}
+
+}  // extern "C"

@@ -88,8 +88,10 @@ static hipEvent_t get_event() {
   return e;
 }
 
+static bool g_capturing = false;
+
 ProfScope::ProfScope(const char *name) {
-  if (!g_prof_on) return;
+  if (!g_prof_on || g_capturing) return;
   int idx = -1;
   for (size_t i = 0; i < g_prof.size(); i++)
     if (g_prof[i].name == name) idx = (int)i;
@@ -265,6 +267,41 @@ IAMX_API int iamx_prof_entry(int i, const char **name, int64_t *launches, double
 IAMX_API double iamx_prof_span_ms(void) {
   prof_collect();
   return g_span_ms;
+}
+
+IAMX_API int iamx_capture_begin(void) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  prof_collect();
+  IAMX_HIP_CHECK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
+  g_capturing = true;
+  return IAMX_OK;
+}
+
+IAMX_API int iamx_capture_end(void **graph_exec) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  g_capturing = false;
+  hipGraph_t g = nullptr;
+  IAMX_HIP_CHECK(hipStreamEndCapture(g_stream, &g));
+  hipGraphExec_t ex = nullptr;
+  hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return check_hip(e, "hipGraphInstantiate");
+  *graph_exec = (void *)ex;
+  return IAMX_OK;
+}
+
+IAMX_API int iamx_graph_launch(void *graph_exec) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  ProfScope ps("graph_launch");
+  IAMX_HIP_CHECK(hipGraphLaunch((hipGraphExec_t)graph_exec, g_stream));
+  return IAMX_OK;
+}
+
+IAMX_API void iamx_graph_destroy(void *graph_exec) {
+  if (graph_exec && g_init_state == 1) (void)hipGraphExecDestroy((hipGraphExec_t)graph_exec);
 }
 
 IAMX_API int iamx_decode_status(void) {

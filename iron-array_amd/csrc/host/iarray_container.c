@@ -127,6 +127,8 @@ static void free_plan(iax_plan_t *p) {
   iamx_free(p->d_lz4);
   iamx_free(p->d_blocks);
   iamx_free(p->d_blk_rc);
+  free(p->h_lz4);
+  free(p->h_blocks);
   memset(p, 0, sizeof(*p));
 }
 
@@ -306,6 +308,8 @@ ina_rc_t iax_ensure_device(iarray_container_t *c) {
   free(host);
   if (rc) return iax_dev_error(rc, "upload chunks");
   c->device_valid = 1;
+  static uint64_t next_generation = 1;
+  c->generation = next_generation++;
   return INA_SUCCESS;
 }
 
@@ -521,7 +525,9 @@ ina_rc_t iax_ensure_plan(iarray_container_t *c) {
   }
   if (INA_SUCCEED(rc)) rc = upload_plan(&jobs, &c->plan);
   c->plan.nbcols = c->nchunks_ax[1] * c->nblocks_ax[1];
-  free_jobs(&jobs);
+  c->plan.h_lz4 = jobs.lz4;
+  c->plan.h_blocks = jobs.blk;
+  free(jobs.rc4);
   if (INA_FAILED(rc)) {
     free_plan(&c->plan);
     return rc;

@@ -169,14 +169,3 @@ INA_API(ina_rc_t) iarray_opt_gemv(iarray_context_t *ctx, iarray_container_t *a, 
     return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_BLOCKSHAPE), "a->blockshape[1] != b->blockshape[0]");
   return matmul_impl(ctx, a, b, storage, NULL, c);
 }
-
-INA_API(ina_rc_t) iarray_linalg_matmul_batch(iarray_context_t *ctx, int n, iarray_container_t **a,
-                                             iarray_container_t **b, iarray_storage_t *storage,
-                                             const iarray_matmul_params_t *params, iarray_container_t **c) {
-  if (!ctx || !a || !b || !storage || !c || n < 0) return INA_ERROR(INA_ERR_NULL_POINTER);
-  for (int i = 0; i < n; i++) {
-    ina_rc_t rc = matmul_impl(ctx, a[i], b[i], &storage[i], params, &c[i]);
-    if (INA_FAILED(rc)) return rc;
-  }
-  return INA_SUCCESS;
-}

@@ -30,7 +30,9 @@ typedef struct iax_plan_s {
   int32_t *d_blk_rc; /* (row0, col0, block column, 0) per block job, for the fused GEMV */
   int64_t scratch_bytes;
   int64_t nbcols; /* block columns of the global block grid */
-  int all_zero;   /* every block is a zero block */
+  /* host copies of the job tables (the batch path concatenates them) */
+  iamx_lz4_job_t *h_lz4;
+  iamx_block_job_t *h_blocks;
 } iax_plan_t;
 
 struct iarray_container_s {
@@ -52,6 +54,7 @@ struct iarray_container_s {
   uint8_t *d_arena;
   int64_t *arena_off;
   int device_valid;
+  uint64_t generation; /* bumped whenever the device arena is (re)built */
   iax_plan_t plan;
   int plan_valid;
   /* views */

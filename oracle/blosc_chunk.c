@@ -134,16 +134,30 @@ void orc_bitshuffle(int ts, int32_t bsize, const uint8_t *src, uint8_t *dst, int
   memcpy(dst + (int64_t)n * ts, src + (int64_t)n * ts, (size_t)(bsize - n * ts));
 }
 
+/* 8x8 bit-matrix transpose (row r = byte r): bit 8r+c -> bit 8c+r. */
+static uint64_t t8(uint64_t x) {
+  uint64_t t;
+  t = (x ^ (x >> 7)) & 0x00AA00AA00AA00AAull;
+  x = x ^ t ^ (t << 7);
+  t = (x ^ (x >> 14)) & 0x0000CCCC0000CCCCull;
+  x = x ^ t ^ (t << 14);
+  t = (x ^ (x >> 28)) & 0x00000000F0F0F0F0ull;
+  x = x ^ t ^ (t << 28);
+  return x;
+}
+
 void orc_bitunshuffle(int ts, int32_t bsize, const uint8_t *src, uint8_t *dst, int fv) {
   int32_t n = bit_elems(ts, bsize, fv);
   if (n < 0) { memcpy(dst, src, (size_t)bsize); return; }
   const int32_t rowlen = n / 8;
-  for (int32_t e = 0; e < n; e++)
+  /* group g = elements 8g..8g+7: gathering byte g of the 8 bit planes of byte
+   * j gives an 8x8 bit matrix whose transpose holds byte j of each element */
+  for (int32_t g = 0; g < rowlen; g++)
     for (int j = 0; j < ts; j++) {
-      uint8_t v = 0;
-      for (int k = 0; k < 8; k++)
-        v |= (uint8_t)(((src[(int64_t)(8 * j + k) * rowlen + e / 8] >> (e % 8)) & 1u) << k);
-      dst[(int64_t)e * ts + j] = v;
+      uint64_t x = 0;
+      for (int k = 0; k < 8; k++) x |= (uint64_t)src[(int64_t)(8 * j + k) * rowlen + g] << (8 * k);
+      const uint64_t y = t8(x);
+      for (int i = 0; i < 8; i++) dst[(int64_t)(8 * g + i) * ts + j] = (uint8_t)(y >> (8 * i));
     }
   memcpy(dst + (int64_t)n * ts, src + (int64_t)n * ts, (size_t)(bsize - n * ts));
 }

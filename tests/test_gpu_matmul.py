@@ -174,3 +174,26 @@ def test_matmul_deterministic(ctx):
     z2 = ia.matmul(ca, cb, (200, 175), (50, 25)).to_numpy()
     assert z1.tobytes() == z2.tobytes()
     assert np.allclose(z1, a @ b, rtol=1e-12, atol=1e-12)
+
+
+def test_batch_ragged_matches_per_product(ctx):
+    """iarray_linalg_matmul_batch (one decode launch, one work-queue GEMM,
+    one pack launch, replayed from a HIP graph) against numpy on integer
+    data (exact), with ragged shapes, ragged edge blocks and all-padding
+    blocks; the second call replays the captured graph."""
+    rng = np.random.default_rng(55)
+    pairs, outs, cs_list, bs_list = [], [], [], []
+    for _ in range(12):
+        M, K, N = (int(v) for v in rng.integers(64, 700, 3))
+        a = rng.integers(-6, 7, (M, K)).astype(np.float32)
+        b = rng.integers(-6, 7, (K, N)).astype(np.float32)
+        ca = ia.Container.from_numpy(ctx, a, (min(M, 512), min(K, 512)), (min(M, 128), min(K, 128)))
+        cb = ia.Container.from_numpy(ctx, b, (min(K, 512), min(N, 512)), (min(K, 128), min(N, 128)))
+        pairs.append((ca, cb))
+        outs.append(a.astype(np.float64) @ b.astype(np.float64))
+        cs_list.append((min(M, 512), min(N, 512)))
+        bs_list.append((min(M, 128), min(N, 128)))
+    for rep in range(3):
+        cs = ia.matmul_batch(pairs, cs_list, bs_list)
+        for c, want in zip(cs, outs):
+            assert np.array_equal(c.to_numpy(), want.astype(np.float32)), f"replay {rep}"

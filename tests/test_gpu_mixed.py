@@ -1,0 +1,97 @@
+"""GPU accuracy of the mixed-precision matmul forms against an FP64 oracle.
+
+The oracle is a double-double (compensated) dot product per output element
+(oracle/tiling_linalg.c orc_dd_gemm_sample), i.e. the exact product rounded
+once.  Metrics per result C against C*:
+  componentwise  max_ij |C - C*|_ij / (|A||B|)_ij
+  normwise       ||C - C*||_F / ||C*||_F
+Bars (written here, derived in DESIGN.md):
+  split-bf16 (float32): componentwise <= 6e-5 (3 bf16 products drop lo*lo ~
+    2^-18 and accumulate in FP32 over K), and never worse than 4x the plain
+    FP32 GPU result on the same inputs plus that bound;
+  ozaki-int8 (float64, values +-10^U[-3,3]): componentwise <= 2e-13 with 7
+    slices and <= 2e-15 with 8 slices.
+"""
+import numpy as np
+import pytest
+
+import pyiarray as ia
+
+pytestmark = pytest.mark.gpu
+
+
+def _metrics(orc, a, b, c):
+    a64, b64 = a.astype(np.float64), b.astype(np.float64)
+    M, N = a.shape[0], b.shape[1]
+    rows, cols = np.meshgrid(np.arange(M), np.arange(N), indexing="ij")
+    ref = orc.dd_gemm_sample(a64, b64, rows.ravel(), cols.ravel()).reshape(M, N)
+    den = np.abs(a64) @ np.abs(b64)
+    err = np.abs(c.astype(np.float64) - ref)
+    comp = float(np.max(err / np.where(den > 0, den, 1)))
+    norm = float(np.linalg.norm(err) / np.linalg.norm(ref))
+    return comp, norm
+
+
+@pytest.mark.parametrize("shape", [(256, 384, 320), (300, 700, 129), (128, 1024, 128)])
+def test_split_bf16_accuracy(orc, shape):
+    M, K, N = shape
+    print(f"case: split-bf16 {shape}", flush=True)
+    ctx = ia.Context(clevel=3, filters=ia.COMP_SHUFFLE)
+    rng = np.random.default_rng(M + K + N)
+    a = rng.standard_normal((M, K)).astype(np.float32)
+    b = rng.standard_normal((K, N)).astype(np.float32)
+    ca = ia.Container.from_numpy(ctx, a, (min(M, 128), min(K, 256)), (64, 64))
+    cb = ia.Container.from_numpy(ctx, b, (min(K, 256), min(N, 128)), (64, 64))
+    cs, bs = (min(M, 128), min(N, 128)), (64, 64)
+    mixed = ia.matmul(ca, cb, cs, bs, precision=ia.SPLIT_BF16).to_numpy()
+    plain = ia.matmul(ca, cb, cs, bs).to_numpy()
+    cm, nm = _metrics(orc, a, b, mixed)
+    cp, npl = _metrics(orc, a, b, plain)
+    print(f"split-bf16 {shape}: componentwise {cm:.3e} normwise {nm:.3e} | plain fp32 {cp:.3e} {npl:.3e}")
+    assert cm <= 6e-5
+    assert cm <= 4 * cp + 6e-5
+
+
+def _wide(rng, shape):
+    return np.where(rng.random(shape) < 0.5, -1.0, 1.0) * 10.0 ** rng.uniform(-3, 3, shape)
+
+
+@pytest.mark.parametrize("slices,bar", [(7, 2e-13), (8, 2e-15)])
+@pytest.mark.parametrize("shape", [(256, 500, 192), (129, 260, 300)])
+def test_ozaki_int8_accuracy(orc, shape, slices, bar):
+    M, K, N = shape
+    print(f"case: ozaki {shape} S={slices}", flush=True)
+    ctx = ia.Context(clevel=5, filters=ia.COMP_SHUFFLE)
+    rng = np.random.default_rng(M * 3 + K)
+    a = _wide(rng, (M, K))
+    b = _wide(rng, (K, N))
+    ca = ia.Container.from_numpy(ctx, a, (min(M, 128), min(K, 250)), (64, 50))
+    cb = ia.Container.from_numpy(ctx, b, (min(K, 250), min(N, 128)), (50, 64))
+    c = ia.matmul(ca, cb, (min(M, 128), min(N, 128)), (64, 64), precision=ia.OZAKI_INT8, slices=slices).to_numpy()
+    cm, nm = _metrics(orc, a, b, c)
+    print(f"ozaki S={slices} {shape}: componentwise {cm:.3e} normwise {nm:.3e}")
+    assert cm <= bar
+
+
+def test_ozaki_exact_on_small_integers(orc):
+    """Integers below 2^7 are one slice: every form must be exact."""
+    ctx = ia.Context(clevel=5, filters=ia.COMP_SHUFFLE)
+    rng = np.random.default_rng(9)
+    a = rng.integers(-100, 101, (200, 300)).astype(np.float64)
+    b = rng.integers(-100, 101, (300, 150)).astype(np.float64)
+    ca = ia.Container.from_numpy(ctx, a, (100, 100), (50, 50))
+    cb = ia.Container.from_numpy(ctx, b, (100, 75), (50, 25))
+    c = ia.matmul(ca, cb, (100, 75), (50, 25), precision=ia.OZAKI_INT8, slices=3).to_numpy()
+    assert np.array_equal(c, a @ b)
+
+
+def test_mixed_dtype_errors():
+    ctx = ia.Context(clevel=5, filters=ia.COMP_SHUFFLE)
+    a = ia.Container.arange(ctx, np.float64, (64, 64), (32, 32), (16, 16))
+    f = ia.Container.arange(ctx, np.float32, (64, 64), (32, 32), (16, 16))
+    with pytest.raises(ia.IArrayError) as e:
+        ia.matmul(a, a, (32, 32), (16, 16), precision=ia.SPLIT_BF16)
+    assert e.value.rc == ia.ERR_INVALID_DTYPE
+    with pytest.raises(ia.IArrayError) as e:
+        ia.matmul(f, f, (32, 32), (16, 16), precision=ia.OZAKI_INT8)
+    assert e.value.rc == ia.ERR_INVALID_DTYPE
