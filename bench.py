@@ -35,7 +35,9 @@ def parse_args():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", default="gemv20k", choices=["gemv20k", "gemm2k_f64", "gemm8k_f32", "gemm6k_f64"])
+    p.add_argument("--workload", default="gemv20k",
+                   choices=["gemv20k", "gemm2k_f64", "gemm8k_f32", "gemm6k_f64", "batch512"])
+    p.add_argument("--slices", type=int, default=0, help="Ozaki slice count (0 = library default)")
     p.add_argument("--precision", default="plain", choices=["plain", "split_bf16", "ozaki_int8"])
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-threads", type=int, default=0)
@@ -98,6 +100,29 @@ def main():
                   "M": M, "K": K, "chunkshape": [2000, 2000], "blockshape": [250, 250], "codec": "lz4",
                   "clevel": 5, "filter": "bitshuffle"}
         step = lambda: ia.matmul(cA, cx, out_chunks, out_blocks)  # noqa: E731
+    elif args.workload == "batch512":
+        ctx = ia.Context(clevel=5, filters=ia.COMP_SHUFFLE)
+        dims = rng.integers(64, 1537, (512, 3))
+        pairs, cs_l, bs_l = [], [], []
+        flops = 0.0
+        for M_, N_, K_ in dims:
+            M_, N_, K_ = int(M_), int(N_), int(K_)
+            a = rng.uniform(-1, 1, (M_, K_)).astype(np.float32)
+            b = rng.uniform(-1, 1, (K_, N_)).astype(np.float32)
+            ca_ = ia.Container.from_numpy(ctx, a, (min(M_, 512), min(K_, 512)), (min(M_, 512, 128), min(K_, 128)))
+            cb_ = ia.Container.from_numpy(ctx, b, (min(K_, 512), min(N_, 512)), (min(K_, 128), min(N_, 128)))
+            pairs.append((ca_, cb_))
+            cs_l.append((min(M_, 512), min(N_, 512)))
+            bs_l.append((min(M_, 128), min(N_, 128)))
+            flops += 2.0 * M_ * N_ * K_
+        cA, cx = pairs[0]
+        step = lambda: ia.matmul_batch(pairs, cs_l, bs_l)  # noqa: E731
+        M = K = 0
+        dtype = "f32"
+        config = {"workload": "512 independent f32 matmuls, M,N,K ~ U{64..1536} (seed 1001), chunks min(dim,512), "
+                              "blocks 128x128 with ragged edges, LZ4 clevel5 + shuffle, one graph replay per step",
+                  "products": 512, "total_flop": flops}
+        args.no_cpu = True
     else:
         spec = {"gemm2k_f64": (2000, np.float64, (500, 500), (125, 125), 5),
                 "gemm8k_f32": (8192, np.float32, (1024, 1024), (256, 256), 3),
@@ -113,11 +138,13 @@ def main():
         else:
             a = np.where(rng.random((n, n)) < 0.5, -1.0, 1.0) * 10.0 ** rng.uniform(-3, 3, (n, n))
             b = np.where(rng.random((n, n)) < 0.5, -1.0, 1.0) * 10.0 ** rng.uniform(-3, 3, (n, n))
-        cA = ia.Container.from_numpy(ctx, a.astype(dt), cs, bs)
-        cx = ia.Container.from_numpy(ctx, b.astype(dt), cs, bs)
-        del a, b
+        a = a.astype(dt)
+        b = b.astype(dt)
+        cA = ia.Container.from_numpy(ctx, a, cs, bs)
+        cx = ia.Container.from_numpy(ctx, b, cs, bs)
         prec = {"plain": ia.PLAIN, "split_bf16": ia.SPLIT_BF16, "ozaki_int8": ia.OZAKI_INT8}[args.precision]
-        step = lambda: ia.matmul(cA, cx, cs, bs, precision=prec)  # noqa: E731
+        step = lambda: ia.matmul(cA, cx, cs, bs, precision=prec, slices=args.slices)  # noqa: E731
+        err_sample = (a, b)
         M = K = n
         flops = 2.0 * n ** 3
         dtype = "f64" if dt == np.float64 else "f32"
@@ -172,17 +199,42 @@ def main():
             "frac": round(achieved / peak, 4), "traffic": None, "algorithmic_bytes_or_flops_per_launch": alg,
             "avg_launch_ms": round(avg_s * 1e3, 4)}
 
+    errors = None
+    if args.workload.startswith("gemm"):
+        errors = error_metrics(c, *err_sample)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args, ia, cA, cx, config, flops)
     out = {"metric": METRIC, "value": round(flops / (ms / 1e3) / 1e9, 3), "unit": "GFLOP/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic (seeded numpy generator)",
-           "config": config, "roofline": roof, "cpu_baseline": cpu,
+           "config": config, "roofline": roof, "cpu_baseline": cpu, "errors_vs_fp64_oracle": errors,
            "compressed_bytes_A": cb, "decoded_bytes_A": nb,
            "host_encode_ms_last_step": round(enc_s * 1e3, 3),
            "compressed_GBps": round(cb / (ms / 1e3) / 1e9, 2), "decoded_GBps": round(nb / (ms / 1e3) / 1e9, 2)}
     print(json.dumps(out), flush=True)
+
+
+def error_metrics(c, a, b, nsample=4096):
+    """Errors of C against a double-double oracle on a seeded sample of entries:
+    elementwise max |C-C*|/|C*|, componentwise max |C-C*|/(|A||B|), and the
+    sample's normwise ||C-C*||/||C*||."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import pyoracle as orc
+
+    rng = np.random.default_rng(7)
+    M, N = a.shape[0], b.shape[1]
+    rows = rng.integers(0, M, nsample)
+    cols = rng.integers(0, N, nsample)
+    a64, b64 = a.astype(np.float64), b.astype(np.float64)
+    ref = orc.dd_gemm_sample(a64, b64, rows, cols)
+    full = c.to_numpy().astype(np.float64)
+    got = full[rows, cols]
+    den = np.einsum("ij,ji->i", np.abs(a64[rows]), np.abs(b64[:, cols]))
+    err = np.abs(got - ref)
+    return {"samples": nsample, "elementwise_max": float(np.max(err / np.maximum(np.abs(ref), 1e-300))),
+            "componentwise_max": float(np.max(err / den)),
+            "normwise": float(np.linalg.norm(err) / np.linalg.norm(ref))}
 
 
 def cpu_baseline(args, ia, cA, cB, config, flops):

@@ -25,6 +25,7 @@ struct MfmaB<double> {
   static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lane, int v) { return 4 * v + (lane >> 4); }
 };
 template <>
 struct MfmaB<float> {
@@ -32,6 +33,7 @@ struct MfmaB<float> {
   static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
 __global__ void k_zero_i32(int *p, int n) {
@@ -121,7 +123,7 @@ __global__ __launch_bounds__(256) void k_gemm_batch(const iamx_batch_prod_t *__r
         const int64_t col = n0 + wc * 64 + j * 16 + (lane & 15);
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-          const int64_t row = m0 + wr * 64 + i * 16 + 4 * (lane >> 4) + v;
+          const int64_t row = m0 + wr * 64 + i * 16 + MfmaB<T>::row(lane, v);
           if (row < M && col < N) C[row * N + col] = acc[i][j][v];
         }
       }

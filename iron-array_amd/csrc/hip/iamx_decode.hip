@@ -41,8 +41,8 @@ __global__ __launch_bounds__(64) void k_lz4_streams(const iamx_lz4_job_t *__rest
   const int j = blockIdx.x;
   if (j >= njobs) return;
   const iamx_lz4_job_t job = jobs[j];
-  const uint8_t *__restrict__ src = reinterpret_cast<const uint8_t *>(job.src);
-  uint8_t *dst = scratch + job.dst_off;
+  const IAMX_GLOBAL uint8_t *__restrict__ src = iamx::gptr<const uint8_t>(job.src);
+  IAMX_GLOBAL uint8_t *dst = iamx::gptr<uint8_t>((uint64_t)(scratch + job.dst_off));
   const int lane = threadIdx.x;
   const int32_t iend = job.csize, oend = job.nbytes;
   int32_t ip = 0, op = 0, fenced = 0;
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(64) void k_lz4_streams(const iamx_lz4_job_t *__rest
 }
 
 struct Planes {
-  const uint8_t *ptr[IAMX_MAX_STREAMS];
+  const IAMX_GLOBAL uint8_t *ptr[IAMX_MAX_STREAMS];
   uint8_t fill[IAMX_MAX_STREAMS];
   uint8_t is_const[IAMX_MAX_STREAMS];
 };
@@ -141,8 +141,8 @@ __global__ __launch_bounds__(256) void k_unfilter(const iamx_block_job_t *__rest
     for (int s = 0; s < IAMX_MAX_STREAMS; s++) {
       P.is_const[s] = (s >= J.nstreams) || J.kind[s] == IAMX_SRC_CONST;
       P.fill[s] = s < J.nstreams ? J.fill[s] : 0;
-      P.ptr[s] = J.kind[s] == IAMX_SRC_ADDR      ? reinterpret_cast<const uint8_t *>(J.src[s])
-                 : J.kind[s] == IAMX_SRC_SCRATCH ? scratch + J.src[s]
+      P.ptr[s] = J.kind[s] == IAMX_SRC_ADDR      ? iamx::gptr<const uint8_t>(J.src[s])
+                 : J.kind[s] == IAMX_SRC_SCRATCH ? iamx::gptr<const uint8_t>((uint64_t)(scratch + J.src[s]))
                                                  : nullptr;
     }
   }
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(256) void k_unfilter(const iamx_block_job_t *__rest
   }
   // byte shuffle (planes) or no filter
   const bool shuffled = J.filter == IAMX_FILTER_SHUFFLE && ts > 1;
-  const uint8_t *plane[8];
+  const IAMX_GLOBAL uint8_t *plane[8];
   uint8_t pconst[8], pfill[8];
   if (shuffled) {
     for (int j = 0; j < ts; j++) {

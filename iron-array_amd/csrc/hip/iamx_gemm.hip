@@ -8,7 +8,8 @@
 // while the current one feeds the matrix cores (one barrier per slab).
 //   float64: v_mfma_f64_16x16x4_f64   float32: v_mfma_f32_16x16x4_f32
 // MFMA operand layout (16x16x4, one block): lane l supplies A[l%16][l/16] and
-// B[l/16][l%16]; it receives D[4*(l/16)+v][l%16], v = 0..3.
+// B[l/16][l%16]; result register v holds D[4*(l/16)+v][l%16] for float32 and
+// D[4*v+l/16][l%16] for float64 (measured with a one-hot probe on gfx950).
 #include "iamx_internal.h"
 
 namespace {
@@ -24,6 +25,8 @@ struct Mfma<double> {
   static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  // result register v of lane l holds row 4*v + l/16 (column l%16)
+  static __device__ __forceinline__ int row(int lane, int v) { return 4 * v + (lane >> 4); }
 };
 template <>
 struct Mfma<float> {
@@ -31,6 +34,8 @@ struct Mfma<float> {
   static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  // result register v of lane l holds row 4*(l/16) + v (column l%16)
+  static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
 constexpr int BM = 128, BN = 128, BK = 16, PAD = 4;
@@ -123,7 +128,7 @@ __global__ __launch_bounds__(256) void k_gemm(int64_t M, int64_t N, int64_t K, c
       const int64_t col = n0 + wc * 64 + j * 16 + (lane & 15);
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int64_t row = m0 + wr * 64 + i * 16 + 4 * (lane >> 4) + v;
+        const int64_t row = m0 + wr * 64 + i * 16 + Mfma<T>::row(lane, v);
         if (row < M && col < N) C[row * ldc + col] = acc[i][j][v];
       }
     }

@@ -25,6 +25,15 @@ struct ProfScope {
   ~ProfScope();
 };
 
+// Device addresses held in job tables are plain integers; reading them through
+// a global-address-space pointer lets the compiler emit global_load/store
+// instead of flat accesses.
+#define IAMX_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ IAMX_GLOBAL T *gptr(uint64_t a) {
+  return (IAMX_GLOBAL T *)a;
+}
+
 }  // namespace iamx
 
 #define IAMX_HIP_CHECK(expr)                                   \

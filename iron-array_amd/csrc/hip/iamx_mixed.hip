@@ -10,12 +10,15 @@
 // SPLIT_BF16 (float32 operands): x = hi + lo with hi = bf16(x), lo = bf16(x-hi)
 //   (round to nearest even); segments (hi,hi), (hi,lo), (lo,hi) accumulate in
 //   one FP32 accumulator on v_mfma_f32_16x16x32_bf16.
-// OZAKI_INT8 (float64 operands): row i of A is scaled by 2^-e_i (e_i from the
-//   largest |A_ik|), column j of B by 2^-f_j, and each scaled value in (-1,1) is
-//   cut into S signed 7-bit digits d_1..d_S (truncation), so
-//   A_ik = 2^e_i sum_s d_s 2^-7s (+ residual).  Level L = s+t collects the
-//   slice products D_s E_t (s,t <= S, L <= S+1) on v_mfma_i32_16x16x64_i8 with
-//   exact INT32 sums (K*S*127^2 < 2^31 is checked); at the end of each level
+// OZAKI_INT8 (float64 operands): row i of A is scaled by 2^-e_i (e_i =
+//   ilogb(max_k |A_ik|) + 2, so every scaled value lies in (-1/2, 1/2)), column j
+//   of B by 2^-f_j, and each scaled value is cut into S balanced base-128 digits
+//   d_s = rint(128 q), q <- 128 q - d_s (|d_s| <= 64, |q| <= 1/2, all exact in
+//   FP64), so A_ik = 2^e_i (sum_s d_s 2^-7s + q_S 2^-7S) with |q_S| <= 1/2.
+//   Rounding to nearest keeps the dropped residual unbiased.  Level L = s+t
+//   collects the slice products D_s E_t (s,t <= S, L <= S+1) on
+//   v_mfma_i32_16x16x64_i8 with exact INT32 sums (K*S*64^2 < 2^31 is checked);
+//   at the end of each level
 //   the INT32 tile is added into an FP64 accumulator with weight 2^-7L, and
 //   the epilogue applies 2^(e_i+f_j).  FP64 arithmetic appears only in that
 //   recombination and in the exponent scaling.
@@ -102,8 +105,8 @@ __global__ __launch_bounds__(256) void k_split_bf16(const float *__restrict__ X,
 }
 
 // ---- Ozaki prep -------------------------------------------------------------------
-// exponent e with |x| * 2^-e < 1 for the largest |x| (0 for an all-zero line)
-__device__ __forceinline__ int exp_of(double m) { return m > 0.0 ? ilogb(m) + 1 : 0; }
+// exponent e with |x| * 2^-e < 1/2 for the largest |x| (0 for an all-zero line)
+__device__ __forceinline__ int exp_of(double m) { return m > 0.0 ? ilogb(m) + 2 : 0; }
 
 __global__ __launch_bounds__(256) void k_row_exp(const double *__restrict__ X, int64_t rows, int64_t cols,
                                                  int64_t ld, int32_t *__restrict__ e) {
@@ -148,11 +151,11 @@ __global__ __launch_bounds__(256) void k_slice_i8(const double *__restrict__ X, 
     if (trans) v = t[tx][i];
     else v = (r < xr && c < xc) ? X[r * ldx + c] : 0.0;
     const int er = r < (trans ? xc : xr) ? e[r] : 0;
-    double q = ldexp(v, -er);  // |q| < 1, exact
+    double q = ldexp(v, -er);  // |q| < 1/2, exact
     for (int s = 0; s < S; s++) {
-      q *= 128.0;                 // exact
-      const double d = trunc(q);  // |d| <= 127
-      q -= d;                     // exact
+      q *= 128.0;                // exact
+      const double d = rint(q);  // |d| <= 64
+      q -= d;                    // exact, |q| <= 1/2
       planes[(int64_t)s * rows_p * cols_p + r * cols_p + c] = (int8_t)d;
     }
   }
@@ -326,7 +329,7 @@ int iamx_gemm_split_bf16(int64_t M, int64_t N, int64_t K, const float *A, int64_
 int iamx_gemm_ozaki_i8(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B, int64_t ldb,
                        double *C, int64_t ldc, int S) {
   if (S < 2 || S > 8) return iamx::set_error(IAMX_EINVAL, "ozaki: slices %d outside 2..8", S);
-  if ((double)K * S * 127.0 * 127.0 >= 2147483647.0)
+  if ((double)K * S * 64.0 * 64.0 >= 2147483647.0)
     return iamx::set_error(IAMX_EINVAL, "ozaki: K=%lld too large for exact INT32 level sums", (long long)K);
   const int64_t Mp = rup(M, TM), Np = rup(N, TN), Kp = rup(K, KB);
   const size_t a_bytes = (size_t)(S * Mp * Kp), b_bytes = (size_t)(S * Np * Kp);

@@ -9,8 +9,11 @@ Bars (written here, derived in DESIGN.md):
   split-bf16 (float32): componentwise <= 6e-5 (3 bf16 products drop lo*lo ~
     2^-18 and accumulate in FP32 over K), and never worse than 4x the plain
     FP32 GPU result on the same inputs plus that bound;
-  ozaki-int8 (float64, values +-10^U[-3,3]): componentwise <= 2e-13 with 7
-    slices and <= 2e-15 with 8 slices.
+  ozaki-int8 (float64, values +-10^U[-3,3]): every entry within the a-priori
+    bound of the slicing -- residual |a - a~| <= 2^(e_i-7S-1) per operand
+    (balanced digits, e_i = ilogb(max_k|A_ik|)+2), dropped levels s+t > S+1
+    at most 1.01 (S-1) 2^(e_i+f_j-7S-2) per product, plus 2S ulp of FP64
+    recombination -- and componentwise <= 1e-12 (S=7), <= 1e-14 (S=8).
 """
 import numpy as np
 import pytest
@@ -56,7 +59,18 @@ def _wide(rng, shape):
     return np.where(rng.random(shape) < 0.5, -1.0, 1.0) * 10.0 ** rng.uniform(-3, 3, shape)
 
 
-@pytest.mark.parametrize("slices,bar", [(7, 2e-13), (8, 2e-15)])
+def _ozaki_bound(a, b, S):
+    ea = np.frexp(np.abs(a).max(axis=1))[1] + 1  # ilogb(max)+2
+    fb = np.frexp(np.abs(b).max(axis=0))[1] + 1
+    K = a.shape[1]
+    ra = np.ldexp(1.0, ea - 7 * S - 1)[:, None]
+    rb = np.ldexp(1.0, fb - 7 * S - 1)[None, :]
+    trunc = ra * np.abs(b).sum(axis=0)[None, :] + (np.abs(a).sum(axis=1)[:, None] + K * ra) * rb
+    dropped = 1.01 * (S - 1) * K * np.ldexp(1.0, (ea[:, None] + fb[None, :]) - 7 * S - 2)
+    return trunc + dropped + 2 * S * 2.0 ** -52 * (np.abs(a) @ np.abs(b))
+
+
+@pytest.mark.parametrize("slices,bar", [(7, 1e-12), (8, 1e-14)])
 @pytest.mark.parametrize("shape", [(256, 500, 192), (129, 260, 300)])
 def test_ozaki_int8_accuracy(orc, shape, slices, bar):
     M, K, N = shape
@@ -70,6 +84,9 @@ def test_ozaki_int8_accuracy(orc, shape, slices, bar):
     c = ia.matmul(ca, cb, (min(M, 128), min(N, 128)), (64, 64), precision=ia.OZAKI_INT8, slices=slices).to_numpy()
     cm, nm = _metrics(orc, a, b, c)
     print(f"ozaki S={slices} {shape}: componentwise {cm:.3e} normwise {nm:.3e}")
+    rows, cols = np.meshgrid(np.arange(M), np.arange(N), indexing="ij")
+    ref = orc.dd_gemm_sample(a, b, rows.ravel(), cols.ravel()).reshape(M, N)
+    assert np.all(np.abs(c - ref) <= _ozaki_bound(a, b, slices))
     assert cm <= bar
 
 
