@@ -36,7 +36,7 @@ def parse_args():
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--workload", default="gemv20k",
-                   choices=["gemv20k", "gemm2k_f64", "gemm8k_f32", "gemm6k_f64", "batch512"])
+                   choices=["gemv20k", "gemm2k_f64", "gemm8k_f32", "gemm6k_f64", "batch512", "hbm_copy"])
     p.add_argument("--slices", type=int, default=0, help="Ozaki slice count (0 = library default)")
     p.add_argument("--precision", default="plain", choices=["plain", "split_bf16", "ozaki_int8"])
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -76,6 +76,9 @@ def main():
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
               "the sharding design is described in DESIGN.md", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "hbm_copy":
+        hbm_copy(args)
+        return
     import pyiarray as ia
 
     name, cus, arch = ia.device_info()
@@ -196,8 +199,8 @@ def main():
         alg = nb
         bound, unit, peak, achieved = "hbm", "GB/s", HBM_PEAK_GBS, alg / avg_s / 1e9
     roof = {"bound": bound, "kernel": dname, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
-            "frac": round(achieved / peak, 4), "traffic": None, "algorithmic_bytes_or_flops_per_launch": alg,
-            "avg_launch_ms": round(avg_s * 1e3, 4)}
+            "frac": round(achieved / peak, 4), "traffic": measured_traffic(args, dname),
+            "algorithmic_bytes_or_flops_per_launch": alg, "avg_launch_ms": round(avg_s * 1e3, 4)}
 
     errors = None
     if args.workload.startswith("gemm"):
@@ -213,6 +216,42 @@ def main():
            "host_encode_ms_last_step": round(enc_s * 1e3, 3),
            "compressed_GBps": round(cb / (ms / 1e3) / 1e9, 2), "decoded_GBps": round(nb / (ms / 1e3) / 1e9, 2)}
     print(json.dumps(out), flush=True)
+
+
+def hbm_copy(args):
+    """Calibration workload: a plain 1 GiB device-to-device copy (torch's
+    elementwise copy kernel), the reference for scaling the FETCH_SIZE and
+    WRITE_SIZE counters to bytes."""
+    import torch
+
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(args.warmup):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    print(json.dumps({"metric": "hbm copy (read + write) GB/s", "value": round(2 * 4 * n / ms / 1e6, 1),
+                      "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                      "ms_per_step": round(ms, 4), "bytes_read": 4 * n, "bytes_written": 4 * n}), flush=True)
+
+
+def measured_traffic(args, kernel):
+    """HBM bytes per launch of `kernel` on this workload, from the FETCH_SIZE
+    and WRITE_SIZE counter runs recorded in profiles/hbm_traffic.json (each
+    already scaled by the copy-kernel calibration); None when no record
+    matches the workload, precision and kernel."""
+    path = os.path.join(REPO, "profiles", "hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        table = json.load(f)
+    rec = table.get(f"{args.workload}/{args.precision}/{kernel}")
+    return None if rec is None else rec["bytes_per_launch"]
 
 
 def error_metrics(c, a, b, nsample=4096):

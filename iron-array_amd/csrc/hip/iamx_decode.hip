@@ -21,6 +21,8 @@
 // chunks).
 #include "iamx_internal.h"
 
+#include <stdlib.h>
+
 namespace {
 
 // Delta-swap transpose of an 8x8 bit matrix held in a uint64 (row r = byte r,
@@ -36,70 +38,250 @@ __device__ __forceinline__ uint64_t transpose8(uint64_t x) {
   return x;
 }
 
+__device__ __forceinline__ uint32_t rdlane(uint32_t v, int32_t l) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
+}
+
+// One wavefront per LZ4 stream.
+//   input   the wave keeps 1 KiB of the compressed stream in four VGPRs (lane
+//           l of w[k] holds bytes wb+256k+4l..+3); sequence headers are parsed
+//           uniformly with readlane, so the serial token chain does not wait
+//           on memory.  The window slides by 256 bytes, loading w[3] two slides
+//           before it is read, or jumps after a long literal run.
+//   output  decoded bytes go to an LDS ring of RING bytes and reach the
+//           stream's scratch slot in dword stores whenever RING/2 bytes are
+//           pending.  Literal runs longer than 64 bytes are copied global ->
+//           global and only their last RING bytes enter the ring.
+//   matches with offset <= RING-64 are copied ring -> ring, 64 bytes per step,
+//           each lane reading `off * ceil((lane+1)/off)` bytes back (`off`
+//           back when off >= 64), so a step never reads what it writes.
+//           Farther matches read the flushed stream back from global memory,
+//           after a workgroup fence if it was stored since the last fence.
+//   fast path  a token below 15 (no literals, match length 4..18) whose match
+//           does not overlap itself is one ring step with all 64 lanes.
+// Ring steps store all 64 lanes: bytes past the sequence land on positions
+// that are written again before anything reads them, and the slots they take
+// belong to positions more than RING-64 bytes back, which no later match
+// reads and which were flushed (at most RING/2 + 64 bytes are ever pending).
+template <int RING>
 __global__ __launch_bounds__(64) void k_lz4_streams(const iamx_lz4_job_t *__restrict__ jobs, int njobs,
                                                     uint8_t *__restrict__ scratch, int *status) {
+  __shared__ __align__(16) uint8_t ring[RING];
+  constexpr int32_t RM = RING - 1;
+  constexpr int32_t NEAR = RING - 64;
+  constexpr int32_t FLUSH_AT = RING / 2;
+  constexpr int32_t BULK = 1024;  // bytes per step of a long literal run
   const int j = blockIdx.x;
   if (j >= njobs) return;
-  const iamx_lz4_job_t job = jobs[j];
-  const IAMX_GLOBAL uint8_t *__restrict__ src = iamx::gptr<const uint8_t>(job.src);
-  IAMX_GLOBAL uint8_t *dst = iamx::gptr<uint8_t>((uint64_t)(scratch + job.dst_off));
   const int lane = threadIdx.x;
-  const int32_t iend = job.csize, oend = job.nbytes;
-  int32_t ip = 0, op = 0, fenced = 0;
+  const iamx_lz4_job_t job = jobs[j];
+  const uint64_t abase = job.src & ~3ull;  // input positions are relative to this aligned address
+  const IAMX_GLOBAL uint32_t *__restrict__ sw = iamx::gptr<const uint32_t>(abase);
+  const IAMX_GLOBAL uint8_t *__restrict__ sb = iamx::gptr<const uint8_t>(abase);
+  IAMX_GLOBAL uint8_t *dst = iamx::gptr<uint8_t>((uint64_t)(scratch + job.dst_off));
+  IAMX_GLOBAL uint32_t *dstw = iamx::gptr<uint32_t>((uint64_t)(scratch + job.dst_off));
+  const uint32_t *ring32 = reinterpret_cast<const uint32_t *>(ring);
+  const int32_t iend = (int32_t)(job.src & 3) + job.csize;
+  const int32_t oend = job.nbytes;
+  const int32_t lastw = (iend - 1) >> 2;  // last word holding stream bytes
+
+  // window words past the stream repeat its last word (never parsed)
+  auto ld_word = [&](int32_t pos) -> uint32_t { return sw[min(pos >> 2, lastw)]; };
+  int32_t wb = 0;
+  uint32_t w0 = ld_word(4 * lane), w1 = ld_word(256 + 4 * lane), w2 = ld_word(512 + 4 * lane),
+           w3 = ld_word(768 + 4 * lane);
+  auto jump = [&](int32_t pos) {
+    wb = pos & ~3;
+    w0 = ld_word(wb + 4 * lane);
+    w1 = ld_word(wb + 256 + 4 * lane);
+    w2 = ld_word(wb + 512 + 4 * lane);
+    w3 = ld_word(wb + 768 + 4 * lane);
+  };
+  // make `pos - wb < 256`; positions only move forward
+  auto refill = [&](int32_t pos) {
+    const int32_t q = pos - wb;
+    if (q >= 256) {
+      if (q < 512) {
+        w0 = w1;
+        w1 = w2;
+        w2 = w3;
+        wb += 256;
+        w3 = ld_word(wb + 768 + 4 * lane);
+      } else {
+        jump(pos);
+      }
+    }
+  };
+  // byte at input position pos, wb <= pos < wb + 512 (uniform)
+  auto rbyte = [&](int32_t pos) -> uint32_t {
+    const int32_t q = pos - wb;
+    const uint32_t w = q < 256 ? rdlane(w0, q >> 2) : rdlane(w1, (q >> 2) - 64);
+    return (w >> ((q & 3) << 3)) & 0xffu;
+  };
+  // four bytes from input position pos, wb <= pos < wb + 256 (uniform)
+  auto read32 = [&](int32_t pos) -> uint32_t {
+    const int32_t q = pos - wb;
+    const int32_t l = q >> 2;
+    const uint32_t lo = rdlane(w0, l);
+    uint32_t hi;
+    if (l < 63) hi = rdlane(w0, l + 1);
+    else hi = rdlane(w1, 0);
+    return (uint32_t)(((((uint64_t)hi) << 32) | lo) >> ((q & 3) << 3));
+  };
+
+  int32_t ip = (int32_t)(job.src & 3), op = 0, flushed = 0, fenced = 0, flush_at = FLUSH_AT;
+  // store the whole dwords of ring bytes [flushed, cur) to the slot; a dword
+  // is stored again only with the same bytes below `flushed`
+  auto flush = [&](int32_t cur) {
+    for (int32_t p = flushed + 4 * lane; p + 4 <= cur; p += 256) dstw[p >> 2] = ring32[(p & RM) >> 2];
+    flushed = __builtin_amdgcn_readfirstlane(cur & ~3);  // uniform; keeps it in an SGPR
+    flush_at = flushed + FLUSH_AT;
+  };
+
   int bad = 0;
   for (;;) {
+    refill(ip);
+    if (ip <= iend - 3 && op <= oend - 18) {
+      const uint32_t h = read32(ip);
+      const int32_t token = (int32_t)(h & 0xffu);
+      if (token < 15) {
+        const int32_t ml = token + 4;
+        const int32_t off = (int32_t)((h >> 8) & 0xffffu);
+        if (off >= ml && off <= min(op, NEAR)) {
+          const uint8_t v = ring[(op - off + lane) & RM];
+          __builtin_amdgcn_wave_barrier();
+          ring[(op + lane) & RM] = v;
+          __builtin_amdgcn_wave_barrier();
+          ip += 3;
+          op += ml;
+          if (op >= flush_at) flush(op);
+          continue;
+        }
+      }
+    }
     if (ip >= iend) { bad = 1; break; }
-    const uint32_t token = src[ip++];
+    const uint32_t token = rbyte(ip);
+    ++ip;
     int32_t lit = (int32_t)(token >> 4);
     if (lit == 15) {
       uint32_t b;
       do {
         if (ip >= iend) { bad = 2; break; }
-        b = src[ip++];
+        refill(ip);
+        b = rbyte(ip);
+        ++ip;
         lit += (int32_t)b;
       } while (b == 255);
       if (bad) break;
     }
     if (lit > iend - ip || lit > oend - op) { bad = 3; break; }
-    for (int32_t i = lane; i < lit; i += 256) {
-      dst[op + i] = src[ip + i];
-      if (i + 64 < lit) dst[op + i + 64] = src[ip + i + 64];
-      if (i + 128 < lit) dst[op + i + 128] = src[ip + i + 128];
-      if (i + 192 < lit) dst[op + i + 192] = src[ip + i + 192];
+    if (lit > 0) {
+      if (lit <= 64) {
+        // literals from the register window
+        refill(ip);
+        const int32_t q = ip - wb + lane;  // < 320
+        const int sl = (q >> 2) & 63;
+        const uint32_t a0 = (uint32_t)__shfl((int)w0, sl, 64), a1 = (uint32_t)__shfl((int)w1, sl, 64);
+        const uint32_t v = ((q < 256 ? a0 : a1) >> ((q & 3) << 3)) & 0xffu;
+        ring[(op + lane) & RM] = (uint8_t)v;
+        __builtin_amdgcn_wave_barrier();
+        ip += lit;
+        op += lit;
+        if (op >= flush_at) flush(op);
+      } else {
+        // long run: what is pending goes out first, then the run is copied
+        // global -> global; the window for the next header is requested now
+        flush(op);
+        if (lane < op - flushed) dst[flushed + lane] = ring[(flushed + lane) & RM];
+        const int32_t nip = ip + lit;
+        if (nip - wb >= 256) jump(nip);
+        const int32_t keep = lit - RING;  // run bytes before this index never reach the ring
+        for (int32_t k = 0; k < lit; k += BULK) {
+          uint8_t v[BULK / 64];
+#pragma unroll
+          for (int t = 0; t < BULK / 64; t++) {
+            const int32_t i = k + 64 * t + lane;
+            v[t] = i < lit ? sb[ip + i] : (uint8_t)0;
+          }
+#pragma unroll
+          for (int t = 0; t < BULK / 64; t++) {
+            const int32_t i = k + 64 * t + lane;
+            if (i < lit) {
+              dst[op + i] = v[t];
+              if (i >= keep) ring[(op + i) & RM] = v[t];
+            }
+          }
+        }
+        __builtin_amdgcn_wave_barrier();
+        ip = nip;
+        op += lit;
+        flushed = __builtin_amdgcn_readfirstlane(op & ~3);
+        flush_at = flushed + FLUSH_AT;
+      }
     }
-    ip += lit;
-    op += lit;
     if (ip == iend) break;  // last sequence: literals only
+    refill(ip);
     if (iend - ip < 2) { bad = 4; break; }
-    const int32_t off = (int32_t)src[ip] | ((int32_t)src[ip + 1] << 8);
+    const int32_t off = (int32_t)(rbyte(ip) | (rbyte(ip + 1) << 8));
     ip += 2;
     int32_t ml = (int32_t)(token & 15u);
     if (ml == 15) {
       uint32_t b;
       do {
         if (ip >= iend) { bad = 5; break; }
-        b = src[ip++];
+        refill(ip);
+        b = rbyte(ip);
+        ++ip;
         ml += (int32_t)b;
       } while (b == 255);
       if (bad) break;
     }
     ml += 4;
     if (off == 0 || off > op || ml > oend - op) { bad = 6; break; }
-    const int32_t mstart = op - off;
-    const int32_t span = ml < off ? ml : off;
-    if (mstart + span > fenced) {
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-      fenced = op;
+    // back distance of this lane's source: off * ceil((lane + 1) / off) for off < 64
+    int32_t back = off;
+    if (off < 64) {
+      const int32_t num = lane + off;
+      int32_t qq = (int32_t)((float)num * __builtin_amdgcn_rcpf((float)off));
+      const int32_t r = num - qq * off;
+      qq += (r >= off) - (r < 0);
+      back = off * qq;
     }
-    if (off >= ml) {
-      for (int32_t i = lane; i < ml; i += 64) dst[op + i] = dst[mstart + i];
+    if (off <= NEAR) {
+      for (int32_t k = 0; k < ml; k += 64) {
+        const uint8_t v = ring[(op + k + lane - back) & RM];
+        __builtin_amdgcn_wave_barrier();
+        ring[(op + k + lane) & RM] = v;
+        __builtin_amdgcn_wave_barrier();
+        const int32_t cur = op + min(ml, k + 64);
+        if (cur >= flush_at) flush(cur);
+      }
     } else {
-      for (int32_t i = lane; i < ml; i += 64) dst[op + i] = dst[mstart + (i % off)];
+      for (int32_t k = 0; k < ml; k += 64) {
+        // positions at or after `lo` are still in the ring; older ones were
+        // flushed and are read back from global memory
+        const int32_t lo = op + k + 64 - RING;
+        const int32_t gmax = min(op + k + 64 - back, lo);  // end of the global positions this step reads
+        if (gmax > fenced) {
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+          fenced = flushed;
+        }
+        const int32_t i = k + lane;
+        const int32_t s = op + i - back;
+        uint8_t v = 0;
+        if (i < ml) v = s >= lo ? ring[s & RM] : dst[s];
+        __builtin_amdgcn_wave_barrier();
+        ring[(op + i) & RM] = v;
+        __builtin_amdgcn_wave_barrier();
+        const int32_t cur = op + min(ml, k + 64);
+        if (cur >= flush_at) flush(cur);
+      }
     }
     op += ml;
   }
   if (!bad && op != oend) bad = 7;
+  if (!bad) flush((oend + 3) & ~3);  // the slot is padded to 16 bytes
   if (bad && lane == 0) atomicOr(status, 1 << bad);
 }
 
@@ -298,6 +480,39 @@ __global__ __launch_bounds__(256) void k_transpose(int64_t rows, int64_t cols, c
 
 }  // namespace
 
+namespace iamx {
+
+// History ring of the LZ4 kernel in KiB (4, 8 or 16): a bigger ring serves
+// more matches from LDS, a smaller one fits more wavefronts per CU.
+static int lz4_ring_kib() {
+  static const int kib = [] {
+    const char *e = getenv("IAMX_LZ4_RING_KIB");
+    const int v = e ? atoi(e) : 8;
+    return (v == 4 || v == 16) ? v : 8;
+  }();
+  return kib;
+}
+
+int launch_lz4(const char *name, const iamx_lz4_job_t *d_lz4, int n_lz4, void *d_scratch) {
+  switch (lz4_ring_kib()) {
+    case 4:
+      IAMX_LAUNCH(name, k_lz4_streams<4096>, dim3(n_lz4), dim3(64), 0, d_lz4, n_lz4, (uint8_t *)d_scratch,
+                  status_word());
+      break;
+    case 16:
+      IAMX_LAUNCH(name, k_lz4_streams<16384>, dim3(n_lz4), dim3(64), 0, d_lz4, n_lz4, (uint8_t *)d_scratch,
+                  status_word());
+      break;
+    default:
+      IAMX_LAUNCH(name, k_lz4_streams<8192>, dim3(n_lz4), dim3(64), 0, d_lz4, n_lz4, (uint8_t *)d_scratch,
+                  status_word());
+      break;
+  }
+  return IAMX_OK;
+}
+
+}  // namespace iamx
+
 extern "C" {
 
 IAMX_API int iamx_decode(const iamx_lz4_job_t *d_lz4, int n_lz4, const iamx_block_job_t *d_blocks, int n_blocks,
@@ -306,8 +521,8 @@ IAMX_API int iamx_decode(const iamx_lz4_job_t *d_lz4, int n_lz4, const iamx_bloc
   if (rc) return rc;
   if (n_lz4 < 0 || n_blocks < 0) return iamx::set_error(IAMX_EINVAL, "iamx_decode: negative job count");
   if (n_lz4 > 0)
-    IAMX_LAUNCH("lz4_streams", k_lz4_streams, dim3(n_lz4), dim3(64), 0, d_lz4, n_lz4, (uint8_t *)d_scratch,
-                iamx::status_word());
+    rc = iamx::launch_lz4("lz4_streams", d_lz4, n_lz4, d_scratch);
+  if (rc) return rc;
   if (n_blocks > 0)
     IAMX_LAUNCH("unfilter", k_unfilter, dim3(n_blocks), dim3(256), 0, d_blocks, n_blocks,
                 (const uint8_t *)d_scratch, (uint8_t *)d_dst);

@@ -2,18 +2,20 @@
 //
 // After the LZ4 streams of A are decoded into scratch (k_lz4_streams), one
 // workgroup per compressed A block turns the block's streams straight into
-// row sums of A_blk x_seg without writing A anywhere:
-//   phase 1  every thread takes groups of 8 consecutive elements: for a
-//            bitshuffled block the 8 elements' bits are byte g of each of the
-//            8*typesize bit planes, and one 8x8 bit-matrix transpose per byte
-//            position rebuilds them; for a byte-shuffled block byte j of the 8
-//            elements is 8 consecutive bytes of plane j.  Each element is
-//            multiplied by x[column] (x staged in LDS) and the group's products
-//            are summed per row (a group touches at most two rows), into LDS;
-//   phase 2  one thread per row adds its groups' partial sums in group order.
-// Rows are processed in windows so the per-group partials fit in LDS.  Row
-// sums land in partial[block column][row]; a second launch adds the block
-// columns in order, so y is bit-identical run to run.
+// row sums of A_blk x_seg without writing A anywhere.
+//
+// Bit-shuffled single-stream blocks (the common case) take the quad path:
+// a thread takes 32 consecutive elements ("quad" u = elements 32u..32u+31):
+// one dword from each of the 8*typesize bit planes (byte 4u..4u+3 of every
+// plane), byte-permutes them into 8-byte columns, runs one 8x8 bit-matrix
+// transpose per (group, element byte) and byte-permutes the results into
+// elements.  A quad spans at most two rows of the block (rows have >= 32
+// elements), so it leaves two partial sums in LDS; then one thread per row
+// adds its quads' partial sums in quad order.  Any other block layout takes
+// the group path: 8 elements per thread, bytes gathered one by one.
+// Rows are processed in windows so the partials fit in LDS.  Row sums land in
+// partial[block column][row]; a second launch adds the block columns in
+// order, so y is bit-identical run to run.
 #include "iamx_internal.h"
 
 namespace {
@@ -41,12 +43,189 @@ __device__ __forceinline__ uint64_t transpose8(uint64_t x) {
 }
 
 constexpr int kMaxCols = 1024;
-constexpr int kGroups = 4096;  // per-window groups whose partial sums live in LDS
+constexpr int kGroups = 2048;  // per-window groups (or quads) whose partial sums live in LDS
+
+// x is staged in LDS with one pad word per 32 columns: the lanes of a wave
+// read columns 32 apart, which then fall in different banks.
+__device__ __forceinline__ int xpad(int c) { return c + (c >> 5); }
+
+// 4x4 byte transpose: r[g] = bytes g of (a, b, c, d)
+__device__ __forceinline__ void tr4(uint32_t a, uint32_t b, uint32_t c, uint32_t d, uint32_t r[4]) {
+  const uint32_t t0 = __builtin_amdgcn_perm(b, a, 0x05010400u);  // a0 b0 a1 b1
+  const uint32_t t1 = __builtin_amdgcn_perm(b, a, 0x07030602u);  // a2 b2 a3 b3
+  const uint32_t t2 = __builtin_amdgcn_perm(d, c, 0x05010400u);  // c0 d0 c1 d1
+  const uint32_t t3 = __builtin_amdgcn_perm(d, c, 0x07030602u);  // c2 d2 c3 d3
+  r[0] = __builtin_amdgcn_perm(t2, t0, 0x05040100u);
+  r[1] = __builtin_amdgcn_perm(t2, t0, 0x07060302u);
+  r[2] = __builtin_amdgcn_perm(t3, t1, 0x05040100u);
+  r[3] = __builtin_amdgcn_perm(t3, t1, 0x07060302u);
+}
+
+// 8x8 bit transpose of (hi:lo) as two 32-bit halves (the delta swaps of
+// transpose8 below, the first two of which stay inside each half)
+__device__ __forceinline__ void transpose8_32(uint32_t &lo, uint32_t &hi) {
+  uint32_t t;
+  t = (lo ^ (lo >> 7)) & 0x00AA00AAu;
+  lo ^= t ^ (t << 7);
+  t = (hi ^ (hi >> 7)) & 0x00AA00AAu;
+  hi ^= t ^ (t << 7);
+  t = (lo ^ (lo >> 14)) & 0x0000CCCCu;
+  lo ^= t ^ (t << 14);
+  t = (hi ^ (hi >> 14)) & 0x0000CCCCu;
+  hi ^= t ^ (t << 14);
+  t = (lo ^ __builtin_amdgcn_alignbit(hi, lo, 28)) & 0xF0F0F0F0u;
+  lo ^= t ^ (t << 28);
+  hi ^= t >> 4;
+}
+
+template <typename T>
+struct Word;
+template <>
+struct Word<float> {
+  using U = uint32_t;
+  static __device__ __forceinline__ float of(uint32_t lo, uint32_t) { return __uint_as_float(lo); }
+};
+template <>
+struct Word<double> {
+  using U = uint64_t;
+  static __device__ __forceinline__ double of(uint32_t lo, uint32_t hi) {
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+  }
+};
 
 struct Src {
   const IAMX_GLOBAL uint8_t *ptr[IAMX_MAX_STREAMS];
   uint8_t fill[IAMX_MAX_STREAMS];
 };
+
+// Row sums of one bit-shuffled single-stream block, quads of 32 elements.
+// Plane p starts at base + p*rowlen; elements [n8, n) follow the planes
+// unshuffled and form one more (short) quad.
+template <typename T>
+__device__ void quad_path(const IAMX_GLOBAL uint8_t *base, int32_t n, int32_t n8, int32_t rowlen, int32_t cols,
+                          int32_t vrows, int32_t vcols, const T *xs, T *p0, T *p1, T *prow, int64_t row0,
+                          int64_t M) {
+  constexpr int TS = sizeof(T);
+  constexpr int NP = 8 * TS;
+  const int tid = threadIdx.x;
+  const int32_t nq = n8 / 32;  // quads drawn whole from the planes
+  const int32_t nqe = (n + 31) / 32;
+  const bool aligned = ((rowlen & 3) == 0) && ((((uint64_t)base) & 3) == 0);
+  const int32_t rows_per_win = (int32_t)(((int64_t)(kGroups - 2) * 32) / cols);
+  for (int32_t ra = 0; ra < vrows; ra += rows_per_win) {
+    const int32_t rb = min(vrows, ra + rows_per_win);
+    const int32_t u0 = (ra * cols) / 32, u1 = min(nqe - 1, (rb * cols - 1) / 32);  // quads [u0, u1]
+    for (int32_t u = u0 + tid; u <= u1; u += 256) {
+      const int32_t e0 = 32 * u;
+      const int32_t r0 = e0 / cols;
+      const int32_t c0 = e0 - r0 * cols;
+      const int32_t k0 = cols - c0;  // elements of the quad in row r0
+      T s0 = 0, s1 = 0;
+      if (u < nq) {
+        uint32_t d[NP];
+        if (aligned) {
+#pragma unroll
+          for (int p = 0; p < NP; p++) d[p] = *(const IAMX_GLOBAL uint32_t *)(base + (int64_t)p * rowlen + 4 * u);
+        } else {
+#pragma unroll
+          for (int p = 0; p < NP; p++) {
+            const IAMX_GLOBAL uint8_t *q = base + (int64_t)p * rowlen + 4 * u;
+            d[p] = (uint32_t)q[0] | ((uint32_t)q[1] << 8) | ((uint32_t)q[2] << 16) | ((uint32_t)q[3] << 24);
+          }
+        }
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+          // Y[j] = bit transpose of byte g of planes 8j..8j+7: byte i is byte j of element 8g+i
+          uint32_t ylo[TS], yhi[TS];
+          const uint32_t sel = (uint32_t)g | ((uint32_t)(g + 4) << 8);
+#pragma unroll
+          for (int j = 0; j < TS; j++) {
+            const uint32_t a0 = __builtin_amdgcn_perm(d[8 * j + 1], d[8 * j + 0], sel);
+            const uint32_t a1 = __builtin_amdgcn_perm(d[8 * j + 3], d[8 * j + 2], sel);
+            const uint32_t a2 = __builtin_amdgcn_perm(d[8 * j + 5], d[8 * j + 4], sel);
+            const uint32_t a3 = __builtin_amdgcn_perm(d[8 * j + 7], d[8 * j + 6], sel);
+            uint32_t lo = __builtin_amdgcn_perm(a1, a0, 0x05040100u);
+            uint32_t hi = __builtin_amdgcn_perm(a3, a2, 0x05040100u);
+            transpose8_32(lo, hi);
+            ylo[j] = lo;
+            yhi[j] = hi;
+          }
+          // element 8g+i, i < 4: bytes i of ylo[0..TS); i >= 4: bytes i-4 of yhi
+          uint32_t elo[8], ehi[8];
+          {
+            uint32_t r[4];
+            tr4(ylo[0], ylo[1], ylo[2], ylo[3], r);
+#pragma unroll
+            for (int i = 0; i < 4; i++) elo[i] = r[i];
+            tr4(yhi[0], yhi[1], yhi[2], yhi[3], r);
+#pragma unroll
+            for (int i = 0; i < 4; i++) elo[4 + i] = r[i];
+            if (TS == 8) {
+              tr4(ylo[4 % TS], ylo[5 % TS], ylo[6 % TS], ylo[7 % TS], r);
+#pragma unroll
+              for (int i = 0; i < 4; i++) ehi[i] = r[i];
+              tr4(yhi[4 % TS], yhi[5 % TS], yhi[6 % TS], yhi[7 % TS], r);
+#pragma unroll
+              for (int i = 0; i < 4; i++) ehi[4 + i] = r[i];
+            } else {
+#pragma unroll
+              for (int i = 0; i < 8; i++) ehi[i] = 0;
+            }
+          }
+#pragma unroll
+          for (int i = 0; i < 8; i++) {
+            const int32_t k = 8 * g + i;
+            const bool first = k < k0;
+            const int32_t c = first ? c0 + k : c0 + k - cols;
+            const T prod = c < vcols ? Word<T>::of(elo[i], ehi[i]) * xs[xpad(c)] : (T)0;
+            if (first) s0 += prod;
+            else s1 += prod;
+          }
+        }
+      } else {
+        // the last, short quad: whole 8-element groups still come from the
+        // planes (byte 4u+g of each, gathered one by one), elements [n8, n)
+        // follow the planes unshuffled
+        for (int32_t e = e0; e < n; e++) {
+          uint32_t lo = 0, hi = 0;
+          if (e < n8) {
+            const int32_t gb = e >> 3, bit = e & 7;
+#pragma unroll 8
+            for (int p = 0; p < NP; p++) {
+              const uint32_t v = (base[(int64_t)p * rowlen + gb] >> bit) & 1u;
+              if (p < 32) lo |= v << p;
+              else hi |= v << (p - 32);
+            }
+          } else {
+            const IAMX_GLOBAL uint8_t *q = base + (int64_t)n8 * TS + (int64_t)(e - n8) * TS;
+#pragma unroll
+            for (int bb = 0; bb < 4; bb++) lo |= (uint32_t)q[bb] << (8 * bb);
+            if (TS == 8) {
+#pragma unroll
+              for (int bb = 0; bb < 4; bb++) hi |= (uint32_t)q[4 + (bb & 3)] << (8 * bb);
+            }
+          }
+          const int32_t k = e - e0;
+          const bool first = k < k0;
+          const int32_t c = first ? c0 + k : c0 + k - cols;
+          const T prod = c < vcols ? Word<T>::of(lo, hi) * xs[xpad(c)] : (T)0;
+          if (first) s0 += prod;
+          else s1 += prod;
+        }
+      }
+      p0[u - u0] = s0;
+      p1[u - u0] = s1;
+    }
+    __syncthreads();
+    for (int32_t r = ra + tid; r < rb; r += 256) {
+      const int32_t ua = (r * cols) / 32, ub = min(nqe - 1, (r * cols + cols - 1) / 32);
+      T acc = 0;
+      for (int32_t u = ua; u <= ub; u++) acc += ((32 * u) / cols == r) ? p0[u - u0] : p1[u - u0];
+      if (row0 + r < M) prow[row0 + r] = acc;
+    }
+    __syncthreads();
+  }
+}
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_gemv_fused(const iamx_block_job_t *__restrict__ jobs,
@@ -55,7 +234,7 @@ __global__ __launch_bounds__(256) void k_gemv_fused(const iamx_block_job_t *__re
                                                     T *__restrict__ partial, int64_t M) {
   __shared__ iamx_block_job_t J;
   __shared__ Src S;
-  __shared__ T xs[kMaxCols];
+  __shared__ T xs[kMaxCols + kMaxCols / 32];
   __shared__ T p0[kGroups], p1[kGroups];
   const int b = blockIdx.x;
   if (b >= njobs) return;
@@ -72,7 +251,7 @@ __global__ __launch_bounds__(256) void k_gemv_fused(const iamx_block_job_t *__re
                                                : nullptr;
     S.fill[s] = s < J.nstreams ? J.fill[s] : 0;
   }
-  for (int c = tid; c < J.vcols; c += 256) xs[c] = x[col0 + c];
+  for (int c = tid; c < J.cols; c += 256) xs[xpad(c)] = c < J.vcols ? x[col0 + c] : (T)0;
   __syncthreads();
 
   const int ts = J.typesize;
@@ -83,6 +262,10 @@ __global__ __launch_bounds__(256) void k_gemv_fused(const iamx_block_job_t *__re
   const bool bits = filter == IAMX_FILTER_BITSHUFFLE || filter == IAMX_FILTER_BITSHUFFLE_V2;
   const int32_t n8 = filter == IAMX_FILTER_BITSHUFFLE ? n - n % 8 : n;
   const int32_t rowlen = n8 / 8;
+  if (bits && J.nstreams == 1 && S.ptr[0] != nullptr && cols >= 32 && n8 >= 32 && ts == (int)sizeof(T)) {
+    quad_path<T>(S.ptr[0], n, n8, rowlen, cols, vrows, vcols, xs, p0, p1, partial + (int64_t)bcol * M, row0, M);
+    return;
+  }
   const bool one_stream = J.nstreams == 1;
   const bool split_planes = filter == IAMX_FILTER_SHUFFLE && ts > 1;
 
@@ -116,7 +299,7 @@ __global__ __launch_bounds__(256) void k_gemv_fused(const iamx_block_job_t *__re
         } else {
           for (int j = 0; j < ts; j++) v |= (uint64_t)byte_at(e * ts + j) << (8 * j);
         }
-        acc += bits_to<T>(v) * xs[c];
+        acc += bits_to<T>(v) * xs[xpad(c)];
       }
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
@@ -166,7 +349,7 @@ __global__ __launch_bounds__(256) void k_gemv_fused(const iamx_block_job_t *__re
 #pragma unroll
       for (int i = 0; i < 8; i++) {
         if (r >= ra && r < rb && c < vcols) {
-          const T prod = bits_to<T>(v[i]) * xs[c];
+          const T prod = bits_to<T>(v[i]) * xs[xpad(c)];
           if (r == rfirst) s0 += prod;
           else s1 += prod;
         }
@@ -212,8 +395,10 @@ IAMX_API int iamx_gemv_fused(int dtype, const iamx_lz4_job_t *d_lz4, int n_lz4, 
   if (bs1 > kMaxCols) return iamx::set_error(IAMX_EINVAL, "iamx_gemv_fused: block width %d > %d", bs1, kMaxCols);
   (void)bs0;
   if (n_blocks <= 0 || nbc <= 0) return iamx::set_error(IAMX_EINVAL, "iamx_gemv_fused: no blocks");
-  rc = iamx_decode(d_lz4, n_lz4, nullptr, 0, d_scratch, nullptr);
-  if (rc) return rc;
+  if (n_lz4 > 0) {
+    rc = iamx::launch_lz4("lz4_streams_gemv", d_lz4, n_lz4, d_scratch);
+    if (rc) return rc;
+  }
   const dim3 rgrid((unsigned)((M + 255) / 256));
   if (dtype == 0) {
     IAMX_LAUNCH("gemv_fused_f64", k_gemv_fused<double>, dim3(n_blocks), dim3(256), 0, d_blocks, d_blk_rc, n_blocks,

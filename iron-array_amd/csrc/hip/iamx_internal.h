@@ -18,6 +18,9 @@ int ensure_init();
 // Device word the decode kernels OR their error bits into.
 int *status_word();
 
+// Launch the LZ4 stream decoder over n_lz4 jobs into d_scratch, profiled as `name`.
+int launch_lz4(const char *name, const iamx_lz4_job_t *d_lz4, int n_lz4, void *d_scratch);
+
 // Profiling: brackets one launch with HIP events when enabled.
 struct ProfScope {
   int slot = -1;

@@ -164,6 +164,32 @@ def test_bitshuffled_gemv_bit_exact(ctx):
     assert np.array_equal(y.to_numpy(), (a.astype(np.float64) @ x.astype(np.float64)).astype(np.float32))
 
 
+BITS_GEMV = [
+    # dtype, A shape, A chunks, A blocks: the quad path (whole quads, short last
+    # quads with plane groups and unshuffled tails, unaligned planes) and the
+    # group path (rows shorter than 32 elements)
+    (np.float32, (300, 410), (150, 205), (33, 40)),
+    (np.float32, (200, 150), (100, 150), (37, 50)),
+    (np.float64, (500, 600), (250, 300), (125, 125)),
+    (np.float64, (400, 256), (200, 256), (64, 64)),
+    (np.float32, (100, 64), (100, 64), (100, 20)),
+    (np.float32, (12, 3000), (8, 2000), (4, 1000)),
+]
+
+
+@pytest.mark.parametrize("dt,shape,cs,bs", BITS_GEMV, ids=[f"{np.dtype(d).name}-{b[0]}x{b[1]}" for d, _, _, b in BITS_GEMV])
+def test_bitshuffled_gemv_block_shapes(dt, shape, cs, bs):
+    print(f"case: bitshuffle gemv {np.dtype(dt).name} {shape} {cs} {bs}", flush=True)
+    bctx = ia.Context(clevel=5, filters=ia.COMP_BITSHUFFLE)
+    rng = np.random.default_rng(shape[0] + bs[1])
+    a = rng.integers(-6, 7, shape).astype(dt)
+    x = rng.integers(-6, 7, (shape[1],)).astype(dt)
+    ca = ia.Container.from_numpy(bctx, a, cs, bs)
+    cx = ia.Container.from_numpy(bctx, x, (cs[1],), (bs[1],))
+    y = ia.matmul(ca, cx, (cs[0],), (bs[0],))
+    assert np.array_equal(y.to_numpy(), (a.astype(np.float64) @ x.astype(np.float64)).astype(dt))
+
+
 def test_matmul_deterministic(ctx):
     rng = np.random.default_rng(2)
     a = rng.standard_normal((400, 300))
