@@ -40,6 +40,14 @@ IAMX_API int iamx_d2h(void *dst, const void *src, size_t bytes);
 IAMX_API int iamx_d2d(void *dst, const void *src, size_t bytes);
 IAMX_API int iamx_memset(void *dst, int value, size_t bytes);
 IAMX_API int iamx_sync(void);
+/* Pinned host staging buffer (grows, kept until iamx_shutdown) and
+ * stream-ordered downloads into it: iamx_d2h_async queues the copy on the
+ * library stream and records event `event` (0..IAMX_COPY_EVENTS-1) after it;
+ * iamx_event_sync blocks the host until that event has completed. */
+#define IAMX_COPY_EVENTS 64
+IAMX_API void *iamx_host_buffer(size_t bytes);
+IAMX_API int iamx_d2h_async(void *dst, const void *src, size_t bytes, int event);
+IAMX_API int iamx_event_sync(int event);
 /* Scratch that lives until iamx_shutdown; grown (never shrunk) on demand.
  * Slot s (0..7) is independent of the others. */
 IAMX_API void *iamx_workspace(int slot, size_t bytes);

@@ -120,13 +120,38 @@ __global__ __launch_bounds__(256) void k_row_exp(const double *__restrict__ X, i
   if (lane == 0) e[r] = exp_of(m);
 }
 
+// Column exponents in three launches: fill with INT_MIN, a grid of 64-column
+// x row-slab tiles (coalesced row reads) folding each slab's max exponent in
+// with atomicMax (exp_of is monotonic in the magnitude), then INT_MIN (an
+// all-zero column) -> 0.
+__global__ __launch_bounds__(256) void k_fill_i32(int32_t *__restrict__ e, int64_t n, int32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) e[i] = v;
+}
+
+constexpr int kColSlab = 128;  // rows per tile of k_col_exp
+
 __global__ __launch_bounds__(256) void k_col_exp(const double *__restrict__ X, int64_t rows, int64_t cols,
                                                  int64_t ld, int32_t *__restrict__ e) {
-  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (c >= cols) return;
+  __shared__ double red[4][64];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int64_t c = (int64_t)blockIdx.x * 64 + tx;
+  const int64_t r0 = (int64_t)blockIdx.y * kColSlab;
+  const int64_t r1 = r0 + kColSlab < rows ? r0 + kColSlab : rows;
   double m = 0.0;
-  for (int64_t r = 0; r < rows; r++) m = fmax(m, fabs(X[r * ld + c]));
-  e[c] = exp_of(m);
+  if (c < cols)
+    for (int64_t r = r0 + ty; r < r1; r += 4) m = fmax(m, fabs(X[r * ld + c]));
+  red[ty][tx] = m;
+  __syncthreads();
+  if (ty == 0 && c < cols) {
+    m = fmax(fmax(red[0][tx], red[1][tx]), fmax(red[2][tx], red[3][tx]));
+    if (m > 0.0) atomicMax(&e[c], exp_of(m));
+  }
+}
+
+__global__ __launch_bounds__(256) void k_col_exp_zero(int32_t *__restrict__ e, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n && e[i] == INT32_MIN) e[i] = 0;
 }
 
 // S digit planes, each rows_p x cols_p int8.  trans as in k_split_bf16; the
@@ -338,7 +363,10 @@ int iamx_gemm_ozaki_i8(int64_t M, int64_t N, int64_t K, const double *A, int64_t
   int8_t *da = (int8_t *)ws, *db = (int8_t *)(ws + a_bytes);
   int32_t *ea = (int32_t *)(ws + a_bytes + b_bytes), *fb = ea + Mp;
   IAMX_LAUNCH("ozaki_row_exp", k_row_exp, dim3((unsigned)((M + 3) / 4)), dim3(256), 0, A, M, K, lda, ea);
-  IAMX_LAUNCH("ozaki_col_exp", k_col_exp, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, B, K, N, ldb, fb);
+  IAMX_LAUNCH("ozaki_col_exp", k_fill_i32, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, fb, N, INT32_MIN);
+  IAMX_LAUNCH("ozaki_col_exp", k_col_exp, dim3((unsigned)((N + 63) / 64), (unsigned)((K + kColSlab - 1) / kColSlab)),
+              dim3(256), 0, B, K, N, ldb, fb);
+  IAMX_LAUNCH("ozaki_col_exp", k_col_exp_zero, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, fb, N);
   IAMX_LAUNCH("ozaki_slice_A", k_slice_i8, dim3((unsigned)(Kp / 32), (unsigned)(Mp / 32)), dim3(256), 0, A, M, K,
               lda, 0, ea, S, da, Mp, Kp);
   IAMX_LAUNCH("ozaki_slice_B", k_slice_i8, dim3((unsigned)(Kp / 32), (unsigned)(Np / 32)), dim3(256), 0, B, K, N,

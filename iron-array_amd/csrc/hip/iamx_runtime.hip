@@ -195,6 +195,55 @@ IAMX_API int iamx_d2h(void *dst, const void *src, size_t bytes) {
   return IAMX_OK;
 }
 
+// Persistent pinned staging buffer for device -> host downloads; grows
+// (synchronising the stream first) and lives until iamx_shutdown.
+static void *g_host_buf = nullptr;
+static size_t g_host_bytes = 0;
+static hipEvent_t g_copy_ev[IAMX_COPY_EVENTS];
+static bool g_copy_ev_made = false;
+
+IAMX_API void *iamx_host_buffer(size_t bytes) {
+  if (ensure_init()) return nullptr;
+  if (g_host_buf && g_host_bytes >= bytes) return g_host_buf;
+  if (g_host_buf) {
+    (void)hipStreamSynchronize(g_stream);
+    (void)hipHostFree(g_host_buf);
+    g_host_buf = nullptr;
+    g_host_bytes = 0;
+  }
+  const size_t want = bytes + bytes / 8 + 4096;
+  if (hipHostMalloc(&g_host_buf, want, hipHostMallocDefault) != hipSuccess) {
+    g_host_buf = nullptr;
+    set_error(IAMX_ENOMEM, "hipHostMalloc(%zu) failed", want);
+    return nullptr;
+  }
+  g_host_bytes = want;
+  return g_host_buf;
+}
+
+IAMX_API int iamx_d2h_async(void *dst, const void *src, size_t bytes, int event) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (event < 0 || event >= IAMX_COPY_EVENTS) return set_error(IAMX_EINVAL, "iamx_d2h_async: event %d", event);
+  if (!g_copy_ev_made) {
+    for (int i = 0; i < IAMX_COPY_EVENTS; i++)
+      IAMX_HIP_CHECK(hipEventCreateWithFlags(&g_copy_ev[i], hipEventDisableTiming));
+    g_copy_ev_made = true;
+  }
+  if (bytes) IAMX_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
+  IAMX_HIP_CHECK(hipEventRecord(g_copy_ev[event], g_stream));
+  return IAMX_OK;
+}
+
+IAMX_API int iamx_event_sync(int event) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (event < 0 || event >= IAMX_COPY_EVENTS || !g_copy_ev_made)
+    return set_error(IAMX_EINVAL, "iamx_event_sync: event %d not recorded", event);
+  IAMX_HIP_CHECK(hipEventSynchronize(g_copy_ev[event]));
+  return IAMX_OK;
+}
+
 IAMX_API int iamx_d2d(void *dst, const void *src, size_t bytes) {
   int rc = ensure_init();
   if (rc) return rc;
@@ -245,6 +294,9 @@ IAMX_API void iamx_shutdown(void) {
     if (w.ptr) (void)hipFree(w.ptr);
     w = Workspace{};
   }
+  if (g_host_buf) (void)hipHostFree(g_host_buf);
+  g_host_buf = nullptr;
+  g_host_bytes = 0;
 }
 
 IAMX_API void iamx_prof_enable(int on) { g_prof_on = on != 0; }

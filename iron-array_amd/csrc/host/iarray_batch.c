@@ -239,7 +239,7 @@ typedef struct {
   iarray_container_t **c;
   const int64_t *first; /* prefix sums of chunk counts */
   int n;
-  const uint8_t *packed;
+  uint8_t *packed;
   const int64_t *out_off;
   int err;
 } wctx_t;
@@ -319,35 +319,31 @@ INA_API(ina_rc_t) iarray_linalg_matmul_batch(iarray_context_t *ctx, int n, iarra
   }
   iamx_prof_reset();
   int e = iamx_graph_launch(g_plan.graph);
-  uint8_t *host = NULL;
-  if (!e) {
-    host = (uint8_t *)malloc((size_t)g_plan.packed_bytes + 16);
-    e = host ? iamx_d2h(host, g_plan.d_packed, (size_t)g_plan.packed_bytes) : IAMX_ENOMEM;
-  }
-  iax_last_device_ms = iamx_prof_span_ms();
-  if (e) {
-    free(host);
-    return iax_dev_error(e, "batch replay");
-  }
+  if (e) return iax_dev_error(e, "batch replay");
   int st = iamx_decode_status();
-  if (st) {
-    free(host);
+  iax_last_device_ms = iamx_prof_span_ms();
+  if (st)
     return st < 0 ? iax_dev_error(st, "batch status")
                   : iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "malformed LZ4 stream (0x%x)", st);
-  }
   int64_t *first = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n + 1));
-  if (!first) {
-    free(host);
-    return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
-  }
+  if (!first) return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
   first[0] = 0;
   for (int i = 0; i < n; i++) first[i + 1] = first[i] + c[i]->nchunks;
-  wctx_t x = {c, first, n, host, g_plan.out_off, 0};
+  int64_t *ends = (int64_t *)malloc(sizeof(int64_t) * (size_t)(first[n] + 1));
+  if (!ends) {
+    free(first);
+    return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+  }
+  for (int i = 0; i < n; i++)
+    for (int64_t k = 0; k < c[i]->nchunks; k++) ends[first[i] + k] = g_plan.out_off[i] + (k + 1) * c[i]->chunk_nbytes;
+  wctx_t x = {c, first, n, NULL, g_plan.out_off, 0};
   const double t0 = iax_now();
-  iax_parallel_for(iax_nthreads(&ctx->cfg), first[n], write_task, &x);
+  rc = iax_download_pipelined(g_plan.d_packed, (size_t)g_plan.packed_bytes, ends, first[n],
+                                       iax_nthreads(&ctx->cfg), write_task, &x, &x.packed);
   iax_last_encode_s = iax_now() - t0;
+  free(ends);
   free(first);
-  free(host);
+  if (INA_FAILED(rc)) return rc;
   if (x.err) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "chunk encode failed");
   return INA_SUCCESS;
 }

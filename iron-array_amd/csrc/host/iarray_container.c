@@ -26,6 +26,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <stdatomic.h>
+
 #include "iarray_private.h"
 
 #define FAIL_NULL(p) \
@@ -577,7 +579,7 @@ static ina_rc_t check_decode_status(void) {
 /* ---- write stage ------------------------------------------------------------------------------- */
 typedef struct {
   iarray_container_t *c;
-  const uint8_t *packed;
+  uint8_t *packed;
   int filtered;
   int err;
 } wctx_t;
@@ -591,6 +593,73 @@ static void write_chunk_task(void *vctx, int64_t nc, int worker) {
   if (iax_encode_chunk(x->packed + nc * c->chunk_nbytes, (int32_t)c->chunk_nbytes, c->blocksize, c->itemsize,
                        &c->cfg, x->filtered, &c->chunks[nc], &c->csizes[nc]) != 0)
     x->err = 1;
+}
+
+typedef struct {
+  iax_task_fn fn;
+  void *ctx;
+  const int64_t *first; /* first task of each piece, np + 1 entries */
+  int np;
+  atomic_int landed[IAMX_COPY_EVENTS];
+  atomic_int err;
+} piped_t;
+
+/* Task wrapper: wait (once per piece) until the piece holding the task's
+ * bytes has landed, then run the task.  Tasks are handed out in order, so
+ * workers block only on the piece being copied right now. */
+static void piped_task(void *v, int64_t i, int worker) {
+  piped_t *p = (piped_t *)v;
+  int lo = 0, hi = p->np - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (p->first[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  if (!atomic_load(&p->landed[lo])) {
+    if (iamx_event_sync(lo) != 0) {
+      atomic_store(&p->err, 1);
+      return;
+    }
+    atomic_store(&p->landed[lo], 1);
+  }
+  p->fn(p->ctx, i, worker);
+}
+
+ina_rc_t iax_download_pipelined(const void *d_src, size_t total, const int64_t *task_end, int64_t ntasks,
+                                int nthreads, iax_task_fn fn, void *ctx, uint8_t **host) {
+  uint8_t *h = (uint8_t *)iamx_host_buffer(total + 16);
+  if (!h) return iax_dev_error(IAMX_ENOMEM, "pinned staging buffer");
+  *host = h;
+  const uint8_t *d = (const uint8_t *)d_src;
+  size_t target = total / 32;
+  if (target < ((size_t)4 << 20)) target = (size_t)4 << 20;
+  int64_t first[IAMX_COPY_EVENTS + 1];
+  int np = 0;
+  int64_t t = 0;
+  size_t start = 0;
+  while (t < ntasks) {
+    int64_t t1 = t;
+    while (t1 < ntasks && (size_t)task_end[t1] - start < target) t1++;
+    if (t1 < ntasks) t1++; /* the task that reaches the target closes the piece */
+    if (np == IAMX_COPY_EVENTS - 1) t1 = ntasks;
+    const size_t end = (size_t)task_end[t1 - 1];
+    const int e = iamx_d2h_async(h + start, d + start, end - start, np);
+    if (e) return iax_dev_error(e, "download");
+    first[np++] = t;
+    t = t1;
+    start = end;
+  }
+  first[np] = ntasks;
+  piped_t p;
+  p.fn = fn;
+  p.ctx = ctx;
+  p.first = first;
+  p.np = np;
+  for (int k = 0; k < IAMX_COPY_EVENTS; k++) atomic_init(&p.landed[k], 0);
+  atomic_init(&p.err, 0);
+  iax_parallel_for(nthreads, ntasks, piped_task, &p);
+  if (atomic_load(&p.err)) return iax_dev_error(IAMX_EINVAL, "download");
+  return INA_SUCCESS;
 }
 
 ina_rc_t iax_write_from_dense(iarray_container_t *c, const void *d_dense) {
@@ -611,18 +680,16 @@ ina_rc_t iax_write_from_dense(iarray_container_t *c, const void *d_dense) {
   if (!d_packed) return iax_dev_error(IAMX_ENOMEM, "pack buffer");
   int e = iamx_pack_blocks(&t, d_dense, c->shape[1], d_packed);
   if (e) return iax_dev_error(e, "pack");
-  uint8_t *host = (uint8_t *)malloc(total);
-  if (!host) return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
-  e = iamx_d2h(host, d_packed, total);
-  if (e) {
-    free(host);
-    return iax_dev_error(e, "download C blocks");
-  }
+  int64_t *ends = (int64_t *)malloc(sizeof(int64_t) * (size_t)c->nchunks);
+  if (!ends) return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+  for (int64_t i = 0; i < c->nchunks; i++) ends[i] = (i + 1) * c->chunk_nbytes;
   const double t0 = iax_now();
-  wctx_t x = {c, host, filtered, 0};
-  iax_parallel_for(iax_nthreads(&c->cfg), c->nchunks, write_chunk_task, &x);
+  wctx_t x = {c, NULL, filtered, 0};
+  ina_rc_t rc = iax_download_pipelined(d_packed, total, ends, c->nchunks, iax_nthreads(&c->cfg), write_chunk_task,
+                                       &x, &x.packed);
   iax_last_encode_s = iax_now() - t0;
-  free(host);
+  free(ends);
+  if (INA_FAILED(rc)) return rc;
   drop_device(c);
   if (x.err) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "chunk encode failed");
   return INA_SUCCESS;

@@ -88,6 +88,13 @@ typedef void (*iax_task_fn)(void *ctx, int64_t i, int worker);
 int iax_nthreads(const iarray_config_t *cfg);
 void iax_parallel_for(int nthreads, int64_t n, iax_task_fn fn, void *ctx);
 double iax_now(void);
+/* Download `total` bytes of d_src into the pinned staging buffer in pieces
+ * and run fn(ctx, task, worker) for each of `ntasks` tasks as soon as the
+ * piece holding its bytes (up to task_end[task]) has landed, so the copy of
+ * later pieces overlaps the work on earlier ones.  *host receives the
+ * staging buffer (owned by the device runtime). */
+ina_rc_t iax_download_pipelined(const void *d_src, size_t total, const int64_t *task_end, int64_t ntasks,
+                                int nthreads, iax_task_fn fn, void *ctx, uint8_t **host);
 
 /* ---- containers (iarray_container.c) ---------------------------------------------------- */
 ina_rc_t iax_container_new(iarray_context_t *ctx, iarray_dtshape_t *dtshape, iarray_storage_t *storage,
