@@ -141,24 +141,54 @@ __global__ __launch_bounds__(64) void k_lz4_streams(const iamx_lz4_job_t *__rest
   int bad = 0;
   for (;;) {
     refill(ip);
-    if (ip <= iend - 3 && op <= oend - 18) {
-      const uint32_t h = read32(ip);
-      const int32_t token = (int32_t)(h & 0xffu);
-      if (token < 15) {
-        const int32_t ml = token + 4;
-        const int32_t off = (int32_t)((h >> 8) & 0xffffu);
-        if (off >= ml && off <= min(op, NEAR)) {
-          const uint8_t v = ring[(op - off + lane) & RM];
-          __builtin_amdgcn_wave_barrier();
-          ring[(op + lane) & RM] = v;
-          __builtin_amdgcn_wave_barrier();
-          ip += 3;
-          op += ml;
-          if (op >= flush_at) flush(op);
-          continue;
+    // Fast loop: sequences with 0..14 literals and a 4..18 byte match whose
+    // header, literals and offset all lie in w0.  It runs while the input
+    // position stays 20 bytes short of w0's end and of the stream end, the
+    // output position 32 bytes short of the slot end and below the next
+    // flush; each test below is a difference whose sign bit flags failure.
+    {
+      const int32_t ilim = min(wb + 236, iend - 17);
+      const int32_t olim = min(oend - 32, flush_at - 1);
+      while (((ilim - ip) | (olim - op)) >= 0) {
+        const int32_t q = ip - wb;
+        const int32_t l = q >> 2;
+        const uint32_t h = (uint32_t)((((uint64_t)rdlane(w0, l + 1)) << 32 | rdlane(w0, l)) >> ((q & 3) << 3));
+        const int32_t lit = (int32_t)((h >> 4) & 15u);
+        const int32_t mlc = (int32_t)(h & 15u);
+        int32_t off;
+        if (lit == 0) {
+          off = (int32_t)((h >> 8) & 0xffffu);
+        } else {
+          const int32_t qo = q + 1 + lit;
+          const int32_t lo2 = qo >> 2;
+          off = (int32_t)(((((uint64_t)rdlane(w0, lo2 + 1)) << 32 | rdlane(w0, lo2)) >> ((qo & 3) << 3)) & 0xffffu);
         }
+        const int32_t o2 = op + lit;
+        if (((14 - lit) | (14 - mlc) | (off - 1) | (min(o2, NEAR) - off)) < 0) break;
+        if (lit) {
+          const int32_t ql = q + 1 + lane;  // < 256 + 64
+          const uint32_t a = (uint32_t)__shfl((int)w0, (ql >> 2) & 63, 64);
+          ring[(op + lane) & RM] = (uint8_t)(a >> ((ql & 3) << 3));
+          __builtin_amdgcn_wave_barrier();
+        }
+        int32_t back = off;
+        if (off < 64) {
+          const int32_t num = lane + off;
+          int32_t qq = (int32_t)((float)num * __builtin_amdgcn_rcpf((float)off));
+          const int32_t r = num - qq * off;
+          qq += (r >= off) - (r < 0);
+          back = off * qq;
+        }
+        const uint8_t v = ring[(o2 + lane - back) & RM];
+        __builtin_amdgcn_wave_barrier();
+        ring[(o2 + lane) & RM] = v;
+        __builtin_amdgcn_wave_barrier();
+        ip += 3 + lit;
+        op = o2 + mlc + 4;
       }
     }
+    if (op >= flush_at) flush(op);
+    refill(ip);
     if (ip >= iend) { bad = 1; break; }
     const uint32_t token = rbyte(ip);
     ++ip;
