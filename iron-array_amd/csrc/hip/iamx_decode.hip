@@ -118,17 +118,6 @@ __global__ __launch_bounds__(64) void k_lz4_streams(const iamx_lz4_job_t *__rest
     const uint32_t w = q < 256 ? rdlane(w0, q >> 2) : rdlane(w1, (q >> 2) - 64);
     return (w >> ((q & 3) << 3)) & 0xffu;
   };
-  // four bytes from input position pos, wb <= pos < wb + 256 (uniform)
-  auto read32 = [&](int32_t pos) -> uint32_t {
-    const int32_t q = pos - wb;
-    const int32_t l = q >> 2;
-    const uint32_t lo = rdlane(w0, l);
-    uint32_t hi;
-    if (l < 63) hi = rdlane(w0, l + 1);
-    else hi = rdlane(w1, 0);
-    return (uint32_t)(((((uint64_t)hi) << 32) | lo) >> ((q & 3) << 3));
-  };
-
   int32_t ip = (int32_t)(job.src & 3), op = 0, flushed = 0, fenced = 0, flush_at = FLUSH_AT;
   // store the whole dwords of ring bytes [flushed, cur) to the slot; a dword
   // is stored again only with the same bytes below `flushed`

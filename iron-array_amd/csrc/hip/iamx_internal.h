@@ -15,6 +15,9 @@ int set_error(int code, const char *fmt, ...);
 int check_hip(hipError_t e, const char *what);
 int ensure_init();
 
+// True between iamx_capture_begin and iamx_capture_end.
+bool capturing();
+
 // Device word the decode kernels OR their error bits into.
 int *status_word();
 

@@ -23,6 +23,7 @@
 //   the epilogue applies 2^(e_i+f_j).  FP64 arithmetic appears only in that
 //   recombination and in the exponent scaling.
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include "iamx_internal.h"
@@ -311,7 +312,211 @@ __global__ __launch_bounds__(256) void k_seg_gemm(const SegArgs P) {
     }
 }
 
+// ---- the segmented GEMM, large tiles --------------------------------------------------------
+// 256x256 output tile per workgroup, 4 wavefronts of 128x128 (4x4 MFMA 32x32
+// tiles, 256 accumulators per lane), 64 operand bytes of K per LDS stage (two
+// 32-byte MFMA k-steps), the next stage prefetched to registers.  Per byte
+// read from LDS a wavefront does twice the MFMA work of the 128x128 kernel.
+// In OZAKI mode one launch handles one level (its segments share the level);
+// the epilogue adds 2^(e_i+f_j-7L) * acc into C (or writes it, for the first
+// level), so levels are still summed from the least significant one up.
+constexpr int BM = 256, BN = 256, KB2 = 64, ROW2 = KB2 + 16;
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef int i32x16 __attribute__((ext_vector_type(16)));
+
+template <int MODE>
+struct Acc32;
+template <>
+struct Acc32<0> {
+  typedef f32x16 t;
+};
+template <>
+struct Acc32<1> {
+  typedef i32x16 t;
+};
+
+template <int MODE>
+__device__ __forceinline__ typename Acc32<MODE>::t mma32(const i32x4 &a, const i32x4 &b, typename Acc32<MODE>::t c);
+template <>
+__device__ __forceinline__ f32x16 mma32<0>(const i32x4 &a, const i32x4 &b, f32x16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                 0, 0);
+}
+template <>
+__device__ __forceinline__ i32x16 mma32<1>(const i32x4 &a, const i32x4 &b, i32x16 c) {
+  return __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c, 0, 0, 0);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_seg_gemm_big(const SegArgs P, int level, int accumulate) {
+  __shared__ __attribute__((aligned(16))) uint8_t As[BM * ROW2];
+  __shared__ __attribute__((aligned(16))) uint8_t Bs[BN * ROW2];
+  typedef typename Acc32<MODE>::t acc_t;
+  const int tm = blockIdx.x / P.tiles_n, tn = blockIdx.x % P.tiles_n;
+  const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  // global -> LDS: 256 rows x 64 bytes per operand = 1024 x 16B; 4 per thread
+  int lr[4], lc[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int idx = tid + 256 * i;
+    lr[i] = idx >> 2;
+    lc[i] = (idx & 3) * 16;
+  }
+  i32x4 ra[4], rb[4];
+  const int64_t ktiles = P.Kb / KB2;
+  const int64_t total = (int64_t)P.nseg * ktiles;
+  auto gload = [&](int64_t it) {
+    const Seg &sg = P.seg[it / ktiles];
+    const int64_t kb = (it % ktiles) * KB2;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      ra[i] = *reinterpret_cast<const i32x4 *>(P.A + sg.a_off + (m0 + lr[i]) * P.lda + kb + lc[i]);
+      rb[i] = *reinterpret_cast<const i32x4 *>(P.B + sg.b_off + (n0 + lr[i]) * P.ldb + kb + lc[i]);
+    }
+  };
+  auto sstore = [&]() {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      *reinterpret_cast<i32x4 *>(&As[lr[i] * ROW2 + lc[i]]) = ra[i];
+      *reinterpret_cast<i32x4 *>(&Bs[lr[i] * ROW2 + lc[i]]) = rb[i];
+    }
+  };
+  acc_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = acc_t{};
+  gload(0);
+  sstore();
+  __syncthreads();
+  const int rsub = lane & 31, ksub = (lane >> 5) * 16;
+  for (int64_t it = 0; it < total; it++) {
+    if (it + 1 < total) gload(it + 1);
+#pragma unroll
+    for (int ks = 0; ks < KB2; ks += 32) {
+      i32x4 a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+        a[i] = *reinterpret_cast<const i32x4 *>(&As[(wr * 128 + i * 32 + rsub) * ROW2 + ks + ksub]);
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        b[j] = *reinterpret_cast<const i32x4 *>(&Bs[(wc * 128 + j * 32 + rsub) * ROW2 + ks + ksub]);
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = mma32<MODE>(a[i], b[j], acc[i][j]);
+    }
+    __syncthreads();
+    if (it + 1 < total) {
+      sstore();
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int64_t col = n0 + wc * 128 + j * 32 + rsub;
+#pragma unroll
+      for (int v = 0; v < 16; v++) {
+        const int64_t row = m0 + wr * 128 + i * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        if (row < P.M && col < P.N) {
+          if (MODE == 0) {
+            reinterpret_cast<float *>(P.C)[row * P.ldc + col] = (float)acc[i][j][v];
+          } else {
+            double *c = reinterpret_cast<double *>(P.C) + row * P.ldc + col;
+            const double t = ldexp((double)acc[i][j][v], P.ea[row] + P.fb[col] - 7 * level);
+            *c = accumulate ? *c + t : t;
+          }
+        }
+      }
+      // one 32x32 tile at a time: keeps the scheduler from holding every
+      // C value of the epilogue in registers at once
+      __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
 int64_t rup(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
+
+// ---- tile autotuner ------------------------------------------------------------------------
+// The first product of a given (form, M, N, K) class (each dimension rounded
+// up to a power of two) runs with both tile shapes, timed with HIP events on
+// the library stream; the faster one is kept for that class and, so that
+// every call of a class returns the same bits, re-run once if it was not the
+// last one tried.  IAMX_TILE=128 or 256 pins the choice.
+struct TuneEntry {
+  int form;
+  int64_t m, n, k;
+  int tile;
+};
+static TuneEntry g_tune[64];
+static int g_ntune = 0;
+
+static int64_t pow2_up(int64_t v) {
+  int64_t p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+static int forced_tile() {
+  static const int t = [] {
+    const char *e = getenv("IAMX_TILE");
+    const int v = e ? atoi(e) : 0;
+    return (v == 128 || v == 256) ? v : 0;
+  }();
+  return t;
+}
+
+// run(tile) enqueues the product with the given tile shape and returns an
+// IAMX status; returns the tile used.
+template <typename F>
+static int tuned(int form, int64_t M, int64_t N, int64_t K, F run, int *rc) {
+  if (forced_tile()) {
+    *rc = run(forced_tile());
+    return forced_tile();
+  }
+  if (M < BM || N < BN) {  // a 256-tile launch would be mostly padding
+    *rc = run(128);
+    return 128;
+  }
+  const int64_t m = pow2_up(M), n = pow2_up(N), k = pow2_up(K);
+  for (int i = 0; i < g_ntune; i++)
+    if (g_tune[i].form == form && g_tune[i].m == m && g_tune[i].n == n && g_tune[i].k == k) {
+      *rc = run(g_tune[i].tile);
+      return g_tune[i].tile;
+    }
+  if (iamx::capturing()) {  // no timing inside a graph capture
+    *rc = run(256);
+    return 256;
+  }
+  hipEvent_t ev[3];
+  for (auto &e : ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      *rc = run(256);
+      return 256;
+    }
+  const int tiles[2] = {128, 256};
+  float ms[2] = {0, 0};
+  *rc = IAMX_OK;
+  (void)hipEventRecord(ev[0], iamx::g_stream);
+  for (int t = 0; t < 2 && !*rc; t++) {
+    *rc = run(tiles[t]);
+    (void)hipEventRecord(ev[t + 1], iamx::g_stream);
+  }
+  int best = 256;
+  if (!*rc && hipEventSynchronize(ev[2]) == hipSuccess) {
+    (void)hipEventElapsedTime(&ms[0], ev[0], ev[1]);
+    (void)hipEventElapsedTime(&ms[1], ev[1], ev[2]);
+    best = ms[0] < ms[1] ? 128 : 256;
+    if (best != 256) *rc = run(best);
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  if (!*rc && g_ntune < 64) g_tune[g_ntune++] = TuneEntry{form, m, n, k, best};
+  return best;
+}
 
 }  // namespace
 
@@ -320,7 +525,7 @@ extern "C" {
 // float32: C = A B through the bf16 hi/lo split (3 products).
 int iamx_gemm_split_bf16(int64_t M, int64_t N, int64_t K, const float *A, int64_t lda, const float *B, int64_t ldb,
                          float *C, int64_t ldc) {
-  const int64_t Mp = rup(M, TM), Np = rup(N, TN), Kp = rup(K, KB / 2);
+  const int64_t Mp = rup(M, BM), Np = rup(N, BN), Kp = rup(K, KB / 2);
   const size_t a_bytes = (size_t)(Mp * Kp * 2), b_bytes = (size_t)(Np * Kp * 2);
   uint8_t *ws = (uint8_t *)iamx_workspace(1, 2 * a_bytes + 2 * b_bytes + 256);
   if (!ws) return iamx::set_error(IAMX_ENOMEM, "split-bf16 workspace");
@@ -341,13 +546,26 @@ int iamx_gemm_split_bf16(int64_t M, int64_t N, int64_t K, const float *A, int64_
   P.N = N;
   P.C = C;
   P.ldc = ldc;
-  P.tiles_n = (int)(Np / TN);
   P.nseg = 3;  // small products first: lo*hi, hi*lo, then hi*hi
   P.seg[0] = Seg{(int64_t)a_bytes, 0, 0, 0};
   P.seg[1] = Seg{0, (int64_t)b_bytes, 0, 0};
   P.seg[2] = Seg{0, 0, 0, 0};
-  IAMX_LAUNCH("gemm_split_bf16_mfma", k_seg_gemm<0>, dim3((unsigned)((Mp / TM) * (Np / TN))), dim3(256), 0, P);
-  return IAMX_OK;
+  int rc = IAMX_OK;
+  tuned(0, M, N, K,
+        [&](int tile) -> int {
+          if (tile == 256) {
+            P.tiles_n = (int)(Np / BN);
+            IAMX_LAUNCH("gemm_split_bf16_mfma", k_seg_gemm_big<0>, dim3((unsigned)((Mp / BM) * (Np / BN))),
+                        dim3(256), 0, P, 0, 0);
+          } else {
+            P.tiles_n = (int)(Np / TN);
+            IAMX_LAUNCH("gemm_split_bf16_mfma", k_seg_gemm<0>, dim3((unsigned)((Mp / TM) * (Np / TN))), dim3(256),
+                        0, P);
+          }
+          return IAMX_OK;
+        },
+        &rc);
+  return rc;
 }
 
 // float64: C = A B through Ozaki INT8 slicing with S slices.
@@ -356,7 +574,7 @@ int iamx_gemm_ozaki_i8(int64_t M, int64_t N, int64_t K, const double *A, int64_t
   if (S < 2 || S > 8) return iamx::set_error(IAMX_EINVAL, "ozaki: slices %d outside 2..8", S);
   if ((double)K * S * 64.0 * 64.0 >= 2147483647.0)
     return iamx::set_error(IAMX_EINVAL, "ozaki: K=%lld too large for exact INT32 level sums", (long long)K);
-  const int64_t Mp = rup(M, TM), Np = rup(N, TN), Kp = rup(K, KB);
+  const int64_t Mp = rup(M, BM), Np = rup(N, BN), Kp = rup(K, KB);
   const size_t a_bytes = (size_t)(S * Mp * Kp), b_bytes = (size_t)(S * Np * Kp);
   uint8_t *ws = (uint8_t *)iamx_workspace(1, a_bytes + b_bytes + (size_t)(Mp + Np) * 4 + 256);
   if (!ws) return iamx::set_error(IAMX_ENOMEM, "ozaki workspace");
@@ -384,21 +602,43 @@ int iamx_gemm_ozaki_i8(int64_t M, int64_t N, int64_t K, const double *A, int64_t
   P.ldc = ldc;
   P.ea = ea;
   P.fb = fb;
-  P.tiles_n = (int)(Np / TN);
-  int n = 0;
-  // smallest levels last would lose them in FP64 rounding order; add from the
-  // least significant level (S+1) up to the most significant (2)
-  for (int L = S + 1; L >= 2; L--) {
-    for (int s = 1; s <= S; s++) {
-      const int t = L - s;
-      if (t < 1 || t > S) continue;
-      P.seg[n++] = Seg{(int64_t)(s - 1) * Mp * Kp, (int64_t)(t - 1) * Np * Kp, L, 0};
-    }
-    P.seg[n - 1].level_end = 1;
-  }
-  P.nseg = n;
-  IAMX_LAUNCH("gemm_ozaki_i8_mfma", k_seg_gemm<1>, dim3((unsigned)((Mp / TM) * (Np / TN))), dim3(256), 0, P);
-  return IAMX_OK;
+  // levels are added from the least significant (S+1) up to the most
+  // significant (2), so small contributions are not lost to FP64 rounding first
+  int rc = IAMX_OK;
+  tuned(1, M, N, K * S,
+        [&](int tile) -> int {
+          if (tile == 256) {
+            P.tiles_n = (int)(Np / BN);
+            for (int L = S + 1; L >= 2; L--) {
+              int n = 0;
+              for (int s = 1; s <= S; s++) {
+                const int t = L - s;
+                if (t < 1 || t > S) continue;
+                P.seg[n++] = Seg{(int64_t)(s - 1) * Mp * Kp, (int64_t)(t - 1) * Np * Kp, L, 0};
+              }
+              P.nseg = n;
+              IAMX_LAUNCH("gemm_ozaki_i8_mfma", k_seg_gemm_big<1>, dim3((unsigned)((Mp / BM) * (Np / BN))),
+                          dim3(256), 0, P, L, L != S + 1);
+            }
+          } else {
+            P.tiles_n = (int)(Np / TN);
+            int n = 0;
+            for (int L = S + 1; L >= 2; L--) {
+              for (int s = 1; s <= S; s++) {
+                const int t = L - s;
+                if (t < 1 || t > S) continue;
+                P.seg[n++] = Seg{(int64_t)(s - 1) * Mp * Kp, (int64_t)(t - 1) * Np * Kp, L, 0};
+              }
+              P.seg[n - 1].level_end = 1;
+            }
+            P.nseg = n;
+            IAMX_LAUNCH("gemm_ozaki_i8_mfma", k_seg_gemm<1>, dim3((unsigned)((Mp / TM) * (Np / TN))), dim3(256), 0,
+                        P);
+          }
+          return IAMX_OK;
+        },
+        &rc);
+  return rc;
 }
 
 }  // extern "C"

@@ -89,6 +89,7 @@ static hipEvent_t get_event() {
 }
 
 static bool g_capturing = false;
+bool capturing() { return g_capturing; }
 
 ProfScope::ProfScope(const char *name) {
   if (!g_prof_on || g_capturing) return;
