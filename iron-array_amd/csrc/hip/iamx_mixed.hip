@@ -229,15 +229,22 @@ __global__ __launch_bounds__(256) void k_seg_gemm(const SegArgs P) {
     lc[i] = (idx & 7) * 16;
   }
   i32x4 ra[4], rb[4];
-  const int64_t ktiles = P.Kb / KB;
-  const int64_t total = (int64_t)P.nseg * ktiles;
-  auto gload = [&](int64_t it) {
-    const Seg &s = P.seg[it / ktiles];
-    const int64_t kb = (it % ktiles) * KB;
+  const int ktiles = (int)(P.Kb / KB);
+  const int total = P.nseg * ktiles;
+  // stage fetched next: segment lseg, byte lk of K (advanced without divisions)
+  int lseg = 0;
+  int64_t lk = 0;
+  auto gload = [&]() {
+    const Seg &s = P.seg[lseg];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      ra[i] = *reinterpret_cast<const i32x4 *>(P.A + s.a_off + (m0 + lr[i]) * P.lda + kb + lc[i]);
-      rb[i] = *reinterpret_cast<const i32x4 *>(P.B + s.b_off + (n0 + lr[i]) * P.ldb + kb + lc[i]);
+      ra[i] = *reinterpret_cast<const i32x4 *>(P.A + s.a_off + (m0 + lr[i]) * P.lda + lk + lc[i]);
+      rb[i] = *reinterpret_cast<const i32x4 *>(P.B + s.b_off + (n0 + lr[i]) * P.ldb + lk + lc[i]);
+    }
+    lk += KB;
+    if (lk == P.Kb) {
+      lk = 0;
+      lseg++;
     }
   };
   auto sstore = [&](int buf) {
@@ -258,12 +265,13 @@ __global__ __launch_bounds__(256) void k_seg_gemm(const SegArgs P) {
 #pragma unroll
         for (int v = 0; v < 4; v++) acc64[i][j][v] = 0.0;
     }
-  gload(0);
+  gload();
   sstore(0);
   __syncthreads();
-  for (int64_t it = 0; it < total; it++) {
-    const int cur = (int)(it & 1);
-    if (it + 1 < total) gload(it + 1);
+  int cseg = 0, ck = 0;  // segment and k-stage being multiplied
+  for (int it = 0; it < total; it++) {
+    const int cur = it & 1;
+    if (it + 1 < total) gload();
 #pragma unroll
     for (int kk = 0; kk < KB; kk += 64) {
       i32x4 a[4], b[4];
@@ -280,8 +288,8 @@ __global__ __launch_bounds__(256) void k_seg_gemm(const SegArgs P) {
         for (int j = 0; j < 4; j++) acc[i][j] = mma<MODE>(a[i], b[j], acc[i][j]);
     }
     if (MODE == 1) {
-      const Seg &s = P.seg[it / ktiles];
-      if (s.level_end && (it % ktiles) == ktiles - 1) {
+      const Seg &s = P.seg[cseg];
+      if (s.level_end && ck == ktiles - 1) {
         const double w = ldexp(1.0, -7 * s.level);
 #pragma unroll
         for (int i = 0; i < 4; i++)
@@ -292,6 +300,10 @@ __global__ __launch_bounds__(256) void k_seg_gemm(const SegArgs P) {
             acc[i][j] = acc_t{0, 0, 0, 0};
           }
       }
+    }
+    if (++ck == ktiles) {
+      ck = 0;
+      cseg++;
     }
     if (it + 1 < total) sstore(cur ^ 1);
     __syncthreads();
@@ -314,13 +326,19 @@ __global__ __launch_bounds__(256) void k_seg_gemm(const SegArgs P) {
 
 // ---- the segmented GEMM, large tiles --------------------------------------------------------
 // 256x256 output tile per workgroup, 4 wavefronts of 128x128 (4x4 MFMA 32x32
-// tiles, 256 accumulators per lane), 64 operand bytes of K per LDS stage (two
-// 32-byte MFMA k-steps), the next stage prefetched to registers.  Per byte
-// read from LDS a wavefront does twice the MFMA work of the 128x128 kernel.
+// tiles, 256 accumulators per lane in AGPRs).  K advances 128 operand bytes
+// per stage (four 32-byte MFMA k-steps, 64 MFMAs per wavefront); stages are
+// double-buffered in LDS (two 73.7 KB buffers of the 160 KB a gfx950
+// workgroup may hold), stage s+2 is fetched into registers while stage s is
+// multiplied and stage s+1 sits in LDS, so there is one barrier per stage.
+// Per byte read from LDS a wavefront does twice the MFMA work of the 128x128
+// kernel.  Tiles are grouped by XCD (consecutive workgroups go to different
+// XCDs; each XCD gets a contiguous run of tiles) and, within that run, in
+// 4-row column-major groups so concurrent tiles share A and B panels in L2.
 // In OZAKI mode one launch handles one level (its segments share the level);
 // the epilogue adds 2^(e_i+f_j-7L) * acc into C (or writes it, for the first
 // level), so levels are still summed from the least significant one up.
-constexpr int BM = 256, BN = 256, KB2 = 64, ROW2 = KB2 + 16;
+constexpr int BM = 256, BN = 256, KB2 = 128, ROW2 = KB2 + 16;
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef int i32x16 __attribute__((ext_vector_type(16)));
@@ -349,39 +367,56 @@ __device__ __forceinline__ i32x16 mma32<1>(const i32x4 &a, const i32x4 &b, i32x1
 }
 
 template <int MODE>
-__global__ __launch_bounds__(256) void k_seg_gemm_big(const SegArgs P, int level, int accumulate) {
-  __shared__ __attribute__((aligned(16))) uint8_t As[BM * ROW2];
-  __shared__ __attribute__((aligned(16))) uint8_t Bs[BN * ROW2];
+__global__ __launch_bounds__(256) void k_seg_gemm_big(const SegArgs P, int level, int accumulate, int tiles_m) {
+  constexpr int BUF = (BM + BN) * ROW2;
+  __shared__ __attribute__((aligned(16))) uint8_t lds[2 * BUF];  // [2][A 256 rows | B 256 rows] x ROW2
   typedef typename Acc32<MODE>::t acc_t;
-  const int tm = blockIdx.x / P.tiles_n, tn = blockIdx.x % P.tiles_n;
+  int id = blockIdx.x;
+  const int nt = tiles_m * P.tiles_n;
+  if (nt % 8 == 0) id = (id % 8) * (nt / 8) + id / 8;
+  // grouped order: consecutive ids walk 4 tile rows, then the next column, so
+  // the tiles running together share few A and B panels
+  constexpr int GM = 4;
+  const int group = id / (GM * P.tiles_n);
+  const int first_m = group * GM;
+  const int gm = min(tiles_m - first_m, GM);
+  const int within = id - group * GM * P.tiles_n;
+  const int tm = first_m + within % gm, tn = within / gm;
   const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
-  // global -> LDS: 256 rows x 64 bytes per operand = 1024 x 16B; 4 per thread
-  int lr[4], lc[4];
+  // global -> LDS: (256 A + 256 B) rows x 128 bytes = 4096 x 16B; 16 per thread
+  // (8 rows of A, 8 of B; 8 threads cover one 128-byte row)
+  const int lrow = tid >> 3, lcol = (tid & 7) * 16;
+  const int ktiles = (int)(P.Kb / KB2);
+  const int total = P.nseg * ktiles;
+  // stage fetched next: segment lseg, byte lk of K (advanced without divisions)
+  int lseg = 0;
+  int64_t lk = 0;
+  // two register sets: stage s is fetched into set s&1 two stages before it is
+  // stored to LDS, so global latency has two stages of MFMA work to hide behind
+  i32x4 r0[16], r1[16];
+  auto gload = [&](i32x4 *r) {
+    const Seg &sg = P.seg[lseg];
+    const uint8_t *pa = P.A + sg.a_off + (m0 + lrow) * P.lda + lk + lcol;
+    const uint8_t *pb = P.B + sg.b_off + (n0 + lrow) * P.ldb + lk + lcol;
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const int idx = tid + 256 * i;
-    lr[i] = idx >> 2;
-    lc[i] = (idx & 3) * 16;
-  }
-  i32x4 ra[4], rb[4];
-  const int64_t ktiles = P.Kb / KB2;
-  const int64_t total = (int64_t)P.nseg * ktiles;
-  auto gload = [&](int64_t it) {
-    const Seg &sg = P.seg[it / ktiles];
-    const int64_t kb = (it % ktiles) * KB2;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      ra[i] = *reinterpret_cast<const i32x4 *>(P.A + sg.a_off + (m0 + lr[i]) * P.lda + kb + lc[i]);
-      rb[i] = *reinterpret_cast<const i32x4 *>(P.B + sg.b_off + (n0 + lr[i]) * P.ldb + kb + lc[i]);
+    for (int i = 0; i < 8; i++) {
+      r[i] = *reinterpret_cast<const i32x4 *>(pa + (int64_t)(32 * i) * P.lda);
+      r[8 + i] = *reinterpret_cast<const i32x4 *>(pb + (int64_t)(32 * i) * P.ldb);
+    }
+    lk += KB2;
+    if (lk == P.Kb) {
+      lk = 0;
+      lseg++;
     }
   };
-  auto sstore = [&]() {
+  auto sstore = [&](const i32x4 *r, int buf) {
+    uint8_t *base = lds + buf * BUF;
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      *reinterpret_cast<i32x4 *>(&As[lr[i] * ROW2 + lc[i]]) = ra[i];
-      *reinterpret_cast<i32x4 *>(&Bs[lr[i] * ROW2 + lc[i]]) = rb[i];
+    for (int i = 0; i < 8; i++) {
+      *reinterpret_cast<i32x4 *>(base + (lrow + 32 * i) * ROW2 + lcol) = r[i];
+      *reinterpret_cast<i32x4 *>(base + (BM + lrow + 32 * i) * ROW2 + lcol) = r[8 + i];
     }
   };
   acc_t acc[4][4];
@@ -389,32 +424,57 @@ __global__ __launch_bounds__(256) void k_seg_gemm_big(const SegArgs P, int level
   for (int i = 0; i < 4; i++)
 #pragma unroll
     for (int j = 0; j < 4; j++) acc[i][j] = acc_t{};
-  gload(0);
-  sstore();
-  __syncthreads();
   const int rsub = lane & 31, ksub = (lane >> 5) * 16;
-  for (int64_t it = 0; it < total; it++) {
-    if (it + 1 < total) gload(it + 1);
+  // multiply the stage held in LDS buffer `cur`
+  auto compute = [&](int cur) {
+    const uint8_t *As = lds + cur * BUF + (wr * 128 + rsub) * ROW2 + ksub;
+    const uint8_t *Bs = lds + cur * BUF + (BM + wc * 128 + rsub) * ROW2 + ksub;
+    // fragments of k-step ks+1 are read from LDS while the 16 MFMAs of k-step
+    // ks run (two register sets), so LDS latency is hidden behind MFMA work
+    i32x4 fa[2][4], fb[2][4];
 #pragma unroll
-    for (int ks = 0; ks < KB2; ks += 32) {
-      i32x4 a[4], b[4];
-#pragma unroll
-      for (int i = 0; i < 4; i++)
-        a[i] = *reinterpret_cast<const i32x4 *>(&As[(wr * 128 + i * 32 + rsub) * ROW2 + ks + ksub]);
-#pragma unroll
-      for (int j = 0; j < 4; j++)
-        b[j] = *reinterpret_cast<const i32x4 *>(&Bs[(wc * 128 + j * 32 + rsub) * ROW2 + ks + ksub]);
-#pragma unroll
-      for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = mma32<MODE>(a[i], b[j], acc[i][j]);
+    for (int i = 0; i < 4; i++) {
+      fa[0][i] = *reinterpret_cast<const i32x4 *>(As + i * 32 * ROW2);
+      fb[0][i] = *reinterpret_cast<const i32x4 *>(Bs + i * 32 * ROW2);
     }
+#pragma unroll
+    for (int step = 0; step < KB2 / 32; step++) {
+      const int c = step & 1;
+      if (step + 1 < KB2 / 32) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          fa[c ^ 1][i] = *reinterpret_cast<const i32x4 *>(As + i * 32 * ROW2 + (step + 1) * 32);
+          fb[c ^ 1][i] = *reinterpret_cast<const i32x4 *>(Bs + i * 32 * ROW2 + (step + 1) * 32);
+        }
+      }
+      // keep the reads above the MFMAs: the scheduler would otherwise sink
+      // them next to their use and expose LDS latency every 4 MFMAs
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = mma32<MODE>(fa[c][i], fb[c][j], acc[i][j]);
+    }
+  };
+  // one stage: store stage it+1 (fetched two stages ago) into the free
+  // buffer, fetch stage it+3 into the set just stored, multiply stage it
+  auto stage = [&](int it, i32x4 *rnext) {
+    if (it + 1 < total) sstore(rnext, (it + 1) & 1);
+    if (it + 3 < total) gload(rnext);
+    compute(it & 1);
     __syncthreads();
-    if (it + 1 < total) {
-      sstore();
-      __syncthreads();
-    }
+  };
+  gload(r0);                       // stage 0
+  if (total > 1) gload(r1);        // stage 1
+  sstore(r0, 0);
+  if (total > 2) gload(r0);        // stage 2
+  __syncthreads();
+  int it = 0;
+  for (; it + 1 < total; it += 2) {
+    stage(it, r1);      // stores stage it+1 from r1, fetches it+3 into r1
+    stage(it + 1, r0);  // stores stage it+2 from r0, fetches it+4 into r0
   }
+  if (it < total) stage(it, r1);
 #pragma unroll
   for (int i = 0; i < 4; i++)
 #pragma unroll
@@ -556,7 +616,7 @@ int iamx_gemm_split_bf16(int64_t M, int64_t N, int64_t K, const float *A, int64_
           if (tile == 256) {
             P.tiles_n = (int)(Np / BN);
             IAMX_LAUNCH("gemm_split_bf16_mfma", k_seg_gemm_big<0>, dim3((unsigned)((Mp / BM) * (Np / BN))),
-                        dim3(256), 0, P, 0, 0);
+                        dim3(256), 0, P, 0, 0, (int)(Mp / BM));
           } else {
             P.tiles_n = (int)(Np / TN);
             IAMX_LAUNCH("gemm_split_bf16_mfma", k_seg_gemm<0>, dim3((unsigned)((Mp / TM) * (Np / TN))), dim3(256),
@@ -618,7 +678,7 @@ int iamx_gemm_ozaki_i8(int64_t M, int64_t N, int64_t K, const double *A, int64_t
               }
               P.nseg = n;
               IAMX_LAUNCH("gemm_ozaki_i8_mfma", k_seg_gemm_big<1>, dim3((unsigned)((Mp / BM) * (Np / BN))),
-                          dim3(256), 0, P, L, L != S + 1);
+                          dim3(256), 0, P, L, L != S + 1, (int)(Mp / BM));
             }
           } else {
             P.tiles_n = (int)(Np / TN);
