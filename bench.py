@@ -37,7 +37,7 @@ def parse_args():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--workload", default="gemv20k",
                    choices=["gemv20k", "gemm2k_f64", "gemm8k_f32", "gemm6k_f64", "batch512", "hbm_copy"])
-    p.add_argument("--slices", type=int, default=0, help="Ozaki slice count (0 = library default)")
+    p.add_argument("--slices", type=int, default=0, help="Ozaki slice count (0 = autotuned to 1e-13)")
     p.add_argument("--precision", default="plain", choices=["plain", "split_bf16", "ozaki_int8"])
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-threads", type=int, default=0)
@@ -172,6 +172,10 @@ def main():
     ms = (t1 - t0) * 1e3 / args.steps
     prof = ia.prof_entries()
     enc_s, _ = ia.last_matmul_times()
+    if args.precision == "ozaki_int8":
+        S, _ = ia.last_ozaki_slices()
+        config["ozaki_slices"] = S
+        config["ozaki_slices_chosen_by"] = "argument" if args.slices else "sampled-error autotune (target 1e-13)"
     ia.prof_enable(False)
     print(f"# kernels: {json.dumps({k: [v[0], round(v[1] / max(v[0], 1), 4)] for k, v in prof.items()})}",
           file=sys.stderr, flush=True)

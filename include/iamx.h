@@ -50,6 +50,7 @@ IAMX_API int iamx_d2h_async(void *dst, const void *src, size_t bytes, int event)
 IAMX_API int iamx_event_sync(int event);
 /* Scratch that lives until iamx_shutdown; grown (never shrunk) on demand.
  * Slot s (0..7) is independent of the others. */
+#define IAMX_WORKSPACES 10 /* slots 0..7 belong to the host library, 8..9 to the Ozaki slice tuner */
 IAMX_API void *iamx_workspace(int slot, size_t bytes);
 IAMX_API void iamx_shutdown(void);
 
@@ -100,6 +101,13 @@ IAMX_API int iamx_decode(const iamx_lz4_job_t *d_lz4, int n_lz4, const iamx_bloc
  * (src/linalg/iarray_gemm.c:276-294): one launch computes the whole dense
  * C(MxN) = A(MxK) B(KxN), row-major.  dtype 0 = float64, 1 = float32.
  * precision: 0 plain, 1 split-bf16 (float32), 2 ozaki-int8 (float64, `slices`). */
+/* Smallest Ozaki slice count S in [smin, 8] whose product reaches a
+ * componentwise error max |C_S - C| / (|A||B|) <= target on a sample of up
+ * to 256 evenly spaced rows of A, measured against the plain FP64 MFMA product
+ * of the same rows (S = 8 if none does).  *err receives the sampled error of
+ * the chosen S.  Replaces nothing in the reference (its FP64 path is MKL). */
+IAMX_API int iamx_ozaki_pick_slices(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B,
+                                    int64_t ldb, double target, int smin, int *slices, double *err);
 IAMX_API int iamx_gemm(int dtype, int precision, int slices, int64_t M, int64_t N, int64_t K, const void *A,
                        int64_t lda, const void *B, int64_t ldb, void *C, int64_t ldc);
 

@@ -76,6 +76,14 @@ INA_API(bool) iarray_prof_entry(int i, const char **name, int64_t *launches, dou
  * the last matmul, and device milliseconds from the first to the last kernel. */
 INA_API(void) iarray_last_matmul_times(double *host_encode_s, double *device_ms);
 
+/* Slice count used by the last OZAKI_INT8 product (0 if none ran) and, when
+ * it was picked automatically (ozaki_slices = 0), the componentwise error
+ * max |C_S - C| / (|A||B|) measured on the sampled rows (else -1).  The
+ * automatic pick is the smallest S >= 4 whose sampled error is at most
+ * IARRAY_OZAKI_TARGET (environment, default 1e-13), cached per (A, B) pair
+ * while neither container changes. */
+INA_API(int) iarray_last_ozaki_slices(double *sampled_err);
+
 /* Device description (name, CU count, gfx arch string). */
 INA_API(ina_rc_t) iarray_device_info(char *name, int namelen, int *cus, char *arch, int archlen);
 

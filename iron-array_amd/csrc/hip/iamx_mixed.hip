@@ -499,6 +499,43 @@ __global__ __launch_bounds__(256) void k_seg_gemm_big(const SegArgs P, int level
     }
 }
 
+// ---- Ozaki slice tuner helpers ---------------------------------------------------------------
+// rows r*M/R of A (R x K, contiguous) and their absolute values
+__global__ __launch_bounds__(256) void k_sample_rows(const double *__restrict__ A, int64_t lda, int64_t M, int64_t K,
+                                                     int R, double *__restrict__ As, double *__restrict__ Aa) {
+  const int r = blockIdx.y;
+  const int64_t src = (int64_t)r * M / R;
+  for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < K; k += (int64_t)gridDim.x * 256) {
+    const double v = A[src * lda + k];
+    As[(int64_t)r * K + k] = v;
+    Aa[(int64_t)r * K + k] = fabs(v);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_abs_rows(const double *__restrict__ B, int64_t ldb, int64_t rows, int64_t cols,
+                                                  double *__restrict__ Ba) {
+  const int64_t n = rows * cols;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const int64_t r = i / cols, c = i - r * cols;
+    Ba[i] = fabs(B[r * ldb + c]);
+  }
+}
+
+// max over entries of |X - Ref| / den (den > 0), folded into *out as the bit
+// pattern of a non-negative double (which orders like the unsigned integer)
+__global__ __launch_bounds__(256) void k_err_max(const double *__restrict__ X, const double *__restrict__ Ref,
+                                                 const double *__restrict__ den, int64_t n,
+                                                 unsigned long long *__restrict__ out) {
+  double m = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    if (den[i] > 0.0) m = fmax(m, fabs(X[i] - Ref[i]) / den[i]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+}
+
+__global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
+
 int64_t rup(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
 
 // ---- tile autotuner ------------------------------------------------------------------------
@@ -699,6 +736,47 @@ int iamx_gemm_ozaki_i8(int64_t M, int64_t N, int64_t K, const double *A, int64_t
         },
         &rc);
   return rc;
+}
+
+IAMX_API int iamx_ozaki_pick_slices(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B,
+                                    int64_t ldb, double target, int smin, int *slices, double *err) {
+  int rc = iamx::ensure_init();
+  if (rc) return rc;
+  if (!slices || M <= 0 || N <= 0 || K <= 0) return iamx::set_error(IAMX_EINVAL, "ozaki tuner: bad arguments");
+  if (smin < 2) smin = 2;
+  if (smin > 8) smin = 8;
+  const int R = (int)(M < 256 ? M : 256);
+  const size_t rk = (size_t)R * (size_t)K, rn = (size_t)R * (size_t)N;
+  uint8_t *w = (uint8_t *)iamx_workspace(8, (2 * rk + 3 * rn) * sizeof(double) + 256);
+  double *Ba = (double *)iamx_workspace(9, (size_t)K * (size_t)N * sizeof(double) + 256);
+  if (!w || !Ba) return iamx::set_error(IAMX_ENOMEM, "ozaki tuner workspace");
+  double *As = (double *)w, *Aa = As + rk, *Cref = Aa + rk, *den = Cref + rn, *Cs = den + rn;
+  unsigned long long *d_err = (unsigned long long *)(Cs + rn);
+  const unsigned gx = (unsigned)((K + 255) / 256 < 64 ? (K + 255) / 256 : 64);
+  IAMX_LAUNCH("ozaki_tune_sample", k_sample_rows, dim3(gx, (unsigned)R), dim3(256), 0, A, lda, M, K, R, As, Aa);
+  IAMX_LAUNCH("ozaki_tune_abs", k_abs_rows, dim3(2048), dim3(256), 0, B, ldb, K, N, Ba);
+  rc = iamx_gemm(0, 0, 0, R, N, K, As, K, B, ldb, Cref, N);
+  if (!rc) rc = iamx_gemm(0, 0, 0, R, N, K, Aa, K, Ba, N, den, N);
+  if (rc) return rc;
+  int chosen = 8;
+  double chosen_err = -1.0;
+  for (int S = smin; S <= 8; S++) {
+    rc = iamx_gemm_ozaki_i8(R, N, K, As, K, B, ldb, Cs, N, S);
+    if (rc) return rc;
+    IAMX_LAUNCH("ozaki_tune_err", k_zero_u64, dim3(1), dim3(1), 0, d_err);
+    IAMX_LAUNCH("ozaki_tune_err", k_err_max, dim3(256), dim3(256), 0, Cs, Cref, den, (int64_t)rn, d_err);
+    unsigned long long bits = 0;
+    rc = iamx_d2h(&bits, d_err, sizeof(bits));
+    if (rc) return rc;
+    double e;
+    memcpy(&e, &bits, sizeof(e));
+    chosen = S;
+    chosen_err = e;
+    if (e <= target) break;
+  }
+  *slices = chosen;
+  if (err) *err = chosen_err;
+  return IAMX_OK;
 }
 
 }  // extern "C"

@@ -141,7 +141,7 @@ struct Workspace {
   void *ptr = nullptr;
   size_t bytes = 0;
 };
-static Workspace g_ws[8];
+static Workspace g_ws[IAMX_WORKSPACES];
 
 }  // namespace iamx
 
@@ -269,7 +269,7 @@ IAMX_API int iamx_sync(void) {
 }
 
 IAMX_API void *iamx_workspace(int slot, size_t bytes) {
-  if (slot < 0 || slot >= 8 || ensure_init()) return nullptr;
+  if (slot < 0 || slot >= IAMX_WORKSPACES || ensure_init()) return nullptr;
   Workspace &w = g_ws[slot];
   if (w.bytes >= bytes && w.ptr) return w.ptr;
   if (w.ptr) {

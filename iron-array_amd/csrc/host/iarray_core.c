@@ -17,6 +17,8 @@ static int g_initialized = 0;
 static __thread char g_msg[512];
 double iax_last_encode_s = 0.0;
 double iax_last_device_ms = 0.0;
+int iax_last_slices = 0;
+double iax_last_slice_err = -1.0;
 
 ina_rc_t iax_set_error(ina_rc_t rc, const char *fmt, ...) {
   va_list ap;
@@ -92,6 +94,11 @@ INA_API(void) iarray_prof_enable(bool on) { iamx_prof_enable(on ? 1 : 0); }
 INA_API(void) iarray_prof_reset(void) { iamx_prof_reset(); }
 INA_API(bool) iarray_prof_entry(int i, const char **name, int64_t *launches, double *total_ms) {
   return iamx_prof_entry(i, name, launches, total_ms) != 0;
+}
+
+INA_API(int) iarray_last_ozaki_slices(double *sampled_err) {
+  if (sampled_err) *sampled_err = iax_last_slice_err;
+  return iax_last_slices;
 }
 
 INA_API(void) iarray_last_matmul_times(double *host_encode_s, double *device_ms) {

@@ -15,6 +15,8 @@
  */
 #include <string.h>
 
+#include <stdlib.h>
+
 #include "iarray_private.h"
 
 static int dtype_code(iarray_data_type_t t) {
@@ -45,6 +47,27 @@ static ina_rc_t run_gemm(iarray_container_t *a, iarray_container_t *b, iarray_co
   if (INA_FAILED(rc)) return rc;
   void *dC = iamx_workspace(4, (size_t)(M * N * a->itemsize) + 16);
   if (!dC) return iax_dev_error(IAMX_ENOMEM, "C buffer");
+  if (prec == IARRAY_MATMUL_OZAKI_INT8) {
+    iax_last_slice_err = -1.0;
+    if (slices == 0) {
+      if (a->tuned_b == b && a->tuned_gen_a == a->generation && a->tuned_gen_b == b->generation &&
+          a->tuned_slices > 0) {
+        slices = a->tuned_slices;
+      } else {
+        const char *env = getenv("IARRAY_OZAKI_TARGET");
+        const double target = env && atof(env) > 0 ? atof(env) : 1e-13;
+        double err = -1.0;
+        int e = iamx_ozaki_pick_slices(M, N, K, (const double *)dA, K, (const double *)dB, N, target, 4, &slices, &err);
+        if (e) return iax_dev_error(e, "ozaki slice tuning");
+        a->tuned_b = b;
+        a->tuned_gen_a = a->generation;
+        a->tuned_gen_b = b->generation;
+        a->tuned_slices = slices;
+        iax_last_slice_err = err;
+      }
+    }
+    iax_last_slices = slices;
+  }
   int e = iamx_gemm(dt, prec, slices, M, N, K, dA, K, dB, N, dC, N);
   if (e) return iax_dev_error(e, "gemm");
   rc = iax_write_from_dense(c, dC);

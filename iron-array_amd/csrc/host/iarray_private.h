@@ -60,6 +60,11 @@ struct iarray_container_s {
   /* views */
   iarray_container_t *viewed;
   bool transposed;
+  /* Ozaki slice count picked for this container as A against one B: valid
+   * while both generations are unchanged */
+  const iarray_container_t *tuned_b;
+  uint64_t tuned_gen_a, tuned_gen_b;
+  int tuned_slices;
 };
 
 /* ---- codec (blosc_codec.c) ---------------------------------------------------- */
@@ -117,5 +122,7 @@ ina_rc_t iax_dev_error(int code, const char *what);
 
 extern double iax_last_encode_s;
 extern double iax_last_device_ms;
+extern int iax_last_slices;
+extern double iax_last_slice_err;
 
 #endif

@@ -114,6 +114,7 @@ def lib():
         "iarray_prof_enable": ([ctypes.c_bool], None), "iarray_prof_reset": ([], None),
         "iarray_prof_entry": ([ctypes.c_int, P(ctypes.c_char_p), P(i64), P(ctypes.c_double)], ctypes.c_bool),
         "iarray_last_matmul_times": ([P(ctypes.c_double), P(ctypes.c_double)], None),
+        "iarray_last_ozaki_slices": ([P(ctypes.c_double)], ctypes.c_int),
         "iarray_device_info": ([ctypes.c_char_p, ctypes.c_int, P(ctypes.c_int), ctypes.c_char_p, ctypes.c_int], rc_t),
     }
     for name, (args, res) in sig.items():
@@ -352,6 +353,14 @@ def prof_entries():
         out[name.value.decode()] = (n.value, ms.value)
         i += 1
     return out
+
+
+def last_ozaki_slices():
+    """(slices used by the last OZAKI_INT8 product, sampled componentwise error
+    of an automatic pick or -1)."""
+    e = ctypes.c_double()
+    s = lib().iarray_last_ozaki_slices(ctypes.byref(e))
+    return s, e.value
 
 
 def last_matmul_times():

@@ -90,6 +90,32 @@ def test_ozaki_int8_accuracy(orc, shape, slices, bar):
     assert cm <= bar
 
 
+def test_ozaki_auto_slices_reach_target(orc):
+    """slices=0 picks the smallest S whose sampled componentwise error is at
+    most 1e-13; the full product must then meet the bar on every entry."""
+    M, K, N = 300, 700, 260
+    ctx = ia.Context(clevel=5, filters=ia.COMP_SHUFFLE)
+    rng = np.random.default_rng(77)
+    a = _wide(rng, (M, K))
+    b = _wide(rng, (K, N))
+    ca = ia.Container.from_numpy(ctx, a, (150, 350), (50, 70))
+    cb = ia.Container.from_numpy(ctx, b, (350, 130), (70, 65))
+    c = ia.matmul(ca, cb, (150, 130), (50, 65), precision=ia.OZAKI_INT8, slices=0).to_numpy()
+    S, err = ia.last_ozaki_slices()
+    cm, nm = _metrics(orc, a, b, c)
+    print(f"auto slices: S={S} sampled {err:.3e} full componentwise {cm:.3e}")
+    assert 4 <= S <= 8
+    assert 0 <= err <= 1e-13 or S == 8
+    assert cm <= 1e-13
+    rows, cols = np.meshgrid(np.arange(M), np.arange(N), indexing="ij")
+    ref = orc.dd_gemm_sample(a, b, rows.ravel(), cols.ravel()).reshape(M, N)
+    assert np.all(np.abs(c - ref) <= _ozaki_bound(a, b, S))
+    # the pick is cached for the pair: the second call reuses it, no sample error
+    c2 = ia.matmul(ca, cb, (150, 130), (50, 65), precision=ia.OZAKI_INT8, slices=0).to_numpy()
+    assert ia.last_ozaki_slices() == (S, -1.0)
+    assert c2.tobytes() == c.tobytes()
+
+
 def test_ozaki_exact_on_small_integers(orc):
     """Integers below 2^7 are one slice: every form must be exact."""
     ctx = ia.Context(clevel=5, filters=ia.COMP_SHUFFLE)
