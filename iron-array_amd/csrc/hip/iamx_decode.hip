@@ -176,7 +176,10 @@ __global__ __launch_bounds__(64) void k_lz4_streams(const iamx_lz4_job_t *__rest
         op = o2 + mlc + 4;
       }
     }
-    if (op >= flush_at) flush(op);
+    if (op >= flush_at) {
+      flush(op);
+      continue;  // the fast loop again
+    }
     refill(ip);
     if (ip >= iend) { bad = 1; break; }
     const uint32_t token = rbyte(ip);
