@@ -50,7 +50,10 @@ IAMX_API int iamx_d2h_async(void *dst, const void *src, size_t bytes, int event)
 IAMX_API int iamx_event_sync(int event);
 /* Scratch that lives until iamx_shutdown; grown (never shrunk) on demand.
  * Slot s (0..7) is independent of the others. */
-#define IAMX_WORKSPACES 10 /* slots 0..7 belong to the host library, 8..9 to the Ozaki slice tuner */
+/* slots 0..7 belong to the host library, 8..9 to the Ozaki slice tuner, 1 and
+ * 10..11 to the mixed-precision products (B planes of one-shot products, B
+ * planes of the row-panel interface, planes of the current A panel) */
+#define IAMX_WORKSPACES 12
 IAMX_API void *iamx_workspace(int slot, size_t bytes);
 IAMX_API void iamx_shutdown(void);
 
@@ -111,6 +114,17 @@ IAMX_API int iamx_ozaki_pick_slices(int64_t M, int64_t N, int64_t K, const doubl
 IAMX_API int iamx_gemm(int dtype, int precision, int slices, int64_t M, int64_t N, int64_t K, const void *A,
                        int64_t lda, const void *B, int64_t ldb, void *C, int64_t ldc);
 
+/* Row-panel form of iamx_gemm for callers that write C out panel by panel
+ * while later panels are still being computed (the reference computes and
+ * compresses C one chunk at a time, src/linalg/iarray_gemm.c:399-425).
+ * iamx_gemm_prepare_b fixes B (K x N) and, for the mixed precisions, builds
+ * its split or sliced planes once; each iamx_gemm_rows call then computes
+ * C[0:M, :] = A[0:M, :] B, with A and C pointing at the panel's first row.
+ * B must stay resident and unchanged until the last iamx_gemm_rows call. */
+IAMX_API int iamx_gemm_prepare_b(int dtype, int precision, int slices, int64_t N, int64_t K, const void *B,
+                                 int64_t ldb);
+IAMX_API int iamx_gemm_rows(int64_t M, const void *A, int64_t lda, void *C, int64_t ldc);
+
 /* Replaces the per-C-block cblas_?gemv of the gemv branch
  * (src/linalg/matmul/gemv.c:92-98) and of iarray_opt_gemv
  * (src/linalg/iarray_gemv.c:212-224): y(M) = A(MxK) x(K). */
@@ -145,6 +159,11 @@ typedef struct iamx_tiling_s {
 } iamx_tiling_t;
 
 IAMX_API int iamx_pack_blocks(const iamx_tiling_t *tiling, const void *d_src, int64_t src_ld, void *d_out);
+/* The same for chunks [chunk_begin, chunk_end) of the chunk grid (row-major
+ * chunk order), written at their place in `out`; d_src is still the whole
+ * dense C, and only the rows of those chunks are read. */
+IAMX_API int iamx_pack_blocks_range(const iamx_tiling_t *tiling, const void *d_src, int64_t src_ld, void *d_out,
+                                    int64_t chunk_begin, int64_t chunk_end);
 
 /* Transpose a dense row-major (rows x cols) matrix: the omatcopy of a
  * transposed view (src/iarray_container.c:718-723). */

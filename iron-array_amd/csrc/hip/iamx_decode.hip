@@ -51,7 +51,8 @@ __device__ __forceinline__ uint32_t rdlane(uint32_t v, int32_t l) {
 //   output  decoded bytes go to an LDS ring of RING bytes and reach the
 //           stream's scratch slot in dword stores whenever RING/2 bytes are
 //           pending.  Literal runs longer than 64 bytes are copied global ->
-//           global and only their last RING bytes enter the ring.
+//           global in 16-byte groups per lane (dword loads realigned with
+//           alignbyte) and also written to the ring in order.
 //   matches with offset <= RING-64 are copied ring -> ring, 64 bytes per step,
 //           each lane reading `off * ceil((lane+1)/off)` bytes back (`off`
 //           back when off >= 64), so a step never reads what it writes.
@@ -70,7 +71,7 @@ __global__ __launch_bounds__(64) void k_lz4_streams(const iamx_lz4_job_t *__rest
   constexpr int32_t RM = RING - 1;
   constexpr int32_t NEAR = RING - 64;
   constexpr int32_t FLUSH_AT = RING / 2;
-  constexpr int32_t BULK = 1024;  // bytes per step of a long literal run
+  constexpr int kRunUnroll = 4;  // 16-byte groups in flight per lane in a long literal run
   const int j = blockIdx.x;
   if (j >= njobs) return;
   const int lane = threadIdx.x;
@@ -81,6 +82,7 @@ __global__ __launch_bounds__(64) void k_lz4_streams(const iamx_lz4_job_t *__rest
   IAMX_GLOBAL uint8_t *dst = iamx::gptr<uint8_t>((uint64_t)(scratch + job.dst_off));
   IAMX_GLOBAL uint32_t *dstw = iamx::gptr<uint32_t>((uint64_t)(scratch + job.dst_off));
   const uint32_t *ring32 = reinterpret_cast<const uint32_t *>(ring);
+  uint32_t *ring32w = reinterpret_cast<uint32_t *>(ring);
   const int32_t iend = (int32_t)(job.src & 3) + job.csize;
   const int32_t oend = job.nbytes;
   const int32_t lastw = (iend - 1) >> 2;  // last word holding stream bytes
@@ -212,27 +214,64 @@ __global__ __launch_bounds__(64) void k_lz4_streams(const iamx_lz4_job_t *__rest
         if (op >= flush_at) flush(op);
       } else {
         // long run: what is pending goes out first, then the run is copied
-        // global -> global; the window for the next header is requested now
+        // global -> global, 16 bytes a lane, the source realigned to the
+        // destination's dwords with alignbyte; every run byte is also written
+        // to the ring in position order, so the ring ends holding the run's
+        // last RING bytes.  The window for the next header is requested now.
         flush(op);
         if (lane < op - flushed) dst[flushed + lane] = ring[(flushed + lane) & RM];
         const int32_t nip = ip + lit;
         if (nip - wb >= 256) jump(nip);
-        const int32_t keep = lit - RING;  // run bytes before this index never reach the ring
-        for (int32_t k = 0; k < lit; k += BULK) {
-          uint8_t v[BULK / 64];
+        const int32_t head = min((4 - (op & 3)) & 3, lit);  // bytes up to the first destination dword
+        const int32_t ng = (lit - head) >> 4;                // whole 16-byte groups after the head
+        const int32_t bend = head + (ng << 4);
+        if (lane < head) {
+          const uint8_t v = sb[ip + lane];
+          dst[op + lane] = v;
+          ring[(op + lane) & RM] = v;
+        }
+        const int32_t d0 = op + head;  // multiple of 4
+        const int32_t sw0 = (ip + head) >> 2;
+        const uint32_t sh = (uint32_t)((ip + head) & 3);  // the same for every group
+        // one group: words k..k+3 hold run bytes, word k+4 only when sh > 0
+        auto group_load = [&](int32_t g, uint32_t *w) {
+          const int32_t k = sw0 + 4 * g;
+          w[0] = sw[k];
+          w[1] = sw[k + 1];
+          w[2] = sw[k + 2];
+          w[3] = sw[k + 3];
+          w[4] = sw[min(k + 4, lastw)];
+        };
+        auto group_store = [&](int32_t g, const uint32_t *w) {
+          const int32_t d = d0 + 16 * g;
+          uint32_t o[4];
 #pragma unroll
-          for (int t = 0; t < BULK / 64; t++) {
-            const int32_t i = k + 64 * t + lane;
-            v[t] = i < lit ? sb[ip + i] : (uint8_t)0;
-          }
+          for (int t = 0; t < 4; t++) o[t] = __builtin_amdgcn_alignbyte(w[t + 1], w[t], sh);
 #pragma unroll
-          for (int t = 0; t < BULK / 64; t++) {
-            const int32_t i = k + 64 * t + lane;
-            if (i < lit) {
-              dst[op + i] = v[t];
-              if (i >= keep) ring[(op + i) & RM] = v[t];
-            }
+          for (int t = 0; t < 4; t++) dstw[(d >> 2) + t] = o[t];
+#pragma unroll
+          for (int t = 0; t < 4; t++) ring32w[((d + 4 * t) & RM) >> 2] = o[t];
+        };
+        int32_t g0 = 0;
+        for (; g0 + 64 * kRunUnroll <= ng; g0 += 64 * kRunUnroll) {  // every lane has all groups
+          uint32_t w[kRunUnroll][5];
+#pragma unroll
+          for (int u = 0; u < kRunUnroll; u++) group_load(g0 + 64 * u + lane, w[u]);
+#pragma unroll
+          for (int u = 0; u < kRunUnroll; u++) group_store(g0 + 64 * u + lane, w[u]);
+        }
+        for (; g0 < ng; g0 += 64) {
+          if (g0 + lane < ng) {
+            uint32_t w[5];
+            group_load(g0 + lane, w);
+            group_store(g0 + lane, w);
           }
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (lane < lit - bend) {  // fewer than 16 tail bytes, after the groups (ring order)
+          const uint8_t v = sb[ip + bend + lane];
+          dst[op + bend + lane] = v;
+          ring[(op + bend + lane) & RM] = v;
         }
         __builtin_amdgcn_wave_barrier();
         ip = nip;
@@ -433,9 +472,9 @@ __device__ __forceinline__ uint64_t load_elem(const uint8_t *src, int ts, int64_
 // One workgroup per C block: gather the tile (zeros outside the valid extent),
 // apply the filter, write the block's bytes at (chunk * nblocks + block) * bsize.
 __global__ __launch_bounds__(256) void k_pack(iamx_tiling_t t, const uint8_t *__restrict__ src, int64_t ld,
-                                              uint8_t *__restrict__ out) {
+                                              uint8_t *__restrict__ out, int64_t id0) {
   const int64_t nbpc = t.nblocks[0] * t.nblocks[1];
-  const int64_t id = blockIdx.x;
+  const int64_t id = id0 + blockIdx.x;
   const int64_t nc = id / nbpc, nb = id % nbpc;
   const int64_t ci0 = nc / t.nchunks[1], ci1 = nc % t.nchunks[1];
   const int64_t bi0 = nb / t.nblocks[1], bi1 = nb % t.nblocks[1];
@@ -551,16 +590,27 @@ IAMX_API int iamx_decode(const iamx_lz4_job_t *d_lz4, int n_lz4, const iamx_bloc
   return IAMX_OK;
 }
 
-IAMX_API int iamx_pack_blocks(const iamx_tiling_t *t, const void *d_src, int64_t src_ld, void *d_out) {
+IAMX_API int iamx_pack_blocks_range(const iamx_tiling_t *t, const void *d_src, int64_t src_ld, void *d_out,
+                                    int64_t chunk_begin, int64_t chunk_end) {
   int rc = iamx::ensure_init();
   if (rc) return rc;
-  if (!t || t->typesize < 1 || t->typesize > 8)
+  if (!t || t->typesize < 1 || t->typesize > 8 || t->block[0] <= 0 || t->block[1] <= 0)
     return iamx::set_error(IAMX_EINVAL, "iamx_pack_blocks: bad tiling");
-  const int64_t nblk = t->nchunks[0] * t->nchunks[1] * t->nblocks[0] * t->nblocks[1];
-  if (nblk <= 0 || nblk > INT32_MAX) return iamx::set_error(IAMX_EINVAL, "iamx_pack_blocks: bad block count");
+  const int64_t nchunks = t->nchunks[0] * t->nchunks[1], nbpc = t->nblocks[0] * t->nblocks[1];
+  if (chunk_begin < 0 || chunk_end > nchunks || chunk_begin > chunk_end)
+    return iamx::set_error(IAMX_EINVAL, "iamx_pack_blocks: chunk range [%lld, %lld) outside [0, %lld)",
+                           (long long)chunk_begin, (long long)chunk_end, (long long)nchunks);
+  const int64_t nblk = (chunk_end - chunk_begin) * nbpc;
+  if (nblk == 0) return IAMX_OK;
+  if (nblk < 0 || nblk > INT32_MAX) return iamx::set_error(IAMX_EINVAL, "iamx_pack_blocks: bad block count");
   IAMX_LAUNCH("pack_blocks", k_pack, dim3((unsigned)nblk), dim3(256), 0, *t, (const uint8_t *)d_src, src_ld,
-              (uint8_t *)d_out);
+              (uint8_t *)d_out, chunk_begin * nbpc);
   return IAMX_OK;
+}
+
+IAMX_API int iamx_pack_blocks(const iamx_tiling_t *t, const void *d_src, int64_t src_ld, void *d_out) {
+  if (!t) return iamx::set_error(IAMX_EINVAL, "iamx_pack_blocks: bad tiling");
+  return iamx_pack_blocks_range(t, d_src, src_ld, d_out, 0, t->nchunks[0] * t->nchunks[1]);
 }
 
 IAMX_API int iamx_transpose(int typesize, int64_t rows, int64_t cols, const void *d_src, void *d_dst) {

@@ -157,6 +157,8 @@ int iamx_gemm_split_bf16(int64_t M, int64_t N, int64_t K, const float *A, int64_
                          float *C, int64_t ldc);
 int iamx_gemm_ozaki_i8(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B, int64_t ldb,
                        double *C, int64_t ldc, int S);
+int iamx_mixed_prepare_b(int form, int64_t N, int64_t K, const void *B, int64_t ldb, int S);
+int iamx_mixed_rows(int form, int64_t M, const void *A, int64_t lda, void *C, int64_t ldc);
 
 IAMX_API int iamx_gemm(int dtype, int precision, int slices, int64_t M, int64_t N, int64_t K, const void *A,
                        int64_t lda, const void *B, int64_t ldb, void *C, int64_t ldc) {
@@ -187,6 +189,49 @@ IAMX_API int iamx_gemm(int dtype, int precision, int slices, int64_t M, int64_t 
   else
     return iamx::set_error(IAMX_EINVAL, "iamx_gemm: dtype %d", dtype);
   return IAMX_OK;
+}
+
+// B operand fixed by iamx_gemm_prepare_b for the iamx_gemm_rows calls that follow
+static struct {
+  int dtype, precision;
+  int64_t N, K, ldb;
+  const void *B;
+  bool valid;
+} g_rows;
+
+IAMX_API int iamx_gemm_prepare_b(int dtype, int precision, int slices, int64_t N, int64_t K, const void *B,
+                                 int64_t ldb) {
+  int rc = iamx::ensure_init();
+  if (rc) return rc;
+  g_rows.valid = false;
+  if (N <= 0 || K < 0 || !B) return iamx::set_error(IAMX_EINVAL, "iamx_gemm_prepare_b: bad shape %lld x %lld",
+                                                     (long long)K, (long long)N);
+  if (dtype != 0 && dtype != 1) return iamx::set_error(IAMX_EINVAL, "iamx_gemm_prepare_b: dtype %d", dtype);
+  if (precision == 1 && dtype != 1) return iamx::set_error(IAMX_EINVAL, "iamx_gemm: split-bf16 needs float32");
+  if (precision == 2 && dtype != 0) return iamx::set_error(IAMX_EINVAL, "iamx_gemm: ozaki-int8 needs float64");
+  if (precision < 0 || precision > 2)
+    return iamx::set_error(IAMX_EINVAL, "iamx_gemm: precision %d not available", precision);
+  if (precision != 0) {
+    rc = iamx_mixed_prepare_b(precision, N, K, B, ldb, precision == 2 ? (slices > 0 ? slices : 7) : 0);
+    if (rc) return rc;
+  }
+  g_rows.dtype = dtype;
+  g_rows.precision = precision;
+  g_rows.N = N;
+  g_rows.K = K;
+  g_rows.ldb = ldb;
+  g_rows.B = B;
+  g_rows.valid = true;
+  return IAMX_OK;
+}
+
+IAMX_API int iamx_gemm_rows(int64_t M, const void *A, int64_t lda, void *C, int64_t ldc) {
+  int rc = iamx::ensure_init();
+  if (rc) return rc;
+  if (!g_rows.valid) return iamx::set_error(IAMX_EINVAL, "iamx_gemm_rows: no B operand prepared");
+  if (M <= 0) return iamx::set_error(IAMX_EINVAL, "iamx_gemm_rows: bad row count %lld", (long long)M);
+  if (g_rows.precision != 0) return iamx_mixed_rows(g_rows.precision, M, A, lda, C, ldc);
+  return iamx_gemm(g_rows.dtype, 0, 0, M, g_rows.N, g_rows.K, A, lda, g_rows.B, g_rows.ldb, C, ldc);
 }
 
 IAMX_API int iamx_gemv(int dtype, int64_t M, int64_t K, const void *A, int64_t lda, const void *x, void *y) {

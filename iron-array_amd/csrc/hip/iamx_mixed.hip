@@ -615,93 +615,115 @@ static int tuned(int form, int64_t M, int64_t N, int64_t K, F run, int *rc) {
   return best;
 }
 
-}  // namespace
+// ---- prepared B operand and row panels -------------------------------------------------------
+// The B side of a mixed product (its bf16 hi/lo planes, or its column
+// exponents and int8 digit planes) is built once by mixed_prepare_b into a
+// workspace slot of its own; mixed_rows then multiplies any row panel of A by
+// it (the panel's planes in slot 11), so C can be produced panel by panel.
+// One-shot products keep their B in slot 1 and the row-panel interface in
+// slot 10, so a one-shot product between two panels leaves the panels' B intact.
+struct PrepB {
+  int form;  // 1 split-bf16, 2 ozaki
+  int64_t N, K, Np, Kp;
+  int S;
+  const uint8_t *planes;  // B planes (transposed), then the column exponents (ozaki)
+  const int32_t *fb;
+  bool valid;
+};
+static PrepB g_pb;  // the row-panel interface's B
 
-extern "C" {
-
-// float32: C = A B through the bf16 hi/lo split (3 products).
-int iamx_gemm_split_bf16(int64_t M, int64_t N, int64_t K, const float *A, int64_t lda, const float *B, int64_t ldb,
-                         float *C, int64_t ldc) {
-  const int64_t Mp = rup(M, BM), Np = rup(N, BN), Kp = rup(K, KB / 2);
-  const size_t a_bytes = (size_t)(Mp * Kp * 2), b_bytes = (size_t)(Np * Kp * 2);
-  uint8_t *ws = (uint8_t *)iamx_workspace(1, 2 * a_bytes + 2 * b_bytes + 256);
-  if (!ws) return iamx::set_error(IAMX_ENOMEM, "split-bf16 workspace");
-  uint16_t *ahi = (uint16_t *)ws, *alo = (uint16_t *)(ws + a_bytes);
-  uint16_t *bhi = (uint16_t *)(ws + 2 * a_bytes), *blo = (uint16_t *)(ws + 2 * a_bytes + b_bytes);
-  IAMX_LAUNCH("split_bf16_A", k_split_bf16, dim3((unsigned)(Kp / 32), (unsigned)(Mp / 32)), dim3(256), 0, A, M, K,
-              lda, 0, ahi, alo, Mp, Kp);
-  IAMX_LAUNCH("split_bf16_B", k_split_bf16, dim3((unsigned)(Kp / 32), (unsigned)(Np / 32)), dim3(256), 0, B, K, N,
-              ldb, 1, bhi, blo, Np, Kp);
-  SegArgs P;
-  memset(&P, 0, sizeof(P));
-  P.A = ws;
-  P.B = ws + 2 * a_bytes;
-  P.lda = Kp * 2;
-  P.ldb = Kp * 2;
-  P.Kb = Kp * 2;
-  P.M = M;
-  P.N = N;
-  P.C = C;
-  P.ldc = ldc;
-  P.nseg = 3;  // small products first: lo*hi, hi*lo, then hi*hi
-  P.seg[0] = Seg{(int64_t)a_bytes, 0, 0, 0};
-  P.seg[1] = Seg{0, (int64_t)b_bytes, 0, 0};
-  P.seg[2] = Seg{0, 0, 0, 0};
-  int rc = IAMX_OK;
-  tuned(0, M, N, K,
-        [&](int tile) -> int {
-          if (tile == 256) {
-            P.tiles_n = (int)(Np / BN);
-            IAMX_LAUNCH("gemm_split_bf16_mfma", k_seg_gemm_big<0>, dim3((unsigned)((Mp / BM) * (Np / BN))),
-                        dim3(256), 0, P, 0, 0, (int)(Mp / BM));
-          } else {
-            P.tiles_n = (int)(Np / TN);
-            IAMX_LAUNCH("gemm_split_bf16_mfma", k_seg_gemm<0>, dim3((unsigned)((Mp / TM) * (Np / TN))), dim3(256),
-                        0, P);
-          }
-          return IAMX_OK;
-        },
-        &rc);
-  return rc;
-}
-
-// float64: C = A B through Ozaki INT8 slicing with S slices.
-int iamx_gemm_ozaki_i8(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B, int64_t ldb,
-                       double *C, int64_t ldc, int S) {
+static int mixed_prepare_b(PrepB &pb, int slot, int form, int64_t N, int64_t K, const void *B, int64_t ldb, int S) {
+  pb.valid = false;
+  const int64_t Np = rup(N, BN);
+  if (form == 1) {
+    const int64_t Kp = rup(K, KB / 2);
+    const size_t b_bytes = (size_t)(Np * Kp * 2);
+    uint8_t *ws = (uint8_t *)iamx_workspace(slot, 2 * b_bytes + 256);
+    if (!ws) return iamx::set_error(IAMX_ENOMEM, "split-bf16 workspace");
+    IAMX_LAUNCH("split_bf16_B", k_split_bf16, dim3((unsigned)(Kp / 32), (unsigned)(Np / 32)), dim3(256), 0,
+                (const float *)B, K, N, ldb, 1, (uint16_t *)ws, (uint16_t *)(ws + b_bytes), Np, Kp);
+    pb = PrepB{1, N, K, Np, Kp, 0, ws, nullptr, true};
+    return IAMX_OK;
+  }
   if (S < 2 || S > 8) return iamx::set_error(IAMX_EINVAL, "ozaki: slices %d outside 2..8", S);
   if ((double)K * S * 64.0 * 64.0 >= 2147483647.0)
     return iamx::set_error(IAMX_EINVAL, "ozaki: K=%lld too large for exact INT32 level sums", (long long)K);
-  const int64_t Mp = rup(M, BM), Np = rup(N, BN), Kp = rup(K, KB);
-  const size_t a_bytes = (size_t)(S * Mp * Kp), b_bytes = (size_t)(S * Np * Kp);
-  uint8_t *ws = (uint8_t *)iamx_workspace(1, a_bytes + b_bytes + (size_t)(Mp + Np) * 4 + 256);
+  const int64_t Kp = rup(K, KB);
+  const size_t b_bytes = (size_t)(S * Np * Kp);
+  uint8_t *ws = (uint8_t *)iamx_workspace(slot, b_bytes + (size_t)Np * 4 + 256);
   if (!ws) return iamx::set_error(IAMX_ENOMEM, "ozaki workspace");
-  int8_t *da = (int8_t *)ws, *db = (int8_t *)(ws + a_bytes);
-  int32_t *ea = (int32_t *)(ws + a_bytes + b_bytes), *fb = ea + Mp;
-  IAMX_LAUNCH("ozaki_row_exp", k_row_exp, dim3((unsigned)((M + 3) / 4)), dim3(256), 0, A, M, K, lda, ea);
+  int32_t *fb = (int32_t *)(ws + b_bytes);
+  const double *Bd = (const double *)B;
   IAMX_LAUNCH("ozaki_col_exp", k_fill_i32, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, fb, N, INT32_MIN);
   IAMX_LAUNCH("ozaki_col_exp", k_col_exp, dim3((unsigned)((N + 63) / 64), (unsigned)((K + kColSlab - 1) / kColSlab)),
-              dim3(256), 0, B, K, N, ldb, fb);
+              dim3(256), 0, Bd, K, N, ldb, fb);
   IAMX_LAUNCH("ozaki_col_exp", k_col_exp_zero, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, fb, N);
-  IAMX_LAUNCH("ozaki_slice_A", k_slice_i8, dim3((unsigned)(Kp / 32), (unsigned)(Mp / 32)), dim3(256), 0, A, M, K,
-              lda, 0, ea, S, da, Mp, Kp);
-  IAMX_LAUNCH("ozaki_slice_B", k_slice_i8, dim3((unsigned)(Kp / 32), (unsigned)(Np / 32)), dim3(256), 0, B, K, N,
-              ldb, 1, fb, S, db, Np, Kp);
+  IAMX_LAUNCH("ozaki_slice_B", k_slice_i8, dim3((unsigned)(Kp / 32), (unsigned)(Np / 32)), dim3(256), 0, Bd, K, N,
+              ldb, 1, fb, S, (int8_t *)ws, Np, Kp);
+  pb = PrepB{2, N, K, Np, Kp, S, ws, fb, true};
+  return IAMX_OK;
+}
+
+// C[0:M, :] = A[0:M, :] B for the prepared B
+static int mixed_rows(const PrepB &pb, int form, int64_t M, const void *A, int64_t lda, void *C, int64_t ldc) {
+  if (!pb.valid || pb.form != form) return iamx::set_error(IAMX_EINVAL, "mixed product: B not prepared");
+  const int64_t N = pb.N, K = pb.K, Np = pb.Np, Kp = pb.Kp;
+  const int64_t Mp = rup(M, BM);
   SegArgs P;
   memset(&P, 0, sizeof(P));
-  P.A = (const uint8_t *)da;
-  P.B = (const uint8_t *)db;
-  P.lda = Kp;
-  P.ldb = Kp;
-  P.Kb = Kp;
+  P.B = pb.planes;
   P.M = M;
   P.N = N;
   P.C = C;
   P.ldc = ldc;
+  int rc = IAMX_OK;
+  if (form == 1) {
+    const size_t a_bytes = (size_t)(Mp * Kp * 2), b_bytes = (size_t)(Np * Kp * 2);
+    uint8_t *ws = (uint8_t *)iamx_workspace(11, 2 * a_bytes + 256);
+    if (!ws) return iamx::set_error(IAMX_ENOMEM, "split-bf16 workspace");
+    IAMX_LAUNCH("split_bf16_A", k_split_bf16, dim3((unsigned)(Kp / 32), (unsigned)(Mp / 32)), dim3(256), 0,
+                (const float *)A, M, K, lda, 0, (uint16_t *)ws, (uint16_t *)(ws + a_bytes), Mp, Kp);
+    P.A = ws;
+    P.lda = Kp * 2;
+    P.ldb = Kp * 2;
+    P.Kb = Kp * 2;
+    P.nseg = 3;  // small products first: lo*hi, hi*lo, then hi*hi
+    P.seg[0] = Seg{(int64_t)a_bytes, 0, 0, 0};
+    P.seg[1] = Seg{0, (int64_t)b_bytes, 0, 0};
+    P.seg[2] = Seg{0, 0, 0, 0};
+    tuned(0, M, N, K,
+          [&](int tile) -> int {
+            if (tile == 256) {
+              P.tiles_n = (int)(Np / BN);
+              IAMX_LAUNCH("gemm_split_bf16_mfma", k_seg_gemm_big<0>, dim3((unsigned)((Mp / BM) * (Np / BN))),
+                          dim3(256), 0, P, 0, 0, (int)(Mp / BM));
+            } else {
+              P.tiles_n = (int)(Np / TN);
+              IAMX_LAUNCH("gemm_split_bf16_mfma", k_seg_gemm<0>, dim3((unsigned)((Mp / TM) * (Np / TN))),
+                          dim3(256), 0, P);
+            }
+            return IAMX_OK;
+          },
+          &rc);
+    return rc;
+  }
+  const int S = pb.S;
+  const size_t a_bytes = (size_t)(S * Mp * Kp);
+  uint8_t *ws = (uint8_t *)iamx_workspace(11, a_bytes + (size_t)Mp * 4 + 256);
+  if (!ws) return iamx::set_error(IAMX_ENOMEM, "ozaki workspace");
+  int32_t *ea = (int32_t *)(ws + a_bytes);
+  const double *Ad = (const double *)A;
+  IAMX_LAUNCH("ozaki_row_exp", k_row_exp, dim3((unsigned)((M + 3) / 4)), dim3(256), 0, Ad, M, K, lda, ea);
+  IAMX_LAUNCH("ozaki_slice_A", k_slice_i8, dim3((unsigned)(Kp / 32), (unsigned)(Mp / 32)), dim3(256), 0, Ad, M, K,
+              lda, 0, ea, S, (int8_t *)ws, Mp, Kp);
+  P.A = ws;
+  P.lda = Kp;
+  P.ldb = Kp;
+  P.Kb = Kp;
   P.ea = ea;
-  P.fb = fb;
+  P.fb = pb.fb;
   // levels are added from the least significant (S+1) up to the most
   // significant (2), so small contributions are not lost to FP64 rounding first
-  int rc = IAMX_OK;
   tuned(1, M, N, K * S,
         [&](int tile) -> int {
           if (tile == 256) {
@@ -736,6 +758,34 @@ int iamx_gemm_ozaki_i8(int64_t M, int64_t N, int64_t K, const double *A, int64_t
         },
         &rc);
   return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+// float32: C = A B through the bf16 hi/lo split (3 products).
+int iamx_gemm_split_bf16(int64_t M, int64_t N, int64_t K, const float *A, int64_t lda, const float *B, int64_t ldb,
+                         float *C, int64_t ldc) {
+  PrepB pb;
+  int rc = mixed_prepare_b(pb, 1, 1, N, K, B, ldb, 0);
+  return rc ? rc : mixed_rows(pb, 1, M, A, lda, C, ldc);
+}
+
+// float64: C = A B through Ozaki INT8 slicing with S slices.
+int iamx_gemm_ozaki_i8(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B, int64_t ldb,
+                       double *C, int64_t ldc, int S) {
+  PrepB pb;
+  int rc = mixed_prepare_b(pb, 1, 2, N, K, B, ldb, S);
+  return rc ? rc : mixed_rows(pb, 2, M, A, lda, C, ldc);
+}
+
+int iamx_mixed_prepare_b(int form, int64_t N, int64_t K, const void *B, int64_t ldb, int S) {
+  return mixed_prepare_b(g_pb, 10, form, N, K, B, ldb, S);
+}
+
+int iamx_mixed_rows(int form, int64_t M, const void *A, int64_t lda, void *C, int64_t ldc) {
+  return mixed_rows(g_pb, form, M, A, lda, C, ldc);
 }
 
 IAMX_API int iamx_ozaki_pick_slices(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B,

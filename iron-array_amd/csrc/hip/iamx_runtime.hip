@@ -200,14 +200,23 @@ IAMX_API int iamx_d2h(void *dst, const void *src, size_t bytes) {
 // (synchronising the stream first) and lives until iamx_shutdown.
 static void *g_host_buf = nullptr;
 static size_t g_host_bytes = 0;
-static hipEvent_t g_copy_ev[IAMX_COPY_EVENTS];
+// Downloads run on a copy stream of their own (default priority), each after
+// an event that marks the work enqueued before it on the compute stream, so
+// kernels enqueued after a download overlap its copy.
+static hipEvent_t g_copy_ev[IAMX_COPY_EVENTS], g_ready_ev[IAMX_COPY_EVENTS];
 static bool g_copy_ev_made = false;
+static hipStream_t g_copy_stream = nullptr;
+
+static void sync_streams() {
+  (void)hipStreamSynchronize(g_stream);
+  if (g_copy_stream) (void)hipStreamSynchronize(g_copy_stream);
+}
 
 IAMX_API void *iamx_host_buffer(size_t bytes) {
   if (ensure_init()) return nullptr;
   if (g_host_buf && g_host_bytes >= bytes) return g_host_buf;
   if (g_host_buf) {
-    (void)hipStreamSynchronize(g_stream);
+    sync_streams();
     (void)hipHostFree(g_host_buf);
     g_host_buf = nullptr;
     g_host_bytes = 0;
@@ -226,13 +235,19 @@ IAMX_API int iamx_d2h_async(void *dst, const void *src, size_t bytes, int event)
   int rc = ensure_init();
   if (rc) return rc;
   if (event < 0 || event >= IAMX_COPY_EVENTS) return set_error(IAMX_EINVAL, "iamx_d2h_async: event %d", event);
+  if (capturing()) return set_error(IAMX_EINVAL, "iamx_d2h_async: not available under graph capture");
   if (!g_copy_ev_made) {
-    for (int i = 0; i < IAMX_COPY_EVENTS; i++)
+    for (int i = 0; i < IAMX_COPY_EVENTS; i++) {
       IAMX_HIP_CHECK(hipEventCreateWithFlags(&g_copy_ev[i], hipEventDisableTiming));
+      IAMX_HIP_CHECK(hipEventCreateWithFlags(&g_ready_ev[i], hipEventDisableTiming));
+    }
+    IAMX_HIP_CHECK(hipStreamCreateWithFlags(&g_copy_stream, hipStreamNonBlocking));
     g_copy_ev_made = true;
   }
-  if (bytes) IAMX_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
-  IAMX_HIP_CHECK(hipEventRecord(g_copy_ev[event], g_stream));
+  IAMX_HIP_CHECK(hipEventRecord(g_ready_ev[event], g_stream));
+  IAMX_HIP_CHECK(hipStreamWaitEvent(g_copy_stream, g_ready_ev[event], 0));
+  if (bytes) IAMX_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_copy_stream));
+  IAMX_HIP_CHECK(hipEventRecord(g_copy_ev[event], g_copy_stream));
   return IAMX_OK;
 }
 
@@ -265,6 +280,7 @@ IAMX_API int iamx_sync(void) {
   int rc = ensure_init();
   if (rc) return rc;
   IAMX_HIP_CHECK(hipStreamSynchronize(g_stream));
+  if (g_copy_stream) IAMX_HIP_CHECK(hipStreamSynchronize(g_copy_stream));
   return IAMX_OK;
 }
 
@@ -273,7 +289,7 @@ IAMX_API void *iamx_workspace(int slot, size_t bytes) {
   Workspace &w = g_ws[slot];
   if (w.bytes >= bytes && w.ptr) return w.ptr;
   if (w.ptr) {
-    (void)hipStreamSynchronize(g_stream);
+    sync_streams();
     (void)hipFree(w.ptr);
     w.ptr = nullptr;
     w.bytes = 0;
@@ -290,7 +306,7 @@ IAMX_API void *iamx_workspace(int slot, size_t bytes) {
 
 IAMX_API void iamx_shutdown(void) {
   if (g_init_state != 1) return;
-  (void)hipStreamSynchronize(g_stream);
+  sync_streams();
   for (auto &w : g_ws) {
     if (w.ptr) (void)hipFree(w.ptr);
     w = Workspace{};

@@ -625,7 +625,13 @@ static void piped_task(void *v, int64_t i, int worker) {
   p->fn(p->ctx, i, worker);
 }
 
-ina_rc_t iax_download_pipelined(const void *d_src, size_t total, const int64_t *task_end, int64_t ntasks,
+/* Core of the pipelined downloads: the tasks are grouped into panels (panel k
+ * holds tasks [panel_first[k], panel_first[k+1])); before the first copy of a
+ * panel, produce(pctx, k, t0, t1) enqueues the device work that fills the
+ * panel's bytes.  Each panel is copied in pieces of about total/32 bytes (at
+ * least 4 MB), a piece never spanning two panels. */
+static ina_rc_t download_panels(const void *d_src, size_t total, const int64_t *task_end, int64_t ntasks,
+                                const int64_t *panel_first, int npanels, iax_panel_fn produce, void *pctx,
                                 int nthreads, iax_task_fn fn, void *ctx, uint8_t **host) {
   uint8_t *h = (uint8_t *)iamx_host_buffer(total + 16);
   if (!h) return iax_dev_error(IAMX_ENOMEM, "pinned staging buffer");
@@ -634,14 +640,30 @@ ina_rc_t iax_download_pipelined(const void *d_src, size_t total, const int64_t *
   size_t target = total / 32;
   if (target < ((size_t)4 << 20)) target = (size_t)4 << 20;
   int64_t first[IAMX_COPY_EVENTS + 1];
-  int np = 0;
+  int np = 0, k = 0;
   int64_t t = 0;
   size_t start = 0;
   while (t < ntasks) {
+    /* enqueue the device work of every panel that starts at or before task t */
+    while (k < npanels && panel_first[k] <= t) {
+      if (produce) {
+        const int e = produce(pctx, k, panel_first[k], panel_first[k + 1]);
+        if (e) return iax_dev_error(e, "panel");
+      }
+      k++;
+    }
+    const int64_t pend = k < npanels ? panel_first[k] : ntasks; /* end of the current panel */
     int64_t t1 = t;
-    while (t1 < ntasks && (size_t)task_end[t1] - start < target) t1++;
-    if (t1 < ntasks) t1++; /* the task that reaches the target closes the piece */
-    if (np == IAMX_COPY_EVENTS - 1) t1 = ntasks;
+    while (t1 < pend && (size_t)task_end[t1] - start < target) t1++;
+    if (t1 < pend) t1++; /* the task that reaches the target closes the piece */
+    if (np == IAMX_COPY_EVENTS - 1) { /* last event: the rest in one piece */
+      for (; k < npanels; k++)
+        if (produce) {
+          const int e = produce(pctx, k, panel_first[k], panel_first[k + 1]);
+          if (e) return iax_dev_error(e, "panel");
+        }
+      t1 = ntasks;
+    }
     const size_t end = (size_t)task_end[t1 - 1];
     const int e = iamx_d2h_async(h + start, d + start, end - start, np);
     if (e) return iax_dev_error(e, "download");
@@ -655,44 +677,104 @@ ina_rc_t iax_download_pipelined(const void *d_src, size_t total, const int64_t *
   p.ctx = ctx;
   p.first = first;
   p.np = np;
-  for (int k = 0; k < IAMX_COPY_EVENTS; k++) atomic_init(&p.landed[k], 0);
+  for (int i = 0; i < IAMX_COPY_EVENTS; i++) atomic_init(&p.landed[i], 0);
   atomic_init(&p.err, 0);
   iax_parallel_for(nthreads, ntasks, piped_task, &p);
-  if (atomic_load(&p.err)) return iax_dev_error(IAMX_EINVAL, "download");
+  if (atomic_load(&p.err)) {
+    for (int i = 0; i < np; i++) (void)iamx_event_sync(i); /* no copy may still be writing the buffer */
+    return iax_dev_error(IAMX_EINVAL, "download");
+  }
   return INA_SUCCESS;
 }
 
-ina_rc_t iax_write_from_dense(iarray_container_t *c, const void *d_dense) {
-  iamx_tiling_t t;
-  memset(&t, 0, sizeof(t));
+ina_rc_t iax_download_pipelined(const void *d_src, size_t total, const int64_t *task_end, int64_t ntasks,
+                                int nthreads, iax_task_fn fn, void *ctx, uint8_t **host) {
+  const int64_t pf[2] = {0, ntasks};
+  return download_panels(d_src, total, task_end, ntasks, pf, 1, NULL, NULL, nthreads, fn, ctx, host);
+}
+
+static void tiling_of(const iarray_container_t *c, iamx_tiling_t *t) {
+  memset(t, 0, sizeof(*t));
   for (int i = 0; i < 2; i++) {
-    t.shape[i] = c->shape[i];
-    t.chunk[i] = c->chunk[i];
-    t.block[i] = c->block[i];
-    t.nchunks[i] = c->nchunks_ax[i];
-    t.nblocks[i] = c->nblocks_ax[i];
+    t->shape[i] = c->shape[i];
+    t->chunk[i] = c->chunk[i];
+    t->block[i] = c->block[i];
+    t->nchunks[i] = c->nchunks_ax[i];
+    t->nblocks[i] = c->nblocks_ax[i];
   }
-  t.typesize = c->itemsize;
+  t->typesize = c->itemsize;
+  t->filter = c->cfg.compression_level > 0 ? iax_filter_code(&c->cfg) : IAMX_FILTER_NONE;
+}
+
+typedef struct {
+  iarray_container_t *c;
+  const iamx_tiling_t *t;
+  const void *d_dense;
+  void *d_packed;
+  iax_rows_fn rows;
+  void *rctx;
+} panel_ctx_t;
+
+/* enqueue the rows of C behind chunks [t0, t1) (whole chunk rows), then their packing */
+static int produce_panel(void *v, int k, int64_t t0, int64_t t1) {
+  (void)k;
+  panel_ctx_t *x = (panel_ctx_t *)v;
+  const iarray_container_t *c = x->c;
+  if (x->rows) {
+    const int64_t r0 = (t0 / c->nchunks_ax[1]) * c->chunk[0];
+    int64_t r1 = (t1 / c->nchunks_ax[1]) * c->chunk[0];
+    if (r1 > c->shape[0]) r1 = c->shape[0];
+    if (r1 > r0) {
+      const int e = x->rows(x->rctx, r0, r1 - r0);
+      if (e) return e;
+    }
+  }
+  return iamx_pack_blocks_range(x->t, x->d_dense, c->shape[1], x->d_packed, t0, t1);
+}
+
+ina_rc_t iax_write_from_dense_rows(iarray_container_t *c, const void *d_dense, int64_t panel_rows,
+                                   iax_rows_fn rows, void *rctx) {
+  iamx_tiling_t t;
+  tiling_of(c, &t);
   const int filtered = c->cfg.compression_level > 0;
-  t.filter = filtered ? iax_filter_code(&c->cfg) : IAMX_FILTER_NONE;
   const size_t total = (size_t)(c->nchunks * c->chunk_nbytes);
   void *d_packed = iamx_workspace(5, total);
   if (!d_packed) return iax_dev_error(IAMX_ENOMEM, "pack buffer");
-  int e = iamx_pack_blocks(&t, d_dense, c->shape[1], d_packed);
-  if (e) return iax_dev_error(e, "pack");
+  /* panels of whole chunk rows, at most 16 of them */
+  const int64_t ncr = c->nchunks_ax[0];
+  int64_t cr = panel_rows > 0 ? (panel_rows + c->chunk[0] - 1) / c->chunk[0] : ncr;
+  if (cr < 1) cr = 1;
+  if ((ncr + cr - 1) / cr > 16) cr = (ncr + 15) / 16;
+  const int npanels = (int)((ncr + cr - 1) / cr);
   int64_t *ends = (int64_t *)malloc(sizeof(int64_t) * (size_t)c->nchunks);
-  if (!ends) return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+  int64_t *pf = (int64_t *)malloc(sizeof(int64_t) * (size_t)(npanels + 1));
+  if (!ends || !pf) {
+    free(ends);
+    free(pf);
+    return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+  }
   for (int64_t i = 0; i < c->nchunks; i++) ends[i] = (i + 1) * c->chunk_nbytes;
+  for (int k = 0; k <= npanels; k++) {
+    int64_t r = (int64_t)k * cr;
+    if (r > ncr) r = ncr;
+    pf[k] = r * c->nchunks_ax[1];
+  }
+  panel_ctx_t px = {c, &t, d_dense, d_packed, rows, rctx};
   const double t0 = iax_now();
   wctx_t x = {c, NULL, filtered, 0};
-  ina_rc_t rc = iax_download_pipelined(d_packed, total, ends, c->nchunks, iax_nthreads(&c->cfg), write_chunk_task,
-                                       &x, &x.packed);
+  ina_rc_t rc = download_panels(d_packed, total, ends, c->nchunks, pf, npanels, produce_panel, &px,
+                                iax_nthreads(&c->cfg), write_chunk_task, &x, &x.packed);
   iax_last_encode_s = iax_now() - t0;
   free(ends);
+  free(pf);
   if (INA_FAILED(rc)) return rc;
   drop_device(c);
   if (x.err) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "chunk encode failed");
   return INA_SUCCESS;
+}
+
+ina_rc_t iax_write_from_dense(iarray_container_t *c, const void *d_dense) {
+  return iax_write_from_dense_rows(c, d_dense, 0, NULL, NULL);
 }
 
 /* ---- read back ----------------------------------------------------------------------------------- */

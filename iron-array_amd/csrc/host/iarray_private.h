@@ -101,6 +101,13 @@ double iax_now(void);
 ina_rc_t iax_download_pipelined(const void *d_src, size_t total, const int64_t *task_end, int64_t ntasks,
                                 int nthreads, iax_task_fn fn, void *ctx, uint8_t **host);
 
+/* Device work that fills part of a buffer before it is downloaded: enqueue
+ * (asynchronously) what produces the bytes of tasks [t0, t1) of panel k;
+ * returns an iamx error code. */
+typedef int (*iax_panel_fn)(void *ctx, int k, int64_t t0, int64_t t1);
+/* Enqueue the computation of rows [row0, row0 + nrows) of a dense result. */
+typedef int (*iax_rows_fn)(void *ctx, int64_t row0, int64_t nrows);
+
 /* ---- containers (iarray_container.c) ---------------------------------------------------- */
 ina_rc_t iax_container_new(iarray_context_t *ctx, iarray_dtshape_t *dtshape, iarray_storage_t *storage,
                            iarray_container_t **c);
@@ -115,6 +122,12 @@ ina_rc_t iax_operand(iarray_container_t *c, int slot, int tmp_slot, void **d_out
 /* Write C from a dense device buffer (logical shape, ld = shape1): GPU pack +
  * host LZ4 encode; fills c->chunks. */
 ina_rc_t iax_write_from_dense(iarray_container_t *c, const void *d_dense);
+/* The same, with C produced panel by panel: rows(rctx, row0, nrows) enqueues
+ * the computation of each panel of about `panel_rows` rows (whole chunk rows,
+ * at most 16 panels) just before that panel is packed and downloaded, so the
+ * host encode of earlier panels overlaps the computation of later ones. */
+ina_rc_t iax_write_from_dense_rows(iarray_container_t *c, const void *d_dense, int64_t panel_rows,
+                                   iax_rows_fn rows, void *rctx);
 int64_t iax_dtype_size(iarray_data_type_t t);
 
 ina_rc_t iax_set_error(ina_rc_t rc, const char *fmt, ...);

@@ -151,6 +151,32 @@ def test_integer_operands_bit_exact(ctx, dt, shape):
     assert np.array_equal(y.to_numpy(), (a.astype(np.float64) @ x.astype(np.float64)).astype(dt))
 
 
+@pytest.mark.parametrize("form", ["f32", "f64", "split_bf16", "ozaki_int8"])
+@pytest.mark.parametrize("panel_rows", ["1", "150"])
+def test_row_panels_bit_exact(ctx, monkeypatch, form, panel_rows):
+    """C computed in row panels (one chunk row each, or three), ragged M and
+    chunk rows, equals the one-panel product bit for bit; integer operands
+    make every form exact, so both also equal the exact product."""
+    dt = np.float32 if form in ("f32", "split_bf16") else np.float64
+    M, K, N = 333, 190, 170
+    print(f"case: row panels {form} panel_rows={panel_rows}", flush=True)
+    rng = np.random.default_rng(len(form) * 31 + int(panel_rows))
+    a = rng.integers(-8, 9, (M, K)).astype(dt)
+    b = rng.integers(-8, 9, (K, N)).astype(dt)
+    ca = ia.Container.from_numpy(ctx, a, (64, 96), (32, 32))
+    cb = ia.Container.from_numpy(ctx, b, (96, 80), (32, 40))
+    kw = {"split_bf16": {"precision": ia.SPLIT_BF16}, "ozaki_int8": {"precision": ia.OZAKI_INT8, "slices": 3}}
+    kw = kw.get(form, {})
+    monkeypatch.setenv("IARRAY_GEMM_PANEL_ROWS", "0")
+    one = ia.matmul(ca, cb, (64, 64), (32, 16), **kw).to_numpy()
+    monkeypatch.setenv("IARRAY_GEMM_PANEL_ROWS", panel_rows)
+    z = ia.matmul(ca, cb, (64, 64), (32, 16), **kw)
+    got = z.to_numpy()
+    want = (a.astype(np.float64) @ b.astype(np.float64)).astype(dt)
+    assert np.array_equal(one, want)
+    assert got.tobytes() == one.tobytes()
+
+
 def test_bitshuffled_gemv_bit_exact(ctx):
     """The headline GEMV layout (float32, LZ4 + bitshuffle, 250x250 blocks,
     62500 elements per block so every block has a 4-element tail)."""
