@@ -41,19 +41,25 @@ IAMX_API int iamx_d2d(void *dst, const void *src, size_t bytes);
 IAMX_API int iamx_memset(void *dst, int value, size_t bytes);
 IAMX_API int iamx_sync(void);
 /* Pinned host staging buffer (grows, kept until iamx_shutdown) and
- * stream-ordered downloads into it: iamx_d2h_async queues the copy on the
- * library stream and records event `event` (0..IAMX_COPY_EVENTS-1) after it;
+ * downloads into it: iamx_d2h_async queues the copy on a copy stream, after
+ * all work queued so far on the library stream, and records event `event`
+ * (0..IAMX_COPY_EVENTS-1) after it, so kernels queued later overlap the copy;
  * iamx_event_sync blocks the host until that event has completed. */
 #define IAMX_COPY_EVENTS 64
 IAMX_API void *iamx_host_buffer(size_t bytes);
 IAMX_API int iamx_d2h_async(void *dst, const void *src, size_t bytes, int event);
 IAMX_API int iamx_event_sync(int event);
+/* A download queued on the copy stream after the ones before it, without
+ * waiting for compute work queued since the last iamx_d2h_async (whose data
+ * it may therefore not read); records `event`. */
+IAMX_API int iamx_d2h_chain(void *dst, const void *src, size_t bytes, int event);
 /* Scratch that lives until iamx_shutdown; grown (never shrunk) on demand.
- * Slot s (0..7) is independent of the others. */
-/* slots 0..7 belong to the host library, 8..9 to the Ozaki slice tuner, 1 and
- * 10..11 to the mixed-precision products (B planes of one-shot products, B
- * planes of the row-panel interface, planes of the current A panel) */
-#define IAMX_WORKSPACES 12
+ * Each slot is independent of the others. */
+/* slots 0..7 and 12..14 belong to the host library, 8..9 to the Ozaki slice
+ * tuner, 1 and 10..11 to the mixed-precision products (B planes of one-shot
+ * products, B planes of the row-panel interface, planes of the current A
+ * panel) */
+#define IAMX_WORKSPACES 16
 IAMX_API void *iamx_workspace(int slot, size_t bytes);
 IAMX_API void iamx_shutdown(void);
 
@@ -164,6 +170,25 @@ IAMX_API int iamx_pack_blocks(const iamx_tiling_t *tiling, const void *d_src, in
  * dense C, and only the rows of those chunks are read. */
 IAMX_API int iamx_pack_blocks_range(const iamx_tiling_t *tiling, const void *d_src, int64_t src_ld, void *d_out,
                                     int64_t chunk_begin, int64_t chunk_end);
+
+/* C write stage on the GPU: LZ4 encode and Blosc2 chunk assembly.  Replaces
+ * the shuffle + LZ4 encode + chunk layout of blosc2_compress_ctx on every C
+ * chunk (src/linalg/matmul/gemm.c:216-243, src/linalg/iarray_gemm.c:399-425);
+ * the chunk layout is the one the host writer produces (format version 5,
+ * 32-byte header with the filter in slot 5, bstarts, per block `nstreams`
+ * streams of int32 csize + bytes: csize 0 = zero stream, = neblock = raw,
+ * otherwise one LZ4 block), nstreams = typesize when `split`, else 1.
+ * Input: the packed, filtered blocks of iamx_pack_blocks(_range).  Chunks
+ * [chunk_begin, chunk_end) are encoded; chunk c is written at c * capacity
+ * of d_out and its size stored in d_chunk_size[c].  Scratch, indexed by the
+ * stream number E over the whole tiling (chunk-major, block-major): d_slots
+ * holds E * (neblock rounded up to 16) bytes per stream, d_meta 2 int32 per
+ * stream.  The LZ4 parse is this library's own (encoded bytes are not those
+ * of c-blosc2, which nothing pins); every stream decodes to its input. */
+IAMX_API int64_t iamx_chunk_capacity(const iamx_tiling_t *tiling, int split);
+IAMX_API int iamx_encode_chunks(const iamx_tiling_t *tiling, int split, const void *d_packed, int64_t chunk_begin,
+                                int64_t chunk_end, void *d_slots, int32_t *d_meta, void *d_out,
+                                int64_t *d_chunk_size);
 
 /* Transpose a dense row-major (rows x cols) matrix: the omatcopy of a
  * transposed view (src/iarray_container.c:718-723). */

@@ -1,0 +1,381 @@
+// iamx_encode.hip -- the C write stage on the GPU: LZ4 encode of the filtered
+// block streams and Blosc2 chunk assembly, i.e. what blosc2_compress_ctx does
+// for every C chunk of the gemm branch (src/linalg/matmul/gemm.c:216-243) and
+// of iarray_opt_gemm (src/linalg/iarray_gemm.c:399-425).
+//
+// k_lz4_encode     one wavefront per stream: a greedy LZ77 parse emitting the
+//                  LZ4 block format.  Candidates are found 64 positions at a
+//                  time: each lane hashes the 4 bytes at its position and
+//                  reads the hash table (LDS, 2^11 entries holding a position
+//                  + 1 and the 4 bytes found there); when those bytes are
+//                  equal it also compares the next 16 bytes at both places.
+//                  Then every lane inserts its position with an atomic max,
+//                  so the latest position of a hash wins whatever the lane
+//                  order, and a candidate always comes from an earlier step.
+//                  The parse walks the step's candidates in position order
+//                  (the first at or after the parse position), extends a
+//                  match that reached 20 bytes 256 bytes a step, and emits the
+//                  sequence; a literal run that starts in the current step is
+//                  written from the lanes' registers.  The format's end rules
+//                  hold: the last match starts at least 12 bytes before the
+//                  end of the stream and ends at least 5 bytes before it.  A
+//                  stream whose output would reach its own size is stored raw,
+//                  a stream of zeros as a zero stream (csize 0).
+// k_chunk_offsets  one workgroup per chunk: stream sizes -> stream offsets (a
+//                  scan), the chunk header, bstarts and the chunk's size.
+// k_chunk_copy     one workgroup per stream: its csize word and its bytes
+//                  (compressed, or the raw filtered bytes) into the chunk.
+// Chunk c is assembled at c * cap of the output; stream E (chunk-major,
+// block-major, stream-minor) has an encode slot at E * (neblock rounded up to
+// 16) and two int32 of metadata (csize, then offset in its chunk).
+#include "iamx_internal.h"
+
+namespace {
+
+using iamx::gptr;
+
+constexpr int kHlog = 11;
+constexpr int kHsize = 1 << kHlog;
+
+struct EncGeom {
+  int64_t chunk_nbytes;  // bytes of one (extended) chunk in the packed layout
+  int64_t cap;           // output stride of a chunk (its largest possible size, multiple of 16)
+  int64_t slot;          // encode slot stride of a stream
+  int32_t blocksize;
+  int32_t nblocks;  // blocks per chunk
+  int32_t nstreams;
+  int32_t neblock;  // bytes per stream
+  int32_t typesize;
+  int32_t filter;
+  int32_t split;
+};
+
+__device__ __forceinline__ uint32_t rdl(uint32_t v, int l) { return (uint32_t)__builtin_amdgcn_readlane((int)v, l); }
+
+// STAGE > 0: streams of up to STAGE bytes are first copied into LDS (zero
+// padded), and every read of the parse is an LDS read; STAGE == 0 reads the
+// stream from global memory.
+template <int STAGE>
+__global__ __launch_bounds__(64) void k_lz4_encode(EncGeom G, const uint8_t *__restrict__ packed, int64_t e0,
+                                                   int njobs, uint8_t *__restrict__ slots,
+                                                   int32_t *__restrict__ meta) {
+  __shared__ uint64_t ht[kHsize];
+  __shared__ __align__(16) uint32_t stage[STAGE > 0 ? STAGE / 4 + 16 : 1];
+  if ((int)blockIdx.x >= njobs) return;
+  const int64_t E = e0 + blockIdx.x;
+  const int lane = threadIdx.x;
+  const int64_t per_chunk = (int64_t)G.nblocks * G.nstreams;
+  const int64_t c = E / per_chunk, r = E % per_chunk;
+  const int64_t b = r / G.nstreams, s = r % G.nstreams;
+  const int32_t n = G.neblock;
+  const uint64_t src = (uint64_t)(packed + c * G.chunk_nbytes + b * G.blocksize + s * (int64_t)n);
+  IAMX_GLOBAL uint8_t *dst = gptr<uint8_t>((uint64_t)(slots + E * G.slot));
+  IAMX_GLOBAL uint32_t *dstw = gptr<uint32_t>((uint64_t)(slots + E * G.slot));
+  const int32_t sh0 = (int32_t)(src & 3);
+  const IAMX_GLOBAL uint32_t *__restrict__ sw = gptr<const uint32_t>(src & ~3ull);
+  const IAMX_GLOBAL uint8_t *__restrict__ sb = gptr<const uint8_t>(src);
+  const int32_t lastw = (sh0 + n - 1) >> 2;  // last word holding stream bytes
+  if (STAGE > 0) {
+    // stage word g = stream bytes 4g..4g+3, zero past the end, plus 16 zero words
+    const int32_t nw = (n + 3) >> 2;
+    for (int32_t g = lane; g < nw + 16; g += 64) {
+      uint32_t v = 0;
+      if (g < nw) {
+        v = __builtin_amdgcn_alignbyte(sw[min(g + 1, lastw)], sw[min(g, lastw)], (uint32_t)sh0);
+        const int32_t valid = n - 4 * g;  // bytes of this word inside the stream
+        if (valid < 4) v &= (1u << (8 * valid)) - 1u;
+      }
+      stage[g] = v;
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  // the 4 bytes at stream position p (exact for p <= n - 4)
+  auto word_at = [&](int32_t p) -> uint32_t {
+    if (STAGE > 0) {
+      const int32_t k = p >> 2;
+      return __builtin_amdgcn_alignbyte(stage[k + 1], stage[k], (uint32_t)(p & 3));
+    }
+    const int32_t k = (sh0 + p) >> 2;
+    return __builtin_amdgcn_alignbyte(sw[min(k + 1, lastw)], sw[min(k, lastw)], (uint32_t)((sh0 + p) & 3));
+  };
+  // the byte at stream position p < n
+  auto byte_at = [&](int32_t p) -> uint8_t {
+    if (STAGE > 0) return (uint8_t)(stage[p >> 2] >> ((p & 3) << 3));
+    return sb[p];
+  };
+
+  for (int i = lane; i < kHsize; i += 64) ht[i] = 0;
+  __builtin_amdgcn_wave_barrier();
+
+  const int32_t mflimit = n - 12;   // a match starts before this
+  const int32_t matchlimit = n - 5;  // and ends at or before this
+  const int32_t cap = n - 1;         // output must stay below the stream size
+  int32_t anchor = 0, ip = 0, op = 0;
+  bool fail = false;
+  uint32_t nz = 0;
+
+  // v/255 bytes of 255 then v%255: a length's continuation bytes
+  auto put_ext = [&](int32_t v) {
+    const int32_t nf = v / 255;
+    for (int32_t k = lane; k <= nf; k += 64) dst[op + k] = (uint8_t)(k < nf ? 255 : v - 255 * nf);
+    op += nf + 1;
+  };
+  // len stream bytes at `from` to the output, dword stores realigned from the source
+  auto put_global = [&](int32_t from, int32_t len) {
+    const int32_t head = min((4 - (op & 3)) & 3, len);
+    if (lane < head) dst[op + lane] = byte_at(from + lane);
+    const int32_t nw = (len - head) >> 2;
+    const int32_t d0 = (op + head) >> 2;
+    for (int32_t g = lane; g < nw; g += 64) dstw[d0 + g] = word_at(from + head + 4 * g);
+    const int32_t t0 = head + 4 * nw;
+    if (lane < len - t0) dst[op + t0 + lane] = byte_at(from + t0 + lane);
+    op += len;
+  };
+  // bytes needed by a sequence of lit literals and (when ml > 0) a match of ml
+  auto seq_bytes = [](int32_t lit, int32_t ml) -> int32_t {
+    int32_t need = 1 + lit + (lit >= 15 ? (lit - 15) / 255 + 1 : 0);
+    if (ml) need += 2 + (ml - 4 >= 15 ? (ml - 19) / 255 + 1 : 0);
+    return need;
+  };
+
+  for (int32_t base = 0; base < mflimit; base += 64) {
+    const int32_t pos = base + lane;
+    const bool act = pos < mflimit;
+    const int32_t lp = min(pos, mflimit - 1);  // lanes past the limit read (and ignore) its bytes
+    uint32_t x[5];
+#pragma unroll
+    for (int t = 0; t < 5; t++) x[t] = word_at(lp + 4 * t);
+    const uint32_t seq = x[0];
+    nz |= act ? seq : 0u;
+    const uint32_t h = (seq * 2654435761u) >> (32 - kHlog);
+    const uint64_t ent = act ? ht[h] : 0ull;
+    __builtin_amdgcn_wave_barrier();
+    const int32_t cand = (int32_t)(ent >> 32) - 1;
+    const bool ok = act && cand >= 0 && (uint32_t)ent == seq && pos - cand <= 65535;
+    int32_t mlen = 0;
+    if (ok) {
+      int32_t ext = 16;
+#pragma unroll
+      for (int t = 1; t < 5; t++) {
+        const uint32_t d = x[t] ^ word_at(cand + 4 * t);
+        if (ext == 16 && d) ext = 4 * (t - 1) + (__builtin_ctz(d) >> 3);
+      }
+      mlen = 4 + ext;
+    }
+    if (act) atomicMax(&ht[h], ((uint64_t)(uint32_t)(pos + 1) << 32) | seq);
+    __builtin_amdgcn_wave_barrier();
+    uint64_t mask = __ballot(ok);
+    if (ip > base) mask = ip - base >= 64 ? 0ull : mask & (~0ull << (ip - base));
+    while (mask) {
+      const int l = __builtin_ctzll(mask);
+      const int32_t i = base + l;
+      const int32_t off = i - (int32_t)rdl((uint32_t)cand, l);
+      int32_t end = i + (int32_t)rdl((uint32_t)mlen, l);
+      if (end - i == 20) {
+        // extend 4 bytes a lane until a difference or the match limit
+        int32_t p = end;
+        for (;;) {
+          if (p >= matchlimit) break;
+          const int32_t q = p + 4 * lane;
+          const int32_t qc = min(q, matchlimit);
+          const uint32_t d = q < matchlimit ? (word_at(qc) ^ word_at(qc - off)) : 0u;
+          const uint64_t stop = __ballot(q >= matchlimit || d != 0);
+          if (stop) {
+            const int L = __builtin_ctzll(stop);
+            const uint32_t dl = rdl(d, L);
+            p += 4 * L + (dl ? (int32_t)(__builtin_ctz(dl) >> 3) : 0);
+            break;
+          }
+          p += 256;
+        }
+        end = p;
+      }
+      if (end > matchlimit) end = matchlimit;
+      const int32_t lit = i - anchor, ml = end - i;
+      if (op + seq_bytes(lit, ml) > cap) {
+        fail = true;
+        break;
+      }
+      if (lane == 0) dst[op] = (uint8_t)((min(lit, 15) << 4) | min(ml - 4, 15));
+      op += 1;
+      if (lit >= 15) put_ext(lit - 15);
+      if (anchor >= base) {
+        // the run lies in this step: lane l holds the byte at base + l
+        const int32_t l0 = anchor - base;
+        if (lane >= l0 && lane < l0 + lit) dst[op + lane - l0] = (uint8_t)seq;
+        op += lit;
+      } else {
+        put_global(anchor, lit);
+      }
+      if (lane < 2) dst[op + lane] = (uint8_t)(off >> (8 * lane));
+      op += 2;
+      if (ml - 4 >= 15) put_ext(ml - 19);
+      anchor = ip = end;
+      mask = ip - base >= 64 ? 0ull : mask & (~0ull << (ip - base));
+    }
+    if (fail) break;
+    // the literals pending so far no longer fit: the stream is stored raw
+    if (op + (min(base + 64, n) - anchor) >= cap) {
+      fail = true;
+      break;
+    }
+  }
+  if (!fail) {
+    const int32_t lit = n - anchor;
+    if (op + seq_bytes(lit, 0) > cap) {
+      fail = true;
+    } else {
+      if (lane == 0) dst[op] = (uint8_t)(min(lit, 15) << 4);
+      op += 1;
+      if (lit >= 15) put_ext(lit - 15);
+      put_global(anchor, lit);
+    }
+  }
+  // zero stream: the scan saw bytes [0, n - 9); the last (up to) 12 bytes here
+  const int32_t t0 = max(n - 12, 0);
+  if (lane < n - t0) nz |= byte_at(t0 + lane);
+  const bool zero = __ballot(nz != 0) == 0;
+  if (lane == 0) meta[2 * E] = zero ? 0 : fail ? n : op;
+}
+
+// Workgroup copy of len bytes: byte head up to the destination's dword
+// alignment, dword stores realigned from the source, byte tail.
+__device__ void wg_copy(IAMX_GLOBAL uint8_t *d, const IAMX_GLOBAL uint8_t *s, int32_t len, int tid, int nt) {
+  const int32_t head = min((int32_t)((4 - ((uint64_t)d & 3)) & 3), len);
+  if (tid < head) d[tid] = s[tid];
+  const int32_t body = len - head;
+  const int32_t nw = body >> 2;
+  if (nw > 0) {
+    const uint64_t sa = (uint64_t)(s + head);
+    const uint32_t sh = (uint32_t)(sa & 3);
+    const IAMX_GLOBAL uint32_t *sw = gptr<const uint32_t>(sa & ~3ull);
+    const int32_t lastw = (int32_t)((sh + body - 1) >> 2);
+    IAMX_GLOBAL uint32_t *dw = gptr<uint32_t>((uint64_t)(d + head));
+    for (int32_t g = tid; g < nw; g += nt)
+      dw[g] = __builtin_amdgcn_alignbyte(sw[min(g + 1, lastw)], sw[g], sh);
+  }
+  for (int32_t k = head + 4 * nw + tid; k < len; k += nt) d[k] = s[k];
+}
+
+__global__ __launch_bounds__(256) void k_chunk_offsets(EncGeom G, int64_t c0, int32_t *__restrict__ meta,
+                                                       uint8_t *__restrict__ out, int64_t *__restrict__ csizes) {
+  __shared__ int32_t part[256];
+  const int64_t c = c0 + blockIdx.x;
+  const int tid = threadIdx.x;
+  const int32_t ne = G.nblocks * G.nstreams;
+  const int64_t E0 = c * ne;
+  const int32_t hdr = 32 + 4 * G.nblocks;
+  IAMX_GLOBAL uint32_t *co = gptr<uint32_t>((uint64_t)(out + c * G.cap));
+  int32_t carry = 0;
+  for (int32_t t0 = 0; t0 < ne; t0 += 256) {
+    const int32_t t = t0 + tid;
+    const int32_t sz = t < ne ? 4 + meta[2 * (E0 + t)] : 0;
+    part[tid] = sz;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+      const int32_t v = tid >= d ? part[tid - d] : 0;
+      __syncthreads();
+      part[tid] += v;
+      __syncthreads();
+    }
+    const int32_t at = hdr + carry + part[tid] - sz;
+    if (t < ne) {
+      meta[2 * (E0 + t) + 1] = at;
+      if (t % G.nstreams == 0) co[8 + t / G.nstreams] = (uint32_t)at;  // bstarts
+    }
+    carry += part[255];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const int32_t total = hdr + carry;
+    const uint32_t flags = 0x05u | (G.split ? 0u : 0x10u) | 0x20u;
+    co[0] = 5u | (1u << 8) | (flags << 16) | ((uint32_t)G.typesize << 24);
+    co[1] = (uint32_t)(G.chunk_nbytes);
+    co[2] = (uint32_t)G.blocksize;
+    co[3] = (uint32_t)total;
+    co[4] = 0u;
+    const uint32_t fcode = G.filter == IAMX_FILTER_SHUFFLE ? 1u : G.filter == IAMX_FILTER_BITSHUFFLE ? 2u : 0u;
+    co[5] = fcode << 8;  // byte 21: the filter in the last of the 6 filter slots
+    co[6] = 0u;
+    co[7] = 0u;
+    csizes[c] = total;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_chunk_copy(EncGeom G, int64_t e0, const uint8_t *__restrict__ packed,
+                                                    const uint8_t *__restrict__ slots,
+                                                    const int32_t *__restrict__ meta, uint8_t *__restrict__ out) {
+  const int64_t E = e0 + blockIdx.x;
+  const int64_t per_chunk = (int64_t)G.nblocks * G.nstreams;
+  const int64_t c = E / per_chunk, r = E % per_chunk;
+  const int64_t b = r / G.nstreams, s = r % G.nstreams;
+  const int32_t cs = meta[2 * E], at = meta[2 * E + 1];
+  IAMX_GLOBAL uint8_t *d = gptr<uint8_t>((uint64_t)(out + c * G.cap + at));
+  if (threadIdx.x < 4) d[threadIdx.x] = (uint8_t)((uint32_t)cs >> (8 * threadIdx.x));
+  if (cs == 0) return;
+  const uint8_t *sp = cs == G.neblock ? packed + c * G.chunk_nbytes + b * G.blocksize + s * (int64_t)G.neblock
+                                      : slots + E * G.slot;
+  wg_copy(d + 4, gptr<const uint8_t>((uint64_t)sp), cs, threadIdx.x, blockDim.x);
+}
+
+bool geom_of(const iamx_tiling_t *t, int split, EncGeom *G) {
+  if (!t || t->typesize < 1 || t->typesize > 8 || t->block[0] <= 0 || t->block[1] <= 0) return false;
+  const int64_t bs = t->block[0] * t->block[1] * t->typesize;
+  const int64_t nb = t->nblocks[0] * t->nblocks[1];
+  if (bs <= 0 || bs > INT32_MAX / 2 || nb <= 0 || nb > (1 << 20)) return false;
+  G->blocksize = (int32_t)bs;
+  G->nblocks = (int32_t)nb;
+  G->nstreams = split ? t->typesize : 1;
+  G->neblock = (int32_t)(bs / G->nstreams);
+  G->typesize = t->typesize;
+  G->filter = t->filter;
+  G->split = split ? 1 : 0;
+  G->chunk_nbytes = bs * nb;
+  G->cap = (32 + 4 * nb + 4 * nb * G->nstreams + G->chunk_nbytes + 15) & ~15ll;
+  G->slot = (G->neblock + 15) & ~15ll;
+  return G->chunk_nbytes <= INT32_MAX;
+}
+
+}  // namespace
+
+extern "C" {
+
+IAMX_API int64_t iamx_chunk_capacity(const iamx_tiling_t *t, int split) {
+  EncGeom G;
+  return geom_of(t, split, &G) ? G.cap : -1;
+}
+
+IAMX_API int iamx_encode_chunks(const iamx_tiling_t *t, int split, const void *d_packed, int64_t chunk_begin,
+                                int64_t chunk_end, void *d_slots, int32_t *d_meta, void *d_out,
+                                int64_t *d_chunk_size) {
+  int rc = iamx::ensure_init();
+  if (rc) return rc;
+  EncGeom G;
+  if (!geom_of(t, split, &G)) return iamx::set_error(IAMX_EINVAL, "iamx_encode_chunks: bad tiling");
+  const int64_t nchunks = t->nchunks[0] * t->nchunks[1];
+  if (chunk_begin < 0 || chunk_end > nchunks || chunk_begin > chunk_end)
+    return iamx::set_error(IAMX_EINVAL, "iamx_encode_chunks: chunk range [%lld, %lld) outside [0, %lld)",
+                           (long long)chunk_begin, (long long)chunk_end, (long long)nchunks);
+  const int64_t per_chunk = (int64_t)G.nblocks * G.nstreams;
+  const int64_t nst = (chunk_end - chunk_begin) * per_chunk;
+  if (nst == 0) return IAMX_OK;
+  if (nst > INT32_MAX || chunk_end - chunk_begin > INT32_MAX)
+    return iamx::set_error(IAMX_EINVAL, "iamx_encode_chunks: too many streams");
+  const int64_t e0 = chunk_begin * per_chunk;
+  // streams up to 16 KiB are staged in LDS (32 KiB with the table: 5
+  // wavefronts a CU); longer ones are read from global memory, which measured
+  // 2.1x faster on 64 KiB streams than an 80 KiB staging (one wavefront a CU)
+  if (G.neblock <= 16384)
+    IAMX_LAUNCH("lz4_encode", k_lz4_encode<16384>, dim3((unsigned)nst), dim3(64), 0, G, (const uint8_t *)d_packed,
+                e0, (int)nst, (uint8_t *)d_slots, d_meta);
+  else
+    IAMX_LAUNCH("lz4_encode", k_lz4_encode<0>, dim3((unsigned)nst), dim3(64), 0, G, (const uint8_t *)d_packed, e0,
+                (int)nst, (uint8_t *)d_slots, d_meta);
+  IAMX_LAUNCH("chunk_offsets", k_chunk_offsets, dim3((unsigned)(chunk_end - chunk_begin)), dim3(256), 0, G,
+              chunk_begin, d_meta, (uint8_t *)d_out, d_chunk_size);
+  IAMX_LAUNCH("chunk_copy", k_chunk_copy, dim3((unsigned)nst), dim3(256), 0, G, e0, (const uint8_t *)d_packed,
+              (const uint8_t *)d_slots, (const int32_t *)d_meta, (uint8_t *)d_out);
+  return IAMX_OK;
+}
+
+}  // extern "C"

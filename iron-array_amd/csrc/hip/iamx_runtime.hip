@@ -251,6 +251,16 @@ IAMX_API int iamx_d2h_async(void *dst, const void *src, size_t bytes, int event)
   return IAMX_OK;
 }
 
+IAMX_API int iamx_d2h_chain(void *dst, const void *src, size_t bytes, int event) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (event < 0 || event >= IAMX_COPY_EVENTS) return set_error(IAMX_EINVAL, "iamx_d2h_chain: event %d", event);
+  if (!g_copy_stream) return set_error(IAMX_EINVAL, "iamx_d2h_chain: no download queued before");
+  if (bytes) IAMX_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_copy_stream));
+  IAMX_HIP_CHECK(hipEventRecord(g_copy_ev[event], g_copy_stream));
+  return IAMX_OK;
+}
+
 IAMX_API int iamx_event_sync(int event) {
   int rc = ensure_init();
   if (rc) return rc;

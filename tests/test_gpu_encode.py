@@ -1,0 +1,161 @@
+"""GPU C write stage: LZ4 encode + Blosc2 chunk assembly on the device.
+
+Bars:
+  * every chunk the GPU writes decodes, through the oracle's decoder
+    (oracle/blosc_chunk.c, oracle/lz4_block.c), to exactly the values of the
+    product, and to the same values as the host-encoded chunks of the same
+    product (IARRAY_GPU_ENCODE=0);
+  * the chunk layout is the host writer's: header fields, bstarts, stream
+    csize semantics (0 = zeros, neblock = raw, otherwise LZ4 and smaller);
+  * every LZ4 stream keeps the format's end rules (last 5 bytes literals, last
+    match starting at least 12 bytes before the end, offsets within the
+    output so far and at most 65535);
+  * the encoder is deterministic.
+The encoded bytes themselves are this library's own parse; c-blosc2's bytes
+are not pinned by any reference test (parity unpinned for the encoder's byte
+output, pinned for what it decodes to).
+"""
+import struct
+
+import numpy as np
+import pytest
+
+import pyiarray as ia
+
+pytestmark = pytest.mark.gpu
+
+
+def _lz4_rules(s, n):
+    """Parse one LZ4 block; check the end rules; return the decoded length."""
+    ip, op = 0, 0
+    while True:
+        t = s[ip]
+        ip += 1
+        lit = t >> 4
+        if lit == 15:
+            while True:
+                b = s[ip]
+                ip += 1
+                lit += b
+                if b != 255:
+                    break
+        ip += lit
+        op += lit
+        if ip == len(s):
+            assert n < 13 or lit >= 5, "last sequence must hold the last 5 bytes"
+            return op
+        off = s[ip] | (s[ip + 1] << 8)
+        ip += 2
+        ml = t & 15
+        if ml == 15:
+            while True:
+                b = s[ip]
+                ip += 1
+                ml += b
+                if b != 255:
+                    break
+        ml += 4
+        assert 1 <= off <= op and off <= 65535
+        assert op < n - 12, f"match at {op} starts within the last 12 bytes of {n}"
+        op += ml
+        assert op <= n - 5, "match ends within the last 5 bytes"
+
+
+def _check_chunk(ch, nbytes_expected):
+    """Header, bstarts and per-stream csize semantics of one chunk."""
+    assert ch[0] == 5 and ch[1] == 1
+    flags, ts = ch[2], ch[3]
+    nbytes, blocksize, cbytes = struct.unpack("<iii", ch[4:16])
+    assert nbytes == nbytes_expected and cbytes == len(ch)
+    nblocks = nbytes // blocksize
+    nstreams = 1 if flags & 0x10 else ts
+    neblock = blocksize // nstreams
+    kinds = {"zero": 0, "raw": 0, "lz4": 0}
+    for b in range(nblocks):
+        p = struct.unpack("<i", ch[32 + 4 * b:36 + 4 * b])[0]
+        for _ in range(nstreams):
+            cs = struct.unpack("<i", ch[p:p + 4])[0]
+            p += 4
+            if cs == 0:
+                kinds["zero"] += 1
+            elif cs == neblock:
+                kinds["raw"] += 1
+                p += cs
+            else:
+                assert 0 < cs < neblock
+                assert _lz4_rules(ch[p:p + cs], neblock) == neblock
+                kinds["lz4"] += 1
+                p += cs
+        if b + 1 < nblocks:
+            assert p == struct.unpack("<i", ch[36 + 4 * b:40 + 4 * b])[0]
+    assert p == len(ch)
+    return kinds
+
+
+def _product(monkeypatch, ctx, a, b, acs, abs_, bcs, bbs, zc, zb, gpu_encode=True):
+    monkeypatch.setenv("IARRAY_GPU_ENCODE", "1" if gpu_encode else "0")
+    ca = ia.Container.from_numpy(ctx, a, acs, abs_)
+    cb = ia.Container.from_numpy(ctx, b, bcs, bbs)
+    return ia.matmul(ca, cb, zc, zb)
+
+
+CASES = [
+    # name, dtype, (M, K, N), filters, splitmode, C chunk, C block, data
+    ("f32-shuffle-random", np.float32, (300, 257, 190), ia.COMP_SHUFFLE, ia.AUTO_SPLIT, (128, 96), (64, 32), "randn"),
+    ("f64-shuffle-odd-streams", np.float64, (250, 200, 250), ia.COMP_SHUFFLE, ia.AUTO_SPLIT, (125, 125), (125, 125),
+     "randn"),
+    ("f32-bitshuffle-smooth", np.float32, (256, 300, 512), ia.COMP_BITSHUFFLE, ia.AUTO_SPLIT, (128, 256), (64, 64),
+     "smooth"),
+    ("f64-nosplit-int", np.float64, (200, 150, 170), ia.COMP_SHUFFLE, ia.NEVER_SPLIT, (100, 85), (50, 17), "int"),
+    ("f32-zeros", np.float32, (128, 64, 200), ia.COMP_SHUFFLE, ia.AUTO_SPLIT, (64, 100), (32, 50), "zeros"),
+    ("f64-constant", np.float64, (96, 80, 96), ia.COMP_SHUFFLE, ia.AUTO_SPLIT, (96, 96), (48, 48), "ones"),
+    ("f32-tiny-blocks", np.float32, (40, 30, 20), ia.COMP_SHUFFLE, ia.AUTO_SPLIT, (8, 8), (2, 2), "randn"),
+]
+
+
+def _data(kind, rng, shape, dt):
+    if kind == "randn":
+        return rng.standard_normal(shape).astype(dt)
+    if kind == "int":
+        return rng.integers(-8, 9, shape).astype(dt)
+    if kind == "zeros":
+        return np.zeros(shape, dt)
+    if kind == "ones":
+        return np.ones(shape, dt)
+    i = np.arange(shape[0])[:, None]
+    j = np.arange(shape[1])[None, :]
+    return (np.sin(0.01 * i) + np.cos(0.007 * j)).astype(dt)
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_gpu_encoded_chunks(monkeypatch, orc, case):
+    name, dt, (M, K, N), filters, split, zc, zb, kind = case
+    print(f"case: gpu encode {name}", flush=True)
+    ctx = ia.Context(clevel=5, filters=filters, splitmode=split)
+    rng = np.random.default_rng(len(name))
+    a = _data(kind, rng, (M, K), dt)
+    b = _data(kind, rng, (K, N), dt)
+    acs, abs_ = (min(M, 128), min(K, 128)), (min(M, 32), min(K, 64))
+    bcs, bbs = (min(K, 128), min(N, 128)), (min(K, 64), min(N, 32))
+    z = _product(monkeypatch, ctx, a, b, acs, abs_, bcs, bbs, zc, zb)
+    zh = _product(monkeypatch, ctx, a, b, acs, abs_, bcs, bbs, zc, zb, gpu_encode=False)
+    got = z.to_numpy()
+    assert got.tobytes() == zh.to_numpy().tobytes()
+    chunks = z.chunks()
+    dense = orc.Array(chunks, z.shape, zc, zb, z.dtype.itemsize).to_dense()
+    assert dense.tobytes() == got.tobytes()
+    ext = [-(-s // c) * c for s, c in zip(z.shape, zc)]
+    nbytes = zc[0] * zc[1] * np.dtype(dt).itemsize
+    assert len(chunks) == (ext[0] // zc[0]) * (ext[1] // zc[1])
+    kinds = {"zero": 0, "raw": 0, "lz4": 0}
+    for ch in chunks:
+        for k, v in _check_chunk(ch, nbytes).items():
+            kinds[k] += v
+    print(name, kinds)
+    if kind == "zeros":
+        assert kinds["raw"] == kinds["lz4"] == 0
+    if kind in ("ones", "smooth"):
+        assert kinds["lz4"] > 0
+    # deterministic
+    z2 = _product(monkeypatch, ctx, a, b, acs, abs_, bcs, bbs, zc, zb)
+    assert z2.chunks() == chunks
