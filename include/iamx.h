@@ -180,14 +180,16 @@ IAMX_API int iamx_pack_blocks_range(const iamx_tiling_t *tiling, const void *d_s
  * otherwise one LZ4 block), nstreams = typesize when `split`, else 1.
  * Input: the packed, filtered blocks of iamx_pack_blocks(_range).  Chunks
  * [chunk_begin, chunk_end) are encoded; chunk c is written at c * capacity
- * of d_out and its size stored in d_chunk_size[c].  Scratch, indexed by the
- * stream number E over the whole tiling (chunk-major, block-major): d_slots
- * holds E * (neblock rounded up to 16) bytes per stream, d_meta 2 int32 per
- * stream.  The LZ4 parse is this library's own (encoded bytes are not those
- * of c-blosc2, which nothing pins); every stream decodes to its input. */
+ * of d_out and its size stored in d_chunk_size[c].  d_scratch holds
+ * iamx_encode_scratch_bytes bytes; d_meta 2 int32 per stream of the tiling
+ * (chunk-major, block-major, stream-minor).  The LZ4 parse is this library's
+ * own (encoded bytes are not those of c-blosc2, which nothing pins): every
+ * 4 KiB of a stream is parsed on its own, matches staying inside it, and the
+ * pieces are joined into one LZ4 block; every stream decodes to its input. */
 IAMX_API int64_t iamx_chunk_capacity(const iamx_tiling_t *tiling, int split);
+IAMX_API int64_t iamx_encode_scratch_bytes(const iamx_tiling_t *tiling, int split);
 IAMX_API int iamx_encode_chunks(const iamx_tiling_t *tiling, int split, const void *d_packed, int64_t chunk_begin,
-                                int64_t chunk_end, void *d_slots, int32_t *d_meta, void *d_out,
+                                int64_t chunk_end, void *d_scratch, int32_t *d_meta, void *d_out,
                                 int64_t *d_chunk_size);
 
 /* Transpose a dense row-major (rows x cols) matrix: the omatcopy of a

@@ -3,10 +3,11 @@
 // for every C chunk of the gemm branch (src/linalg/matmul/gemm.c:216-243) and
 // of iarray_opt_gemm (src/linalg/iarray_gemm.c:399-425).
 //
-// k_lz4_encode     one wavefront per stream: a greedy LZ77 parse emitting the
-//                  LZ4 block format.  Candidates are found 64 positions at a
+// k_lz4_encode_sub one wavefront per 4 KiB sub-block of a stream, staged in
+//                  LDS: a greedy LZ77 parse whose matches start and end inside
+//                  the sub-block.  Candidates are found 64 positions at a
 //                  time: each lane hashes the 4 bytes at its position and
-//                  reads the hash table (LDS, 2^11 entries holding a position
+//                  reads the hash table (LDS, 2^10 entries holding a position
 //                  + 1 and the 4 bytes found there); when those bytes are
 //                  equal it also compares the next 16 bytes at both places.
 //                  Then every lane inserts its position with an atomic max,
@@ -14,19 +15,26 @@
 //                  order, and a candidate always comes from an earlier step.
 //                  The parse walks the step's candidates in position order
 //                  (the first at or after the parse position), extends a
-//                  match that reached 20 bytes 256 bytes a step, and emits the
-//                  sequence; a literal run that starts in the current step is
-//                  written from the lanes' registers.  The format's end rules
-//                  hold: the last match starts at least 12 bytes before the
-//                  end of the stream and ends at least 5 bytes before it.  A
-//                  stream whose output would reach its own size is stored raw,
-//                  a stream of zeros as a zero stream (csize 0).
+//                  match that reached 20 bytes 256 bytes a step, and writes
+//                  the sequences; a literal run that starts in the current
+//                  step is written from the lanes' registers.  The leading
+//                  literals (before the first match) and the trailing ones
+//                  (after the last) are left to the merge.
+// k_lz4_merge      one wavefront per stream: joins each sub-block's leading
+//                  literals with the previous one's trailing literals into one
+//                  run (token and length written here), copies the
+//                  sub-blocks' sequences after it, and ends with the stream's
+//                  last literals.  The format's end rules hold: the last match
+//                  starts at least 12 bytes before the end of the stream and
+//                  ends at least 5 before it.  A stream whose output would
+//                  reach its own size is stored raw, a stream of zeros as a
+//                  zero stream (csize 0).
 // k_chunk_offsets  one workgroup per chunk: stream sizes -> stream offsets (a
 //                  scan), the chunk header, bstarts and the chunk's size.
 // k_chunk_copy     one workgroup per stream: its csize word and its bytes
 //                  (compressed, or the raw filtered bytes) into the chunk.
 // Chunk c is assembled at c * cap of the output; stream E (chunk-major,
-// block-major, stream-minor) has an encode slot at E * (neblock rounded up to
+// block-major, stream-minor) has an output slot at E * (neblock rounded up to
 // 16) and two int32 of metadata (csize, then offset in its chunk).
 #include "iamx_internal.h"
 
@@ -52,106 +60,123 @@ struct EncGeom {
 
 __device__ __forceinline__ uint32_t rdl(uint32_t v, int l) { return (uint32_t)__builtin_amdgcn_readlane((int)v, l); }
 
-// STAGE > 0: streams of up to STAGE bytes are first copied into LDS (zero
-// padded), and every read of the parse is an LDS read; STAGE == 0 reads the
-// stream from global memory.
-template <int STAGE>
-__global__ __launch_bounds__(64) void k_lz4_encode(EncGeom G, const uint8_t *__restrict__ packed, int64_t e0,
-                                                   int njobs, uint8_t *__restrict__ slots,
-                                                   int32_t *__restrict__ meta) {
-  __shared__ uint64_t ht[kHsize];
-  __shared__ __align__(16) uint32_t stage[STAGE > 0 ? STAGE / 4 + 16 : 1];
-  if ((int)blockIdx.x >= njobs) return;
-  const int64_t E = e0 + blockIdx.x;
+// Group copy of len bytes: byte head up to the destination's dword alignment,
+// dword stores realigned from the source, byte tail.
+__device__ void wg_copy(IAMX_GLOBAL uint8_t *d, const IAMX_GLOBAL uint8_t *s, int32_t len, int tid, int nt) {
+  const int32_t head = min((int32_t)((4 - ((uint64_t)d & 3)) & 3), len);
+  if (tid < head) d[tid] = s[tid];
+  const int32_t body = len - head;
+  const int32_t nw = body >> 2;
+  if (nw > 0) {
+    const uint64_t sa = (uint64_t)(s + head);
+    const uint32_t sh = (uint32_t)(sa & 3);
+    const IAMX_GLOBAL uint32_t *sw = gptr<const uint32_t>(sa & ~3ull);
+    const int32_t lastw = (int32_t)((sh + body - 1) >> 2);
+    IAMX_GLOBAL uint32_t *dw = gptr<uint32_t>((uint64_t)(d + head));
+    for (int32_t g = tid; g < nw; g += nt)
+      dw[g] = __builtin_amdgcn_alignbyte(sw[min(g + 1, lastw)], sw[g], sh);
+  }
+  for (int32_t k = head + 4 * nw + tid; k < len; k += nt) d[k] = s[k];
+}
+
+// continuation bytes of an LZ4 length field for a count v (>= 0)
+__device__ __forceinline__ int32_t ext_bytes(int32_t v) { return v / 255 + 1; }
+
+// Sub-block parse.  Sub-block k of a stream covers bytes [k*kSub, (k+1)*kSub);
+// its matches start and end inside it (and keep the stream's end rules) and
+// refer to it only.  What it writes to its slot:
+//   the rest of its first sequence after the literals (offset and length
+//   continuation bytes), then its other sequences whole;
+// and to its info record: first match position, offset and length, bytes
+// written, where its trailing literals start, whether any byte is nonzero,
+// and overflow.  The merge joins each sub-block's leading literals with the
+// previous one's trailing literals into one run.
+constexpr int kSub = 4096;
+constexpr int kSubHlog = 10;
+constexpr int kSubSlot = kSub + 256;  // a sub-block's worst output, rounded
+constexpr int kInfo = 8;              // int32 per info record
+enum { I_HAS = 0, I_POS, I_OFF, I_ML, I_LEN, I_TRAIL, I_NZ, I_FAIL };
+
+__global__ __launch_bounds__(64) void k_lz4_encode_sub(EncGeom G, const uint8_t *__restrict__ packed, int64_t e0,
+                                                       int64_t nsub, int32_t K, uint8_t *__restrict__ subslots,
+                                                       int32_t *__restrict__ info) {
+  __shared__ uint64_t ht[1 << kSubHlog];
+  __shared__ __align__(16) uint32_t stage[kSub / 4 + 16];
+  const int64_t sbid = blockIdx.x;
+  if (sbid >= nsub) return;
   const int lane = threadIdx.x;
+  const int64_t E = e0 + sbid / K;
+  const int32_t k = (int32_t)(sbid % K);
   const int64_t per_chunk = (int64_t)G.nblocks * G.nstreams;
   const int64_t c = E / per_chunk, r = E % per_chunk;
   const int64_t b = r / G.nstreams, s = r % G.nstreams;
   const int32_t n = G.neblock;
-  const uint64_t src = (uint64_t)(packed + c * G.chunk_nbytes + b * G.blocksize + s * (int64_t)n);
-  IAMX_GLOBAL uint8_t *dst = gptr<uint8_t>((uint64_t)(slots + E * G.slot));
-  IAMX_GLOBAL uint32_t *dstw = gptr<uint32_t>((uint64_t)(slots + E * G.slot));
+  const int32_t S0 = k * kSub, m = min(n - S0, kSub);  // bytes of this sub-block
+  const uint64_t src = (uint64_t)(packed + c * G.chunk_nbytes + b * G.blocksize + s * (int64_t)n + S0);
+  IAMX_GLOBAL uint8_t *dst = gptr<uint8_t>((uint64_t)(subslots + sbid * kSubSlot));
+  IAMX_GLOBAL int32_t *inf = gptr<int32_t>((uint64_t)(info + sbid * kInfo));
   const int32_t sh0 = (int32_t)(src & 3);
   const IAMX_GLOBAL uint32_t *__restrict__ sw = gptr<const uint32_t>(src & ~3ull);
-  const IAMX_GLOBAL uint8_t *__restrict__ sb = gptr<const uint8_t>(src);
-  const int32_t lastw = (sh0 + n - 1) >> 2;  // last word holding stream bytes
-  if (STAGE > 0) {
-    // stage word g = stream bytes 4g..4g+3, zero past the end, plus 16 zero words
-    const int32_t nw = (n + 3) >> 2;
-    for (int32_t g = lane; g < nw + 16; g += 64) {
-      uint32_t v = 0;
-      if (g < nw) {
-        v = __builtin_amdgcn_alignbyte(sw[min(g + 1, lastw)], sw[min(g, lastw)], (uint32_t)sh0);
-        const int32_t valid = n - 4 * g;  // bytes of this word inside the stream
-        if (valid < 4) v &= (1u << (8 * valid)) - 1u;
-      }
-      stage[g] = v;
-    }
-    __builtin_amdgcn_wave_barrier();
-  }
-  // the 4 bytes at stream position p (exact for p <= n - 4)
-  auto word_at = [&](int32_t p) -> uint32_t {
-    if (STAGE > 0) {
-      const int32_t k = p >> 2;
-      return __builtin_amdgcn_alignbyte(stage[k + 1], stage[k], (uint32_t)(p & 3));
-    }
-    const int32_t k = (sh0 + p) >> 2;
-    return __builtin_amdgcn_alignbyte(sw[min(k + 1, lastw)], sw[min(k, lastw)], (uint32_t)((sh0 + p) & 3));
-  };
-  // the byte at stream position p < n
-  auto byte_at = [&](int32_t p) -> uint8_t {
-    if (STAGE > 0) return (uint8_t)(stage[p >> 2] >> ((p & 3) << 3));
-    return sb[p];
-  };
-
-  for (int i = lane; i < kHsize; i += 64) ht[i] = 0;
-  __builtin_amdgcn_wave_barrier();
-
-  const int32_t mflimit = n - 12;   // a match starts before this
-  const int32_t matchlimit = n - 5;  // and ends at or before this
-  const int32_t cap = n - 1;         // output must stay below the stream size
-  int32_t anchor = 0, ip = 0, op = 0;
-  bool fail = false;
+  const int32_t lastw = (sh0 + m - 1) >> 2;
+  // stage word g = sub-block bytes 4g..4g+3, zero past its end, plus 16 zero words
+  const int32_t nw = (m + 3) >> 2;
   uint32_t nz = 0;
+  for (int32_t g = lane; g < kSub / 4 + 16; g += 64) {
+    uint32_t v = 0;
+    if (g < nw) {
+      v = __builtin_amdgcn_alignbyte(sw[min(g + 1, lastw)], sw[min(g, lastw)], (uint32_t)sh0);
+      const int32_t valid = m - 4 * g;
+      if (valid < 4) v &= (1u << (8 * valid)) - 1u;
+    }
+    nz |= v;
+    stage[g] = v;
+  }
+  for (int i = lane; i < (1 << kSubHlog); i += 64) ht[i] = 0;
+  __builtin_amdgcn_wave_barrier();
+  auto word_at = [&](int32_t p) -> uint32_t {  // 4 bytes at local position p (p <= m + 40)
+    const int32_t q = p >> 2;
+    return __builtin_amdgcn_alignbyte(stage[q + 1], stage[q], (uint32_t)(p & 3));
+  };
+  auto byte_at = [&](int32_t p) -> uint32_t { return (stage[p >> 2] >> ((p & 3) << 3)) & 255u; };
 
-  // v/255 bytes of 255 then v%255: a length's continuation bytes
+  // local limits: a match starts before mflimit and ends at or before
+  // matchlimit, so its 4 minimum bytes fit inside the sub-block
+  const int32_t mflimit = min(m - 3, n - 12 - S0);
+  const int32_t matchlimit = min(m, n - 5 - S0);
+  const int32_t cap = kSubSlot - 32;
+  int32_t anchor = 0, ip = 0, op = 0;
+  bool fail = false, first = true;
+  int32_t f_pos = 0, f_off = 0, f_ml = 0;
+
   auto put_ext = [&](int32_t v) {
     const int32_t nf = v / 255;
-    for (int32_t k = lane; k <= nf; k += 64) dst[op + k] = (uint8_t)(k < nf ? 255 : v - 255 * nf);
+    for (int32_t q = lane; q <= nf; q += 64) dst[op + q] = (uint8_t)(q < nf ? 255 : v - 255 * nf);
     op += nf + 1;
   };
-  // len stream bytes at `from` to the output, dword stores realigned from the source
-  auto put_global = [&](int32_t from, int32_t len) {
+  auto put_stage = [&](int32_t from, int32_t len) {  // local bytes [from, from+len) to the output
     const int32_t head = min((4 - (op & 3)) & 3, len);
-    if (lane < head) dst[op + lane] = byte_at(from + lane);
-    const int32_t nw = (len - head) >> 2;
-    const int32_t d0 = (op + head) >> 2;
-    for (int32_t g = lane; g < nw; g += 64) dstw[d0 + g] = word_at(from + head + 4 * g);
-    const int32_t t0 = head + 4 * nw;
-    if (lane < len - t0) dst[op + t0 + lane] = byte_at(from + t0 + lane);
+    if (lane < head) dst[op + lane] = (uint8_t)byte_at(from + lane);
+    const int32_t nwd = (len - head) >> 2;
+    IAMX_GLOBAL uint32_t *dw = gptr<uint32_t>((uint64_t)(dst + op + head));
+    for (int32_t g = lane; g < nwd; g += 64) dw[g] = word_at(from + head + 4 * g);
+    const int32_t t0 = head + 4 * nwd;
+    if (lane < len - t0) dst[op + t0 + lane] = (uint8_t)byte_at(from + t0 + lane);
     op += len;
-  };
-  // bytes needed by a sequence of lit literals and (when ml > 0) a match of ml
-  auto seq_bytes = [](int32_t lit, int32_t ml) -> int32_t {
-    int32_t need = 1 + lit + (lit >= 15 ? (lit - 15) / 255 + 1 : 0);
-    if (ml) need += 2 + (ml - 4 >= 15 ? (ml - 19) / 255 + 1 : 0);
-    return need;
   };
 
   for (int32_t base = 0; base < mflimit; base += 64) {
     const int32_t pos = base + lane;
     const bool act = pos < mflimit;
-    const int32_t lp = min(pos, mflimit - 1);  // lanes past the limit read (and ignore) its bytes
+    const int32_t lp = min(pos, mflimit - 1);
     uint32_t x[5];
 #pragma unroll
     for (int t = 0; t < 5; t++) x[t] = word_at(lp + 4 * t);
     const uint32_t seq = x[0];
-    nz |= act ? seq : 0u;
-    const uint32_t h = (seq * 2654435761u) >> (32 - kHlog);
+    const uint32_t h = (seq * 2654435761u) >> (32 - kSubHlog);
     const uint64_t ent = act ? ht[h] : 0ull;
     __builtin_amdgcn_wave_barrier();
     const int32_t cand = (int32_t)(ent >> 32) - 1;
-    const bool ok = act && cand >= 0 && (uint32_t)ent == seq && pos - cand <= 65535;
+    const bool ok = act && cand >= 0 && (uint32_t)ent == seq;
     int32_t mlen = 0;
     if (ok) {
       int32_t ext = 16;
@@ -172,7 +197,6 @@ __global__ __launch_bounds__(64) void k_lz4_encode(EncGeom G, const uint8_t *__r
       const int32_t off = i - (int32_t)rdl((uint32_t)cand, l);
       int32_t end = i + (int32_t)rdl((uint32_t)mlen, l);
       if (end - i == 20) {
-        // extend 4 bytes a lane until a difference or the match limit
         int32_t p = end;
         for (;;) {
           if (p >= matchlimit) break;
@@ -191,70 +215,121 @@ __global__ __launch_bounds__(64) void k_lz4_encode(EncGeom G, const uint8_t *__r
         end = p;
       }
       if (end > matchlimit) end = matchlimit;
-      const int32_t lit = i - anchor, ml = end - i;
-      if (op + seq_bytes(lit, ml) > cap) {
-        fail = true;
-        break;
-      }
-      if (lane == 0) dst[op] = (uint8_t)((min(lit, 15) << 4) | min(ml - 4, 15));
-      op += 1;
-      if (lit >= 15) put_ext(lit - 15);
-      if (anchor >= base) {
-        // the run lies in this step: lane l holds the byte at base + l
-        const int32_t l0 = anchor - base;
-        if (lane >= l0 && lane < l0 + lit) dst[op + lane - l0] = (uint8_t)seq;
-        op += lit;
+      const int32_t ml = end - i;
+      if (first) {
+        // leading literals are left to the merge: offset and length continuation only
+        first = false;
+        f_pos = i;
+        f_off = off;
+        f_ml = ml;
+        if (lane < 2) dst[op + lane] = (uint8_t)(off >> (8 * lane));
+        op += 2;
+        if (ml - 4 >= 15) put_ext(ml - 19);
       } else {
-        put_global(anchor, lit);
+        const int32_t lit = i - anchor;
+        if (op + 1 + lit + ext_bytes(lit) + 2 + ext_bytes(ml) > cap) {
+          fail = true;
+          break;
+        }
+        if (lane == 0) dst[op] = (uint8_t)((min(lit, 15) << 4) | min(ml - 4, 15));
+        op += 1;
+        if (lit >= 15) put_ext(lit - 15);
+        if (anchor >= base) {
+          const int32_t l0 = anchor - base;
+          if (lane >= l0 && lane < l0 + lit) dst[op + lane - l0] = (uint8_t)seq;
+          op += lit;
+        } else {
+          put_stage(anchor, lit);
+        }
+        if (lane < 2) dst[op + lane] = (uint8_t)(off >> (8 * lane));
+        op += 2;
+        if (ml - 4 >= 15) put_ext(ml - 19);
       }
-      if (lane < 2) dst[op + lane] = (uint8_t)(off >> (8 * lane));
-      op += 2;
-      if (ml - 4 >= 15) put_ext(ml - 19);
       anchor = ip = end;
       mask = ip - base >= 64 ? 0ull : mask & (~0ull << (ip - base));
     }
     if (fail) break;
-    // the literals pending so far no longer fit: the stream is stored raw
-    if (op + (min(base + 64, n) - anchor) >= cap) {
-      fail = true;
-      break;
-    }
   }
-  if (!fail) {
-    const int32_t lit = n - anchor;
-    if (op + seq_bytes(lit, 0) > cap) {
-      fail = true;
-    } else {
-      if (lane == 0) dst[op] = (uint8_t)(min(lit, 15) << 4);
-      op += 1;
-      if (lit >= 15) put_ext(lit - 15);
-      put_global(anchor, lit);
-    }
+  const bool anynz = __ballot(nz != 0) != 0;
+  if (lane == 0) {
+    inf[I_HAS] = first ? 0 : 1;
+    inf[I_POS] = S0 + f_pos;
+    inf[I_OFF] = f_off;
+    inf[I_ML] = f_ml;
+    inf[I_LEN] = op;
+    inf[I_TRAIL] = S0 + anchor;
+    inf[I_NZ] = anynz ? 1 : 0;
+    inf[I_FAIL] = fail ? 1 : 0;
   }
-  // zero stream: the scan saw bytes [0, n - 9); the last (up to) 12 bytes here
-  const int32_t t0 = max(n - 12, 0);
-  if (lane < n - t0) nz |= byte_at(t0 + lane);
-  const bool zero = __ballot(nz != 0) == 0;
-  if (lane == 0) meta[2 * E] = zero ? 0 : fail ? n : op;
 }
 
-// Workgroup copy of len bytes: byte head up to the destination's dword
-// alignment, dword stores realigned from the source, byte tail.
-__device__ void wg_copy(IAMX_GLOBAL uint8_t *d, const IAMX_GLOBAL uint8_t *s, int32_t len, int tid, int nt) {
-  const int32_t head = min((int32_t)((4 - ((uint64_t)d & 3)) & 3), len);
-  if (tid < head) d[tid] = s[tid];
-  const int32_t body = len - head;
-  const int32_t nw = body >> 2;
-  if (nw > 0) {
-    const uint64_t sa = (uint64_t)(s + head);
-    const uint32_t sh = (uint32_t)(sa & 3);
-    const IAMX_GLOBAL uint32_t *sw = gptr<const uint32_t>(sa & ~3ull);
-    const int32_t lastw = (int32_t)((sh + body - 1) >> 2);
-    IAMX_GLOBAL uint32_t *dw = gptr<uint32_t>((uint64_t)(d + head));
-    for (int32_t g = tid; g < nw; g += nt)
-      dw[g] = __builtin_amdgcn_alignbyte(sw[min(g + 1, lastw)], sw[g], sh);
+// Merge: one wavefront per stream joins its sub-blocks into one LZ4 block in
+// the stream's slot (or records a zero / raw stream).
+__global__ __launch_bounds__(64) void k_lz4_merge(EncGeom G, const uint8_t *__restrict__ packed, int64_t e0,
+                                                  int njobs, int32_t K, const uint8_t *__restrict__ subslots,
+                                                  const int32_t *__restrict__ info, uint8_t *__restrict__ slots,
+                                                  int32_t *__restrict__ meta) {
+  if ((int)blockIdx.x >= njobs) return;
+  const int lane = threadIdx.x;
+  const int64_t E = e0 + blockIdx.x;
+  const int64_t sb0 = (int64_t)blockIdx.x * K;  // first sub-block record of this stream
+  const int64_t per_chunk = (int64_t)G.nblocks * G.nstreams;
+  const int64_t c = E / per_chunk, r = E % per_chunk;
+  const int64_t b = r / G.nstreams, s = r % G.nstreams;
+  const int32_t n = G.neblock;
+  const IAMX_GLOBAL uint8_t *srcb =
+      gptr<const uint8_t>((uint64_t)(packed + c * G.chunk_nbytes + b * G.blocksize + s * (int64_t)n));
+  IAMX_GLOBAL uint8_t *dst = gptr<uint8_t>((uint64_t)(slots + E * G.slot));
+  const IAMX_GLOBAL int32_t *inf = gptr<const int32_t>((uint64_t)(info + sb0 * kInfo));
+  // sizes first (uniform scalar walk over the records)
+  bool anynz = false, fail = false;
+  int32_t run = 0, total = 0;
+  for (int32_t k = 0; k < K; k++) {
+    const IAMX_GLOBAL int32_t *q = inf + k * kInfo;
+    anynz |= q[I_NZ] != 0;
+    fail |= q[I_FAIL] != 0;
+    if (q[I_HAS]) {
+      const int32_t lit = q[I_POS] - run;
+      total += 1 + (lit >= 15 ? ext_bytes(lit - 15) : 0) + lit + q[I_LEN];
+      run = q[I_TRAIL];
+    }
   }
-  for (int32_t k = head + 4 * nw + tid; k < len; k += nt) d[k] = s[k];
+  {
+    const int32_t lit = n - run;
+    total += 1 + (lit >= 15 ? ext_bytes(lit - 15) : 0) + lit;
+  }
+  int32_t cs;
+  if (!anynz) {
+    cs = 0;
+  } else if (fail || total >= n) {
+    cs = n;
+  } else {
+    cs = total;
+    int32_t op = 0;
+    run = 0;
+    auto put_run = [&](int32_t lit, int32_t mlc) {  // token, literal continuation, literals [run, run+lit)
+      if (lane == 0) dst[op] = (uint8_t)((min(lit, 15) << 4) | min(mlc, 15));
+      op += 1;
+      if (lit >= 15) {
+        const int32_t v = lit - 15, nf = v / 255;
+        for (int32_t q = lane; q <= nf; q += 64) dst[op + q] = (uint8_t)(q < nf ? 255 : v - 255 * nf);
+        op += nf + 1;
+      }
+      wg_copy(dst + op, srcb + run, lit, lane, 64);
+      op += lit;
+    };
+    for (int32_t k = 0; k < K; k++) {
+      const IAMX_GLOBAL int32_t *q = inf + k * kInfo;
+      if (!q[I_HAS]) continue;
+      put_run(q[I_POS] - run, q[I_ML] - 4);
+      const int32_t len = q[I_LEN];
+      wg_copy(dst + op, gptr<const uint8_t>((uint64_t)(subslots + (sb0 + k) * kSubSlot)), len, lane, 64);
+      op += len;
+      run = q[I_TRAIL];
+    }
+    put_run(n - run, 0);
+  }
+  if (lane == 0) meta[2 * E] = cs;
 }
 
 __global__ __launch_bounds__(256) void k_chunk_offsets(EncGeom G, int64_t c0, int32_t *__restrict__ meta,
@@ -336,6 +411,22 @@ bool geom_of(const iamx_tiling_t *t, int split, EncGeom *G) {
   return G->chunk_nbytes <= INT32_MAX;
 }
 
+// scratch layout: stream slots (all streams of the tiling), then sub-block
+// slots and info records (for the streams of one call)
+struct ScratchLayout {
+  int64_t nst_all, K, sub_off, info_off, bytes;
+};
+
+ScratchLayout layout_of(const iamx_tiling_t *t, const EncGeom &G) {
+  ScratchLayout L;
+  L.nst_all = t->nchunks[0] * t->nchunks[1] * (int64_t)G.nblocks * G.nstreams;
+  L.K = (G.neblock + kSub - 1) / kSub;
+  L.sub_off = L.nst_all * G.slot;
+  L.info_off = L.sub_off + L.nst_all * L.K * kSubSlot;
+  L.bytes = L.info_off + L.nst_all * L.K * kInfo * 4 + 256;
+  return L;
+}
+
 }  // namespace
 
 extern "C" {
@@ -345,8 +436,13 @@ IAMX_API int64_t iamx_chunk_capacity(const iamx_tiling_t *t, int split) {
   return geom_of(t, split, &G) ? G.cap : -1;
 }
 
+IAMX_API int64_t iamx_encode_scratch_bytes(const iamx_tiling_t *t, int split) {
+  EncGeom G;
+  return geom_of(t, split, &G) ? layout_of(t, G).bytes : -1;
+}
+
 IAMX_API int iamx_encode_chunks(const iamx_tiling_t *t, int split, const void *d_packed, int64_t chunk_begin,
-                                int64_t chunk_end, void *d_slots, int32_t *d_meta, void *d_out,
+                                int64_t chunk_end, void *d_scratch, int32_t *d_meta, void *d_out,
                                 int64_t *d_chunk_size) {
   int rc = iamx::ensure_init();
   if (rc) return rc;
@@ -356,25 +452,25 @@ IAMX_API int iamx_encode_chunks(const iamx_tiling_t *t, int split, const void *d
   if (chunk_begin < 0 || chunk_end > nchunks || chunk_begin > chunk_end)
     return iamx::set_error(IAMX_EINVAL, "iamx_encode_chunks: chunk range [%lld, %lld) outside [0, %lld)",
                            (long long)chunk_begin, (long long)chunk_end, (long long)nchunks);
+  const ScratchLayout L = layout_of(t, G);
   const int64_t per_chunk = (int64_t)G.nblocks * G.nstreams;
   const int64_t nst = (chunk_end - chunk_begin) * per_chunk;
   if (nst == 0) return IAMX_OK;
-  if (nst > INT32_MAX || chunk_end - chunk_begin > INT32_MAX)
+  const int64_t nsub = nst * L.K;
+  if (nsub > INT32_MAX || chunk_end - chunk_begin > INT32_MAX)
     return iamx::set_error(IAMX_EINVAL, "iamx_encode_chunks: too many streams");
   const int64_t e0 = chunk_begin * per_chunk;
-  // streams up to 16 KiB are staged in LDS (32 KiB with the table: 5
-  // wavefronts a CU); longer ones are read from global memory, which measured
-  // 2.1x faster on 64 KiB streams than an 80 KiB staging (one wavefront a CU)
-  if (G.neblock <= 16384)
-    IAMX_LAUNCH("lz4_encode", k_lz4_encode<16384>, dim3((unsigned)nst), dim3(64), 0, G, (const uint8_t *)d_packed,
-                e0, (int)nst, (uint8_t *)d_slots, d_meta);
-  else
-    IAMX_LAUNCH("lz4_encode", k_lz4_encode<0>, dim3((unsigned)nst), dim3(64), 0, G, (const uint8_t *)d_packed, e0,
-                (int)nst, (uint8_t *)d_slots, d_meta);
+  uint8_t *slots = (uint8_t *)d_scratch;
+  uint8_t *subslots = slots + L.sub_off;
+  int32_t *info = (int32_t *)(slots + L.info_off);
+  IAMX_LAUNCH("lz4_encode", k_lz4_encode_sub, dim3((unsigned)nsub), dim3(64), 0, G, (const uint8_t *)d_packed, e0,
+              nsub, (int32_t)L.K, subslots, info);
+  IAMX_LAUNCH("lz4_merge", k_lz4_merge, dim3((unsigned)nst), dim3(64), 0, G, (const uint8_t *)d_packed, e0,
+              (int)nst, (int32_t)L.K, (const uint8_t *)subslots, (const int32_t *)info, slots, d_meta);
   IAMX_LAUNCH("chunk_offsets", k_chunk_offsets, dim3((unsigned)(chunk_end - chunk_begin)), dim3(256), 0, G,
               chunk_begin, d_meta, (uint8_t *)d_out, d_chunk_size);
   IAMX_LAUNCH("chunk_copy", k_chunk_copy, dim3((unsigned)nst), dim3(256), 0, G, e0, (const uint8_t *)d_packed,
-              (const uint8_t *)d_slots, (const int32_t *)d_meta, (uint8_t *)d_out);
+              (const uint8_t *)slots, (const int32_t *)d_meta, (uint8_t *)d_out);
   return IAMX_OK;
 }
 

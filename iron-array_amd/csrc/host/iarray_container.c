@@ -769,9 +769,10 @@ static ina_rc_t write_gpu_encoded(iarray_container_t *c, const iamx_tiling_t *t,
   if (cap <= 0) return iax_dev_error(IAMX_EINVAL, "chunk capacity");
   const int64_t nstreams = split ? c->itemsize : 1;
   const int64_t nst = c->nchunks * c->nblocks_ax[0] * c->nblocks_ax[1] * nstreams;
-  const int64_t slot = (c->blocksize / nstreams + 15) / 16 * 16;
+  const int64_t scratch = iamx_encode_scratch_bytes(t, split);
+  if (scratch <= 0) return iax_dev_error(IAMX_EINVAL, "encode scratch");
   void *d_packed = iamx_workspace(5, (size_t)(c->nchunks * c->chunk_nbytes));
-  void *d_slots = iamx_workspace(12, (size_t)(nst * slot));
+  void *d_slots = iamx_workspace(12, (size_t)scratch);
   int32_t *d_meta = (int32_t *)iamx_workspace(13, (size_t)(nst * 8 + c->nchunks * 8 + 16));
   uint8_t *d_out = (uint8_t *)iamx_workspace(14, (size_t)(c->nchunks * cap));
   if (!d_packed || !d_slots || !d_meta || !d_out) return iax_dev_error(IAMX_ENOMEM, "encode buffers");
@@ -825,10 +826,12 @@ static ina_rc_t write_gpu_encoded(iarray_container_t *c, const iamx_tiling_t *t,
 }
 
 /* The GPU encoder (IARRAY_GPU_ENCODE=1) covers LZ4 chunks of whole blocks
- * with up to 8 streams.  It is not the default: its parse is one serial walk
- * per stream, and on 8192^2 float32 products (64 KiB streams, one stream in
- * four dense with short matches) that walk took 3.8 ms per 1024-row panel,
- * against a host encode that overlaps the next panels' GEMMs. */
+ * with up to 8 streams.  It is not the default: it adds its own time to the
+ * GPU's (0.65 ms per 1024-row panel of an 8192^2 float32 product, 1.0 ms per
+ * 1500-row panel of a 6000^2 float64 one), while the host encode of a panel
+ * runs during the next panels' GEMMs; end to end the host encode measured
+ * 21.8 ms against 23.5 ms (split-bf16) and 27.3 against 33.1 ms (plain) on
+ * the 8192^2 product. */
 static int gpu_encode_ok(const iarray_container_t *c) {
   const char *env = getenv("IARRAY_GPU_ENCODE");
   if (!env || atoi(env) != 1) return 0;
