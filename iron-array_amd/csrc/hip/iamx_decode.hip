@@ -132,50 +132,79 @@ __global__ __launch_bounds__(64) void k_lz4_streams(const iamx_lz4_job_t *__rest
   int bad = 0;
   for (;;) {
     refill(ip);
-    // Fast loop: sequences with 0..14 literals and a 4..18 byte match whose
-    // header, literals and offset all lie in w0.  It runs while the input
-    // position stays 20 bytes short of w0's end and of the stream end, the
-    // output position 32 bytes short of the slot end and below the next
-    // flush; each test below is a difference whose sign bit flags failure.
-    {
-      const int32_t ilim = min(wb + 236, iend - 17);
-      const int32_t olim = min(oend - 32, flush_at - 1);
-      while (((ilim - ip) | (olim - op)) >= 0) {
-        const int32_t q = ip - wb;
-        const int32_t l = q >> 2;
-        const uint32_t h = (uint32_t)((((uint64_t)rdlane(w0, l + 1)) << 32 | rdlane(w0, l)) >> ((q & 3) << 3));
-        const int32_t lit = (int32_t)((h >> 4) & 15u);
-        const int32_t mlc = (int32_t)(h & 15u);
-        int32_t off;
-        if (lit == 0) {
-          off = (int32_t)((h >> 8) & 0xffffu);
-        } else {
-          const int32_t qo = q + 1 + lit;
-          const int32_t lo2 = qo >> 2;
-          off = (int32_t)(((((uint64_t)rdlane(w0, lo2 + 1)) << 32 | rdlane(w0, lo2)) >> ((qo & 3) << 3)) & 0xffffu);
+    // Fast section.  For the 64 input positions from the parse position on,
+    // lane l decodes the header of the sequence that would start at position
+    // ip + l (the window bytes are gathered with ds_bpermute) into one word:
+    // offset (bits 0-15), literal count (16-19), match length (20-28), a
+    // length continuation byte (29) and a fast flag (31): 0..14 literals, a
+    // match of 4..273 bytes (at most one continuation byte), the header inside
+    // the stream and starting at most 236 bytes into w0 (so its literals lie
+    // in w0).  The parse then costs one readlane per sequence.  It runs while
+    // the output position stays 288 bytes short of the slot end and below the
+    // next flush; each test is a difference whose sign bit flags failure.
+    if (ip - wb <= 236) {
+      const int32_t olim = min(oend - 288, flush_at - 1);
+      int32_t qs = ip - wb;  // window position of lane 0's candidate
+      uint32_t info = 0;
+      auto decode_headers = [&]() {
+        const int32_t r = qs + lane;  // <= 299
+        auto gather = [&](int32_t d) -> uint32_t {  // window dword d < 128
+          const uint32_t a0 = (uint32_t)__shfl((int)w0, d & 63, 64);
+          const uint32_t a1 = (uint32_t)__shfl((int)w1, d & 63, 64);
+          return d < 64 ? a0 : a1;
+        };
+        const uint32_t h = __builtin_amdgcn_alignbyte(gather((r >> 2) + 1), gather(r >> 2), (uint32_t)(r & 3));
+        const uint32_t hl = (h >> 4) & 15u, hm = h & 15u;
+        const int32_t r2 = r + 1 + (int32_t)hl;  // offset field, <= 315
+        const uint32_t h2 = __builtin_amdgcn_alignbyte(gather((r2 >> 2) + 1), gather(r2 >> 2), (uint32_t)(r2 & 3));
+        const uint32_t hoff = h2 & 0xffffu, he = (h2 >> 16) & 255u;
+        const uint32_t ext = hm == 15 ? 1u : 0u;
+        const uint32_t hml = hm + 4 + (ext ? he : 0u);
+        const int32_t hdr = 3 + (int32_t)hl + (int32_t)ext;
+        const bool hfast = hl < 15 && !(ext && he == 255) && hoff >= 1 && r <= 236 && hdr <= iend - wb - r;
+        info = hoff | (hl << 16) | (hml << 20) | (ext << 29) | (hfast ? 0x80000000u : 0u);
+      };
+      for (;;) {
+        decode_headers();
+        // one exit: the next header is read at the bottom of the body
+        int32_t l = ip - wb - qs;
+        uint32_t inf = rdlane(info, l & 63);
+        int32_t lit = (int32_t)((inf >> 16) & 15u);
+        int32_t off = (int32_t)(inf & 0xffffu);
+        while (((int32_t)~inf | (63 - l) | (min(op + lit, NEAR) - off) | (olim - op)) >= 0) {
+          const int32_t ml = (int32_t)((inf >> 20) & 511u);
+          const int32_t o2 = op + lit;
+          if (lit) {
+            const int32_t ql = ip - wb + 1 + lane;  // < 256 + 64
+            const uint32_t a = (uint32_t)__shfl((int)w0, (ql >> 2) & 63, 64);
+            ring[(op + lane) & RM] = (uint8_t)(a >> ((ql & 3) << 3));
+            __builtin_amdgcn_wave_barrier();
+          }
+          int32_t back = off;
+          if (off < 64) {
+            const int32_t num = lane + off;
+            int32_t qq = (int32_t)((float)num * __builtin_amdgcn_rcpf((float)off));
+            const int32_t rr = num - qq * off;
+            qq += (rr >= off) - (rr < 0);
+            back = off * qq;
+          }
+          int32_t k = 0;
+          do {
+            const uint8_t v = ring[(o2 + k + lane - back) & RM];
+            __builtin_amdgcn_wave_barrier();
+            ring[(o2 + k + lane) & RM] = v;
+            __builtin_amdgcn_wave_barrier();
+            k += 64;
+          } while (k < ml);
+          ip += 3 + lit + (int32_t)((inf >> 29) & 1u);
+          op = o2 + ml;
+          l = ip - wb - qs;
+          inf = rdlane(info, l & 63);
+          lit = (int32_t)((inf >> 16) & 15u);
+          off = (int32_t)(inf & 0xffffu);
         }
-        const int32_t o2 = op + lit;
-        if (((14 - lit) | (14 - mlc) | (off - 1) | (min(o2, NEAR) - off)) < 0) break;
-        if (lit) {
-          const int32_t ql = q + 1 + lane;  // < 256 + 64
-          const uint32_t a = (uint32_t)__shfl((int)w0, (ql >> 2) & 63, 64);
-          ring[(op + lane) & RM] = (uint8_t)(a >> ((ql & 3) << 3));
-          __builtin_amdgcn_wave_barrier();
-        }
-        int32_t back = off;
-        if (off < 64) {
-          const int32_t num = lane + off;
-          int32_t qq = (int32_t)((float)num * __builtin_amdgcn_rcpf((float)off));
-          const int32_t r = num - qq * off;
-          qq += (r >= off) - (r < 0);
-          back = off * qq;
-        }
-        const uint8_t v = ring[(o2 + lane - back) & RM];
-        __builtin_amdgcn_wave_barrier();
-        ring[(o2 + lane) & RM] = v;
-        __builtin_amdgcn_wave_barrier();
-        ip += 3 + lit;
-        op = o2 + mlc + 4;
+        if (l < 64 || ip - wb > 236 || op > olim) break;
+        qs = ip - wb;  // the next 64 positions
       }
     }
     if (op >= flush_at) {

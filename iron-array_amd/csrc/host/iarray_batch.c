@@ -39,6 +39,9 @@ typedef struct {
   iamx_batch_pack_t *d_packs;
   int64_t packed_bytes;
   int64_t *out_off; /* per product: byte offset of its packed blocks */
+  /* the captured launches' arguments, for an eager replay */
+  int n_lz4, n_blk, ntiles, npb;
+  int64_t b_base; /* byte offset of the B images in d_dense */
 } batch_plan_t;
 
 static batch_plan_t g_plan;
@@ -218,6 +221,11 @@ static ina_rc_t build_plan(batch_plan_t *p, int n, iarray_container_t **a, iarra
   }
   if (INA_SUCCEED(rc)) {
     uint8_t *dense_b = (uint8_t *)p->d_dense + na * is;
+    p->n_lz4 = (int)il;
+    p->n_blk = (int)ib;
+    p->ntiles = (int)ntiles;
+    p->npb = (int)npb;
+    p->b_base = na * is;
     int e = iamx_capture_begin();
     if (!e) e = iamx_decode(p->d_lz4, (int)il, p->d_blocks, (int)ib, p->d_scratch, p->d_dense);
     if (!e) e = iamx_gemm_batch(dt, p->d_prods, p->d_tiles, (int)ntiles, p->d_dense, dense_b, p->d_dense_c);
@@ -318,7 +326,19 @@ INA_API(ina_rc_t) iarray_linalg_matmul_batch(iarray_context_t *ctx, int n, iarra
     return rc;
   }
   iamx_prof_reset();
-  int e = iamx_graph_launch(g_plan.graph);
+  /* IARRAY_BATCH_EAGER=1 launches the captured stages one by one (same
+   * kernels and arguments), so each can be timed on its own */
+  const char *eager = getenv("IARRAY_BATCH_EAGER");
+  int e;
+  if (eager && atoi(eager) == 1) {
+    e = iamx_decode(g_plan.d_lz4, g_plan.n_lz4, g_plan.d_blocks, g_plan.n_blk, g_plan.d_scratch, g_plan.d_dense);
+    if (!e)
+      e = iamx_gemm_batch(dt, g_plan.d_prods, g_plan.d_tiles, g_plan.ntiles, g_plan.d_dense,
+                          (uint8_t *)g_plan.d_dense + g_plan.b_base, g_plan.d_dense_c);
+    if (!e) e = iamx_pack_batch(g_plan.d_packs, g_plan.d_pblocks, g_plan.npb, g_plan.d_dense_c, g_plan.d_packed);
+  } else {
+    e = iamx_graph_launch(g_plan.graph);
+  }
   if (e) return iax_dev_error(e, "batch replay");
   int st = iamx_decode_status();
   iax_last_device_ms = iamx_prof_span_ms();

@@ -134,17 +134,22 @@ static int gemm_rows(void *v, int64_t row0, int64_t nrows) {
   return iamx_gemm_rows(nrows, x->A + row0 * x->K * x->is, x->K, x->C + row0 * x->N * x->is, x->N);
 }
 
-/* rows per C panel (rounded up to whole chunk rows): M/8, at least 256, so
- * the encode of seven panels overlaps GEMM work (8192-row products measured
- * 27.3 ms with 1024-row panels against 27.6 ms with 2048 and 36.4 ms with one
- * panel); IARRAY_GEMM_PANEL_ROWS overrides (0 = one panel) */
-static int64_t panel_rows(int64_t M) {
+/* rows per C panel (rounded up to whole chunk rows): M/8, so the encode of
+ * seven panels overlaps GEMM work (8192-row products measured 27.3 ms with
+ * 1024-row panels against 27.6 ms with 2048 and 36.4 ms with one panel), but
+ * at least 2^23/N rows, 512 tiles of 128x128, so a panel still fills the 256
+ * CUs (2000^2 products: 4 panels of 500 rows took 0.375 ms each, one launch
+ * for all of C 0.39 ms); IARRAY_GEMM_PANEL_ROWS overrides (0 = one panel) */
+static int64_t panel_rows(int64_t M, int64_t N) {
   const char *env = getenv("IARRAY_GEMM_PANEL_ROWS");
   if (env) {
     const long long v = atoll(env);
     return v > 0 ? (int64_t)v : M;
   }
-  return M / 8 > 256 ? M / 8 : 256;
+  int64_t r = M / 8;
+  const int64_t fill = ((int64_t)1 << 23) / (N > 0 ? N : 1);
+  if (r < fill) r = fill;
+  return r;
 }
 
 /* C = A B with A, B (views allowed) already validated; c allocated.  C is
@@ -189,7 +194,7 @@ static ina_rc_t run_gemm(iarray_container_t *a, iarray_container_t *b, iarray_co
   int e = iamx_gemm_prepare_b(dt, prec, slices, N, K, dB, N);
   if (e) return iax_dev_error(e, "gemm");
   rows_ctx_t rx = {(const uint8_t *)dA, (uint8_t *)dC, K, N, a->itemsize};
-  rc = iax_write_from_dense_rows(c, dC, panel_rows(M), gemm_rows, &rx);
+  rc = iax_write_from_dense_rows(c, dC, panel_rows(M, N), gemm_rows, &rx);
   if (INA_FAILED(rc)) return rc;
   return check_status();
 }
