@@ -38,7 +38,7 @@ struct Mfma<float> {
   static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
-constexpr int BM = 128, BN = 128, BK = 16, PAD = 4;
+constexpr int BM = 128, BN = 128, PAD = 4;
 
 // Map the launch index onto output tiles so that the 8 XCDs (workgroups are
 // dealt round-robin to XCDs) each get a contiguous band of tile rows: tiles
@@ -54,42 +54,71 @@ __device__ __forceinline__ void tile_coords(int &tm, int &tn, int tiles_m, int t
   tn = id % tiles_n;
 }
 
+// K slab depth: 32 for float32 (LDS 2 x 2 x 32 x 132 x 4 B = 68 KB), 16 for
+// float64 (the same bytes).
+template <typename T>
+struct Slab {
+  static constexpr int BK = sizeof(T) == 4 ? 32 : 16;
+};
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_gemm(int64_t M, int64_t N, int64_t K, const T *__restrict__ A, int64_t lda,
                                               const T *__restrict__ B, int64_t ldb, T *__restrict__ C, int64_t ldc,
                                               int tiles_m, int tiles_n) {
+  constexpr int BK = Slab<T>::BK;
+  constexpr int PER = BK * BM / 256;  // elements per thread per operand and slab (16 f32, 8 f64)
+  constexpr int VEC = 16 / sizeof(T);  // elements per 16-byte load
   __shared__ T As[2][BK][BM + PAD];
   __shared__ T Bs[2][BK][BN + PAD];
   typedef typename Mfma<T>::acc_t acc_t;
+  typedef T vec_t __attribute__((ext_vector_type(VEC)));
   int tm, tn;
   tile_coords(tm, tn, tiles_m, tiles_n);
   const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
 
-  // global->LDS mapping: A: row tid/2, k (tid%2)*8..+8 ; B: k tid/16, col (tid%16)*8..+8
-  const int a_r = tid >> 1, a_k = (tid & 1) * 8;
-  const int b_k = tid >> 4, b_c = (tid & 15) * 8;
-  T ra[8], rb[8];
+  // global->LDS mapping: A: row tid/2, k (tid%2)*PER..+PER ; B: k tid/(BN/PER), cols ..+PER
+  const int a_r = tid >> 1, a_k = (tid & 1) * PER;
+  const int b_k = tid / (BN / PER), b_c = (tid % (BN / PER)) * PER;
+  const bool rows_in = m0 + BM <= M && n0 + BN <= N;
+  T ra[PER], rb[PER];
   auto gload = [&](int64_t k0) {
+    if (rows_in && k0 + BK <= K) {
+      // interior slab: 16-byte loads
+      const T *pa = A + (m0 + a_r) * lda + k0 + a_k;
+      const T *pb = B + (k0 + b_k) * ldb + n0 + b_c;
+#pragma unroll
+      for (int v = 0; v < PER / VEC; v++) {
+        vec_t x, y;
+        __builtin_memcpy(&x, pa + v * VEC, sizeof(x));
+        __builtin_memcpy(&y, pb + v * VEC, sizeof(y));
+#pragma unroll
+        for (int e = 0; e < VEC; e++) {
+          ra[v * VEC + e] = x[e];
+          rb[v * VEC + e] = y[e];
+        }
+      }
+      return;
+    }
     const int64_t gr = m0 + a_r;
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
+    for (int i = 0; i < PER; i++) {
       const int64_t gk = k0 + a_k + i;
       ra[i] = (gr < M && gk < K) ? A[gr * lda + gk] : (T)0;
     }
     const int64_t gk = k0 + b_k;
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
+    for (int i = 0; i < PER; i++) {
       const int64_t gc = n0 + b_c + i;
       rb[i] = (gk < K && gc < N) ? B[gk * ldb + gc] : (T)0;
     }
   };
   auto sstore = [&](int buf) {
 #pragma unroll
-    for (int i = 0; i < 8; i++) As[buf][a_k + i][a_r] = ra[i];
+    for (int i = 0; i < PER; i++) As[buf][a_k + i][a_r] = ra[i];
 #pragma unroll
-    for (int i = 0; i < 8; i++) Bs[buf][b_k][b_c + i] = rb[i];
+    for (int i = 0; i < PER; i++) Bs[buf][b_k][b_c + i] = rb[i];
   };
 
   acc_t acc[4][4];
