@@ -42,7 +42,7 @@ IAMX_API int iamx_memset(void *dst, int value, size_t bytes);
 IAMX_API int iamx_sync(void);
 /* Pinned host staging buffer (grows, kept until iamx_shutdown) and
  * downloads into it: iamx_d2h_async queues the copy on a copy stream, after
- * all work queued so far on the library stream, and records event `event`
+ * all work queued so far on the current compute stream, and records event `event`
  * (0..IAMX_COPY_EVENTS-1) after it, so kernels queued later overlap the copy;
  * iamx_event_sync blocks the host until that event has completed. */
 #define IAMX_COPY_EVENTS 64
@@ -53,6 +53,12 @@ IAMX_API int iamx_event_sync(int event);
  * waiting for compute work queued since the last iamx_d2h_async (whose data
  * it may therefore not read); records `event`. */
 IAMX_API int iamx_d2h_chain(void *dst, const void *src, size_t bytes, int event);
+/* A second compute stream (default priority).  iamx_side_wait makes it wait
+ * for the work queued so far on the library stream; between iamx_use_stream(1)
+ * and iamx_use_stream(0) launches and the ready point of iamx_d2h_async go to
+ * it, so that work overlaps what the library stream runs next. */
+IAMX_API int iamx_side_wait(void);
+IAMX_API int iamx_use_stream(int side);
 /* Scratch that lives until iamx_shutdown; grown (never shrunk) on demand.
  * Each slot is independent of the others. */
 /* slots 0..7 and 12..14 belong to the host library, 8..9 to the Ozaki slice

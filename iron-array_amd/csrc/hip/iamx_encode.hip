@@ -42,8 +42,6 @@ namespace {
 
 using iamx::gptr;
 
-constexpr int kHlog = 11;
-constexpr int kHsize = 1 << kHlog;
 
 struct EncGeom {
   int64_t chunk_nbytes;  // bytes of one (extended) chunk in the packed layout

@@ -598,10 +598,10 @@ static int tuned(int form, int64_t M, int64_t N, int64_t K, F run, int *rc) {
   const int tiles[2] = {128, 256};
   float ms[2] = {0, 0};
   *rc = IAMX_OK;
-  (void)hipEventRecord(ev[0], iamx::g_stream);
+  (void)hipEventRecord(ev[0], iamx::g_cur);
   for (int t = 0; t < 2 && !*rc; t++) {
     *rc = run(tiles[t]);
-    (void)hipEventRecord(ev[t + 1], iamx::g_stream);
+    (void)hipEventRecord(ev[t + 1], iamx::g_cur);
   }
   int best = 256;
   if (!*rc && hipEventSynchronize(ev[2]) == hipSuccess) {

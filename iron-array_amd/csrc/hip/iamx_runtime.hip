@@ -16,6 +16,8 @@
 namespace iamx {
 
 hipStream_t g_stream = nullptr;
+hipStream_t g_cur = nullptr;
+static hipStream_t g_side = nullptr;  // second compute stream, default priority
 int g_num_cus = 0;
 static int g_init_state = 0;  // 0 not tried, 1 ok, <0 failed
 static char g_err[512] = "";
@@ -51,6 +53,9 @@ int ensure_init() {
   g_num_cus = p.multiProcessorCount;
   if ((e = hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking)) != hipSuccess)
     return check_hip(e, "hipStreamCreateWithFlags");
+  if ((e = hipStreamCreateWithFlags(&g_side, hipStreamNonBlocking)) != hipSuccess)
+    return check_hip(e, "hipStreamCreateWithFlags");
+  g_cur = g_stream;
   if ((e = hipMalloc(&g_status, 256)) != hipSuccess) return check_hip(e, "hipMalloc(status)");
   if ((e = hipMemset(g_status, 0, 256)) != hipSuccess) return check_hip(e, "hipMemset(status)");
   if ((e = hipDeviceSynchronize()) != hipSuccess) return check_hip(e, "hipDeviceSynchronize");
@@ -102,7 +107,7 @@ ProfScope::ProfScope(const char *name) {
   }
   Pending p{idx, get_event(), get_event()};
   if (!p.a || !p.b) return;
-  (void)hipEventRecord(p.a, g_stream);
+  (void)hipEventRecord(p.a, g_cur);
   if (!g_first) g_first = p.a;
   g_pending.push_back(p);
   slot = (int)g_pending.size() - 1;
@@ -110,13 +115,14 @@ ProfScope::ProfScope(const char *name) {
 
 ProfScope::~ProfScope() {
   if (slot < 0) return;
-  (void)hipEventRecord(g_pending[slot].b, g_stream);
+  (void)hipEventRecord(g_pending[slot].b, g_cur);
   g_last = g_pending[slot].b;
 }
 
 static void prof_collect() {
   if (g_pending.empty()) return;
   (void)hipStreamSynchronize(g_stream);
+  if (g_side) (void)hipStreamSynchronize(g_side);
   for (auto &p : g_pending) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
@@ -209,6 +215,7 @@ static hipStream_t g_copy_stream = nullptr;
 
 static void sync_streams() {
   (void)hipStreamSynchronize(g_stream);
+  if (g_side) (void)hipStreamSynchronize(g_side);
   if (g_copy_stream) (void)hipStreamSynchronize(g_copy_stream);
 }
 
@@ -244,7 +251,7 @@ IAMX_API int iamx_d2h_async(void *dst, const void *src, size_t bytes, int event)
     IAMX_HIP_CHECK(hipStreamCreateWithFlags(&g_copy_stream, hipStreamNonBlocking));
     g_copy_ev_made = true;
   }
-  IAMX_HIP_CHECK(hipEventRecord(g_ready_ev[event], g_stream));
+  IAMX_HIP_CHECK(hipEventRecord(g_ready_ev[event], g_cur));
   IAMX_HIP_CHECK(hipStreamWaitEvent(g_copy_stream, g_ready_ev[event], 0));
   if (bytes) IAMX_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_copy_stream));
   IAMX_HIP_CHECK(hipEventRecord(g_copy_ev[event], g_copy_stream));
@@ -290,7 +297,27 @@ IAMX_API int iamx_sync(void) {
   int rc = ensure_init();
   if (rc) return rc;
   IAMX_HIP_CHECK(hipStreamSynchronize(g_stream));
+  if (g_side) IAMX_HIP_CHECK(hipStreamSynchronize(g_side));
   if (g_copy_stream) IAMX_HIP_CHECK(hipStreamSynchronize(g_copy_stream));
+  return IAMX_OK;
+}
+
+IAMX_API int iamx_use_stream(int side) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (side && capturing()) return set_error(IAMX_EINVAL, "iamx_use_stream: not available under graph capture");
+  g_cur = side ? g_side : g_stream;
+  return IAMX_OK;
+}
+
+IAMX_API int iamx_side_wait(void) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (capturing()) return set_error(IAMX_EINVAL, "iamx_side_wait: not available under graph capture");
+  static hipEvent_t ev = nullptr;
+  if (!ev) IAMX_HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  IAMX_HIP_CHECK(hipEventRecord(ev, g_stream));
+  IAMX_HIP_CHECK(hipStreamWaitEvent(g_side, ev, 0));
   return IAMX_OK;
 }
 

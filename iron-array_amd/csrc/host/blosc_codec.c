@@ -147,11 +147,116 @@ static int emit(uint8_t **op, uint8_t *oend, const uint8_t *lit, size_t litlen, 
   return 1;
 }
 
+/* Streams of at most 64 KiB: 16-bit positions in a 2^12-entry table (8 KiB,
+ * no empty marker: a candidate counts only when its 4 bytes are equal), the
+ * catch-up of a match backwards, and a probe of the position right after
+ * each match before searching on.  Literal runs of up to 16 bytes are copied
+ * 16 bytes at a time when both buffers have the room. */
+#define IAX_HLOG16 12
+static inline uint32_t hash16(uint32_t v) { return (v * 2654435761u) >> (32 - IAX_HLOG16); }
+
+static int put_seq(uint8_t **opp, uint8_t *oend, const uint8_t *anchor, const uint8_t *srcend, size_t lit, size_t off,
+                   size_t ml) {
+  uint8_t *op = *opp;
+  const size_t need = 1 + lit + lit / 255 + 1 + (ml ? 2 + (ml - 4) / 255 + 1 : 0);
+  if ((size_t)(oend - op) < need) return 0;
+  uint8_t *tok = op++;
+  size_t t = (lit < 15 ? lit : 15) << 4;
+  if (lit >= 15) {
+    size_t v = lit - 15;
+    for (; v >= 255; v -= 255) *op++ = 255;
+    *op++ = (uint8_t)v;
+  }
+  if (lit <= 16 && oend - op >= 16 && srcend - anchor >= 16) memcpy(op, anchor, 16);
+  else memcpy(op, anchor, lit);
+  op += lit;
+  if (ml) {
+    op[0] = (uint8_t)off;
+    op[1] = (uint8_t)(off >> 8);
+    op += 2;
+    const size_t m = ml - 4;
+    t |= m < 15 ? m : 15;
+    if (m >= 15) {
+      size_t v = m - 15;
+      for (; v >= 255; v -= 255) *op++ = 255;
+      *op++ = (uint8_t)v;
+    }
+  }
+  *tok = (uint8_t)t;
+  *opp = op;
+  return 1;
+}
+
+static int compress_small(const uint8_t *src, int n, uint8_t *dst, int cap, int accel, uint16_t *ht) {
+  uint8_t *op = dst, *const oend = dst + cap;
+  const uint8_t *const srcend = src + n;
+  const uint8_t *anchor = src;
+  if (n >= 13) {
+    memset(ht, 0, sizeof(uint16_t) << IAX_HLOG16);
+    const uint8_t *const mflimit = src + n - 12; /* a match starts before this */
+    const uint8_t *const matchlimit = src + n - 5; /* and ends at or before this */
+    const uint8_t *ip = src + 1;
+    for (;;) {
+      const uint8_t *ref;
+      int misses = 0;
+      for (;;) { /* search */
+        if (ip >= mflimit) goto last;
+        const uint32_t seq = rd32(ip);
+        const uint32_t h = hash16(seq);
+        ref = src + ht[h];
+        ht[h] = (uint16_t)(ip - src);
+        if (ref < ip && rd32(ref) == seq) break;
+        ip += 1 + ((misses++ * accel) >> 6);
+      }
+      while (ip > anchor && ref > src && ip[-1] == ref[-1]) { /* catch up */
+        ip--;
+        ref--;
+      }
+      for (;;) {
+        const uint8_t *p = ip + 4, *r = ref + 4;
+        while (p + 8 <= matchlimit) {
+          const uint64_t x = rd64(p) ^ rd64(r);
+          if (x) {
+            p += __builtin_ctzll(x) >> 3;
+            goto ended;
+          }
+          p += 8;
+          r += 8;
+        }
+        while (p < matchlimit && *p == *r) {
+          p++;
+          r++;
+        }
+      ended:
+        if (p > matchlimit) p = matchlimit;
+        if (!put_seq(&op, oend, anchor, srcend, (size_t)(ip - anchor), (size_t)(ip - ref), (size_t)(p - ip)))
+          return 0;
+        ip = p;
+        anchor = ip;
+        if (ip >= mflimit) goto last;
+        ht[hash16(rd32(ip - 2))] = (uint16_t)(ip - 2 - src);
+        const uint32_t seq = rd32(ip); /* the position right after the match */
+        const uint32_t h = hash16(seq);
+        ref = src + ht[h];
+        ht[h] = (uint16_t)(ip - src);
+        if (!(ref < ip && rd32(ref) == seq)) {
+          ip++;
+          break;
+        }
+      }
+    }
+  }
+last:
+  if (!put_seq(&op, oend, anchor, srcend, (size_t)(srcend - anchor), 0, 0)) return 0;
+  return (int)(op - dst);
+}
+
 /* Returns the compressed size, or 0 when the output does not fit in `cap`. */
 int iax_lz4_compress(const uint8_t *src, int n, uint8_t *dst, int cap, int accel, uint32_t *ht) {
+  if (accel < 1) accel = 1;
+  if (n <= 65536) return compress_small(src, n, dst, cap, accel, (uint16_t *)ht);
   uint8_t *op = dst, *const oend = dst + cap;
   int anchor = 0;
-  if (accel < 1) accel = 1;
   if (n >= 13) {
     memset(ht, 0, sizeof(uint32_t) << IAX_HLOG);
     const int mflimit = n - 12;     /* a match may start at ip < mflimit */

@@ -789,15 +789,25 @@ static ina_rc_t write_gpu_encoded(iarray_container_t *c, const iamx_tiling_t *t,
     e = (call);                           \
     if (e) return iax_dev_error(e, what); \
   } while (0)
-  /* queue panel k: compute, pack, encode */
-#define ENC_PRODUCE(k)                                                                                          \
-  do {                                                                                                          \
-    ENC_TRY(produce_panel(&px, (k), pf[k], pf[(k) + 1]), "panel");                                              \
-    ENC_TRY(iamx_encode_chunks(t, split, d_packed, pf[k], pf[(k) + 1], d_slots, d_meta, d_out, d_sizes), "encode"); \
+  /* queue panel k: compute and pack on the library stream, encode on the
+   * side stream once the pack is done (it overlaps panel k+1's GEMM) */
+#define ENC_PRODUCE(k)                                                                                            \
+  do {                                                                                                            \
+    ENC_TRY(produce_panel(&px, (k), pf[k], pf[(k) + 1]), "panel");                                                \
+    ENC_TRY(iamx_side_wait(), "encode");                                                                          \
+    ENC_TRY(iamx_use_stream(1), "encode");                                                                        \
+    e = iamx_encode_chunks(t, split, d_packed, pf[k], pf[(k) + 1], d_slots, d_meta, d_out, d_sizes);             \
+    iamx_use_stream(0);                                                                                           \
+    if (e) return iax_dev_error(e, "encode");                                                                     \
   } while (0)
-#define ENC_SIZES(k)                                                                                          \
-  ENC_TRY(iamx_d2h_async(h_sizes + pf[k], d_sizes + pf[k], (size_t)((pf[(k) + 1] - pf[k]) * 8), 2 * (k)), \
-          "download")
+  /* the sizes of panel k's chunks, after its encode on the side stream */
+#define ENC_SIZES(k)                                                                                            \
+  do {                                                                                                          \
+    ENC_TRY(iamx_use_stream(1), "download");                                                                    \
+    e = iamx_d2h_async(h_sizes + pf[k], d_sizes + pf[k], (size_t)((pf[(k) + 1] - pf[k]) * 8), 2 * (k));        \
+    iamx_use_stream(0);                                                                                         \
+    if (e) return iax_dev_error(e, "download");                                                                 \
+  } while (0)
   ENC_PRODUCE(0);
   ENC_SIZES(0);
   for (int k = 0; k < npanels; k++) {
