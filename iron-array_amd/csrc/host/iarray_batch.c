@@ -204,6 +204,7 @@ static ina_rc_t build_plan(batch_plan_t *p, int n, iarray_container_t **a, iarra
       tiles[3 * k + 2] = tl[k].tn;
     }
   }
+  iax_sort_lz4_jobs(lz4, il);
   if (INA_SUCCEED(rc)) rc = upload((void **)&p->d_lz4, lz4, sizeof(iamx_lz4_job_t) * (size_t)il);
   if (INA_SUCCEED(rc)) rc = upload((void **)&p->d_blocks, blk, sizeof(iamx_block_job_t) * (size_t)ib);
   if (INA_SUCCEED(rc)) rc = upload((void **)&p->d_prods, prods, sizeof(iamx_batch_prod_t) * (size_t)n);

@@ -443,8 +443,22 @@ static ina_rc_t block_streams(const uint8_t *ch, int32_t csize, uint64_t dev, co
   return INA_SUCCESS;
 }
 
+static int by_csize_desc(const void *x, const void *y) {
+  const iamx_lz4_job_t *a = (const iamx_lz4_job_t *)x, *b = (const iamx_lz4_job_t *)y;
+  if (a->csize != b->csize) return a->csize < b->csize ? 1 : -1;
+  return a->dst_off < b->dst_off ? -1 : a->dst_off > b->dst_off;
+}
+
+/* LZ4 jobs longest first: a launch holds more streams than the GPU has
+ * wavefront slots, and starting the long ones first shortens the tail (the
+ * block jobs refer to scratch offsets, not to job positions) */
+void iax_sort_lz4_jobs(iamx_lz4_job_t *jobs, int64_t n) {
+  if (n > 1) qsort(jobs, (size_t)n, sizeof(*jobs), by_csize_desc);
+}
+
 static ina_rc_t upload_plan(jobs_t *j, iax_plan_t *plan) {
   memset(plan, 0, sizeof(*plan));
+  iax_sort_lz4_jobs(j->lz4, j->n_lz4);
   plan->n_lz4 = (int)j->n_lz4;
   plan->n_blocks = (int)j->n_blk;
   plan->scratch_bytes = j->scratch;

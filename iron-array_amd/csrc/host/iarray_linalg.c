@@ -75,6 +75,7 @@ static ina_rc_t build_pair(iarray_container_t *a, iarray_container_t *b) {
     J.dst_off += b_off;
     blk[pa->n_blocks + i] = J;
   }
+  iax_sort_lz4_jobs(lz4, nl);
   ina_rc_t rc = INA_SUCCESS;
   g_pair.d_lz4 = (iamx_lz4_job_t *)iamx_malloc(sizeof(iamx_lz4_job_t) * (size_t)(nl ? nl : 1));
   g_pair.d_blocks = (iamx_block_job_t *)iamx_malloc(sizeof(iamx_block_job_t) * (size_t)(nb ? nb : 1));

@@ -98,6 +98,8 @@ double iax_now(void);
  * piece holding its bytes (up to task_end[task]) has landed, so the copy of
  * later pieces overlaps the work on earlier ones.  *host receives the
  * staging buffer (owned by the device runtime). */
+/* Order LZ4 decode jobs by compressed size, largest first. */
+void iax_sort_lz4_jobs(iamx_lz4_job_t *jobs, int64_t n);
 ina_rc_t iax_download_pipelined(const void *d_src, size_t total, const int64_t *task_end, int64_t ntasks,
                                 int nthreads, iax_task_fn fn, void *ctx, uint8_t **host);
 
