@@ -746,123 +746,151 @@ static int produce_panel(void *v, int k, int64_t t0, int64_t t1) {
   return iamx_pack_blocks_range(x->t, x->d_dense, c->shape[1], x->d_packed, t0, t1);
 }
 
-/* ---- C write stage with the LZ4 encode on the GPU ------------------------------------------------ */
+/* ---- C write stage shared between the host and the GPU encoder ---------------------------------- *
+ * The last `share` of the panels are LZ4-encoded and assembled into chunks on
+ * the GPU (on the side stream, overlapping the next panel's GEMM), the others
+ * on the host as before.  Every chunk lands in the pinned staging buffer at
+ * chunk * cap: packed bytes for a host panel, the finished chunk (its size in
+ * its header) for a GPU panel; all of it is queued up front, and the host
+ * threads take each chunk as soon as its panel has landed. */
 typedef struct {
   iarray_container_t *c;
-  const uint8_t *h;      /* pinned staging: chunk nc at nc * cap */
-  const int64_t *sizes;  /* chunk sizes (pinned) */
-  int64_t cap, c0;
-  int err;
-} gcopy_t;
+  const uint8_t *h;
+  int64_t cap;
+  const int64_t *first; /* first chunk of each panel, np + 1 entries */
+  const uint8_t *on_gpu;
+  int np, filtered;
+  atomic_int landed[IAMX_COPY_EVENTS];
+  atomic_int err;
+} hyb_t;
 
-static void take_chunk_task(void *v, int64_t i, int worker) {
-  (void)worker;
-  gcopy_t *x = (gcopy_t *)v;
-  iarray_container_t *c = x->c;
-  const int64_t nc = x->c0 + i;
-  const int64_t sz = x->sizes[nc];
-  free(c->chunks[nc]);
-  c->chunks[nc] = NULL;
-  uint8_t *p = sz > 0 && sz <= x->cap ? (uint8_t *)malloc((size_t)sz) : NULL;
-  if (!p) {
-    x->err = 1;
-    return;
-  }
-  memcpy(p, x->h + nc * x->cap, (size_t)sz);
-  c->chunks[nc] = p;
-  c->csizes[nc] = (int32_t)sz;
+static uint32_t rd32le(const uint8_t *p) {
+  uint32_t v;
+  memcpy(&v, p, 4);
+  return v;
 }
 
-/* Panel k: its rows of C, their packing and encoding are queued (produce),
- * the chunk sizes downloaded (event 2k), then each chunk's bytes (event
- * 2k+1); the host takes a panel's chunks while later panels compute. */
-static ina_rc_t write_gpu_encoded(iarray_container_t *c, const iamx_tiling_t *t, const void *d_dense,
-                                  const int64_t *pf, int npanels, iax_rows_fn rows, void *rctx) {
+static void hybrid_task(void *v, int64_t nc, int worker) {
+  (void)worker;
+  hyb_t *x = (hyb_t *)v;
+  int lo = 0, hi = x->np - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (x->first[mid] <= nc) lo = mid;
+    else hi = mid - 1;
+  }
+  if (!atomic_load(&x->landed[lo])) {
+    if (iamx_event_sync(lo) != 0) {
+      atomic_store(&x->err, 1);
+      return;
+    }
+    atomic_store(&x->landed[lo], 1);
+  }
+  iarray_container_t *c = x->c;
+  const uint8_t *src = x->h + nc * x->cap;
+  free(c->chunks[nc]);
+  c->chunks[nc] = NULL;
+  if (x->on_gpu[lo]) {
+    const int32_t sz = (int32_t)rd32le(src + 12); /* cbytes of the assembled chunk */
+    uint8_t *p = sz > 32 && sz <= x->cap ? (uint8_t *)malloc((size_t)sz) : NULL;
+    if (!p) {
+      atomic_store(&x->err, 1);
+      return;
+    }
+    memcpy(p, src, (size_t)sz);
+    c->chunks[nc] = p;
+    c->csizes[nc] = sz;
+  } else if (iax_encode_chunk(src, (int32_t)c->chunk_nbytes, c->blocksize, c->itemsize, &c->cfg, x->filtered,
+                              &c->chunks[nc], &c->csizes[nc]) != 0) {
+    atomic_store(&x->err, 1);
+  }
+}
+
+static ina_rc_t write_hybrid(iarray_container_t *c, const iamx_tiling_t *t, const void *d_dense, void *d_packed,
+                             const int64_t *pf, int npanels, iax_rows_fn rows, void *rctx, double share) {
   const int split = iax_split(&c->cfg, c->itemsize, c->blocksize);
   const int64_t cap = iamx_chunk_capacity(t, split);
-  if (cap <= 0) return iax_dev_error(IAMX_EINVAL, "chunk capacity");
-  const int64_t nstreams = split ? c->itemsize : 1;
-  const int64_t nst = c->nchunks * c->nblocks_ax[0] * c->nblocks_ax[1] * nstreams;
-  const int64_t scratch = iamx_encode_scratch_bytes(t, split);
-  if (scratch <= 0) return iax_dev_error(IAMX_EINVAL, "encode scratch");
-  void *d_packed = iamx_workspace(5, (size_t)(c->nchunks * c->chunk_nbytes));
-  void *d_slots = iamx_workspace(12, (size_t)scratch);
-  int32_t *d_meta = (int32_t *)iamx_workspace(13, (size_t)(nst * 8 + c->nchunks * 8 + 16));
-  uint8_t *d_out = (uint8_t *)iamx_workspace(14, (size_t)(c->nchunks * cap));
-  if (!d_packed || !d_slots || !d_meta || !d_out) return iax_dev_error(IAMX_ENOMEM, "encode buffers");
-  int64_t *d_sizes = (int64_t *)((uint8_t *)d_meta + (nst * 8 + 7) / 8 * 8);
-  uint8_t *h = (uint8_t *)iamx_host_buffer((size_t)(c->nchunks * cap + c->nchunks * 8));
+  if (cap < c->chunk_nbytes) return iax_dev_error(IAMX_EINVAL, "chunk capacity");
+  int ngpu = (int)(share * npanels + 0.5);
+  if (ngpu > npanels) ngpu = npanels;
+  uint8_t on_gpu[IAMX_COPY_EVENTS];
+  for (int k = 0; k < npanels; k++) on_gpu[k] = k >= npanels - ngpu; /* the last panels: a shorter tail */
+  void *d_scratch = NULL;
+  int32_t *d_meta = NULL;
+  uint8_t *d_out = NULL;
+  int64_t *d_sizes = NULL;
+  if (ngpu > 0) {
+    const int64_t nstreams = split ? c->itemsize : 1;
+    const int64_t nst = c->nchunks * c->nblocks_ax[0] * c->nblocks_ax[1] * nstreams;
+    const int64_t scratch = iamx_encode_scratch_bytes(t, split);
+    if (scratch <= 0) return iax_dev_error(IAMX_EINVAL, "encode scratch");
+    d_scratch = iamx_workspace(12, (size_t)scratch);
+    d_meta = (int32_t *)iamx_workspace(13, (size_t)(nst * 8 + c->nchunks * 8 + 16));
+    d_out = (uint8_t *)iamx_workspace(14, (size_t)(c->nchunks * cap));
+    if (!d_scratch || !d_meta || !d_out) return iax_dev_error(IAMX_ENOMEM, "encode buffers");
+    d_sizes = (int64_t *)((uint8_t *)d_meta + (nst * 8 + 7) / 8 * 8);
+  }
+  uint8_t *h = (uint8_t *)iamx_host_buffer((size_t)(c->nchunks * cap));
   if (!h) return iax_dev_error(IAMX_ENOMEM, "pinned staging buffer");
-  int64_t *h_sizes = (int64_t *)(h + c->nchunks * cap);
   panel_ctx_t px = {c, t, d_dense, d_packed, rows, rctx};
-  gcopy_t gx = {c, h, h_sizes, cap, 0, 0};
-  const int nt = iax_nthreads(&c->cfg);
-  int e = 0;
-#define ENC_TRY(call, what)               \
-  do {                                    \
-    e = (call);                           \
-    if (e) return iax_dev_error(e, what); \
-  } while (0)
-  /* queue panel k: compute and pack on the library stream, encode on the
-   * side stream once the pack is done (it overlaps panel k+1's GEMM) */
-#define ENC_PRODUCE(k)                                                                                            \
-  do {                                                                                                            \
-    ENC_TRY(produce_panel(&px, (k), pf[k], pf[(k) + 1]), "panel");                                                \
-    ENC_TRY(iamx_side_wait(), "encode");                                                                          \
-    ENC_TRY(iamx_use_stream(1), "encode");                                                                        \
-    e = iamx_encode_chunks(t, split, d_packed, pf[k], pf[(k) + 1], d_slots, d_meta, d_out, d_sizes);             \
-    iamx_use_stream(0);                                                                                           \
-    if (e) return iax_dev_error(e, "encode");                                                                     \
-  } while (0)
-  /* the sizes of panel k's chunks, after its encode on the side stream */
-#define ENC_SIZES(k)                                                                                            \
-  do {                                                                                                          \
-    ENC_TRY(iamx_use_stream(1), "download");                                                                    \
-    e = iamx_d2h_async(h_sizes + pf[k], d_sizes + pf[k], (size_t)((pf[(k) + 1] - pf[k]) * 8), 2 * (k));        \
-    iamx_use_stream(0);                                                                                         \
-    if (e) return iax_dev_error(e, "download");                                                                 \
-  } while (0)
-  ENC_PRODUCE(0);
-  ENC_SIZES(0);
   for (int k = 0; k < npanels; k++) {
-    if (k + 1 < npanels) ENC_PRODUCE(k + 1);
-    ENC_TRY(iamx_event_sync(2 * k), "download");
-    for (int64_t nc = pf[k]; nc < pf[k + 1]; nc++) {
-      const int64_t sz = h_sizes[nc];
-      if (sz <= 0 || sz > cap) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "encoded chunk size");
-      ENC_TRY(iamx_d2h_chain(h + nc * cap, d_out + nc * cap, (size_t)sz, 2 * k + 1), "download");
-    }
-    if (k + 1 < npanels) ENC_SIZES(k + 1);
-    if (k > 0) {
-      ENC_TRY(iamx_event_sync(2 * k - 1), "download");
-      gx.c0 = pf[k - 1];
-      iax_parallel_for(nt, pf[k] - pf[k - 1], take_chunk_task, &gx);
+    int e = produce_panel(&px, k, pf[k], pf[k + 1]);
+    if (e) return iax_dev_error(e, "panel");
+    if (on_gpu[k]) {
+      if ((e = iamx_side_wait()) != 0 || (e = iamx_use_stream(1)) != 0) return iax_dev_error(e, "encode");
+      e = iamx_encode_chunks(t, split, d_packed, pf[k], pf[k + 1], d_scratch, d_meta, d_out, d_sizes);
+      for (int64_t nc = pf[k]; nc < pf[k + 1] && !e; nc++)
+        e = nc == pf[k] ? iamx_d2h_async(h + nc * cap, d_out + nc * cap, (size_t)cap, k)
+                        : iamx_d2h_chain(h + nc * cap, d_out + nc * cap, (size_t)cap, k);
+      iamx_use_stream(0);
+      if (e) return iax_dev_error(e, "encode");
+    } else {
+      const uint8_t *pk = (const uint8_t *)d_packed;
+      for (int64_t nc = pf[k]; nc < pf[k + 1] && !e; nc++)
+        e = nc == pf[k] ? iamx_d2h_async(h + nc * cap, pk + nc * c->chunk_nbytes, (size_t)c->chunk_nbytes, k)
+                        : iamx_d2h_chain(h + nc * cap, pk + nc * c->chunk_nbytes, (size_t)c->chunk_nbytes, k);
+      if (e) return iax_dev_error(e, "download");
     }
   }
-  ENC_TRY(iamx_event_sync(2 * npanels - 1), "download");
-  gx.c0 = pf[npanels - 1];
-  iax_parallel_for(nt, pf[npanels] - pf[npanels - 1], take_chunk_task, &gx);
-#undef ENC_SIZES
-#undef ENC_PRODUCE
-#undef ENC_TRY
-  if (gx.err) return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+  hyb_t x;
+  x.c = c;
+  x.h = h;
+  x.cap = cap;
+  x.first = pf;
+  x.on_gpu = on_gpu;
+  x.np = npanels;
+  x.filtered = c->cfg.compression_level > 0;
+  for (int k = 0; k < IAMX_COPY_EVENTS; k++) atomic_init(&x.landed[k], 0);
+  atomic_init(&x.err, 0);
+  iax_parallel_for(iax_nthreads(&c->cfg), c->nchunks, hybrid_task, &x);
+  if (atomic_load(&x.err)) {
+    for (int k = 0; k < npanels; k++) (void)iamx_event_sync(k);
+    return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "chunk write failed");
+  }
   return INA_SUCCESS;
 }
 
-/* The GPU encoder (IARRAY_GPU_ENCODE=1) covers LZ4 chunks of whole blocks
- * with up to 8 streams.  It is not the default: it adds its own time to the
- * GPU's (0.65 ms per 1024-row panel of an 8192^2 float32 product, 1.0 ms per
- * 1500-row panel of a 6000^2 float64 one), while the host encode of a panel
- * runs during the next panels' GEMMs; end to end the host encode measured
- * 21.8 ms against 23.5 ms (split-bf16) and 27.3 against 33.1 ms (plain) on
- * the 8192^2 product. */
-static int gpu_encode_ok(const iarray_container_t *c) {
+/* Share of the C panels encoded on the GPU.  IARRAY_GPU_ENCODE (0..1) sets it;
+ * unset, it is 1/4 for outputs of at least 128 MiB in 4 or more panels and 0
+ * otherwise.  The GPU encoder covers LZ4 chunks of whole blocks with up to 8
+ * streams.  Measured with 16 host threads: the GPU encode of a 1024-row panel
+ * of an 8192^2 float32 product takes 0.65 ms against about 2 ms of host
+ * thread time, but it adds to the GPU's own work.  A quarter of the panels
+ * on the GPU took 8192^2 float32 products from 20.5 to 18.7 ms (split-bf16)
+ * and 6144^2 float64 ones from 24.4 to 21.7 ms; half or more gave less, and a
+ * 2000^2 float64 product (32 MB, 2 panels) lost 10%. */
+static double gpu_encode_share(const iarray_container_t *c, int npanels) {
+  const int ok = c->cfg.compression_level > 0 &&
+                 (c->cfg.compression_codec == IARRAY_COMPRESSION_LZ4 ||
+                  c->cfg.compression_codec == IARRAY_COMPRESSION_LZ4HC) &&
+                 c->itemsize <= IAMX_MAX_STREAMS && c->nchunks > 0 && c->chunk_nbytes % c->blocksize == 0 &&
+                 c->chunk_nbytes <= INT32_MAX / 2;
+  if (!ok) return 0.0;
   const char *env = getenv("IARRAY_GPU_ENCODE");
-  if (!env || atoi(env) != 1) return 0;
-  return c->cfg.compression_level > 0 &&
-         (c->cfg.compression_codec == IARRAY_COMPRESSION_LZ4 || c->cfg.compression_codec == IARRAY_COMPRESSION_LZ4HC) &&
-         c->itemsize <= IAMX_MAX_STREAMS && c->nchunks > 0 && c->chunk_nbytes % c->blocksize == 0 &&
-         c->chunk_nbytes <= INT32_MAX / 2;
+  if (!env) return (c->nchunks * c->chunk_nbytes >= ((int64_t)128 << 20) && npanels >= 4) ? 0.25 : 0.0;
+  double v = atof(env);
+  if (!(v > 0.0)) return 0.0;
+  return v > 1.0 ? 1.0 : v;
 }
 
 ina_rc_t iax_write_from_dense_rows(iarray_container_t *c, const void *d_dense, int64_t panel_rows,
@@ -892,9 +920,10 @@ ina_rc_t iax_write_from_dense_rows(iarray_container_t *c, const void *d_dense, i
     if (r > ncr) r = ncr;
     pf[k] = r * c->nchunks_ax[1];
   }
-  if (gpu_encode_ok(c)) {
+  const double share = gpu_encode_share(c, npanels);
+  if (share > 0.0) {
     const double t0 = iax_now();
-    ina_rc_t rc = write_gpu_encoded(c, &t, d_dense, pf, npanels, rows, rctx);
+    ina_rc_t rc = write_hybrid(c, &t, d_dense, d_packed, pf, npanels, rows, rctx, share);
     iax_last_encode_s = iax_now() - t0;
     free(ends);
     free(pf);

@@ -159,3 +159,29 @@ def test_gpu_encoded_chunks(monkeypatch, orc, case):
     # deterministic
     z2 = _product(monkeypatch, ctx, a, b, acs, abs_, bcs, bbs, zc, zb)
     assert z2.chunks() == chunks
+
+
+@pytest.mark.parametrize("share", ["0.5", "0.25"])
+def test_mixed_host_and_gpu_panels(monkeypatch, orc, share):
+    """Some panels encoded on the GPU, the others on the host, in one product:
+    every chunk valid, values equal to the all-host product."""
+    print(f"case: gpu encode share {share}", flush=True)
+    ctx = ia.Context(clevel=5, filters=ia.COMP_SHUFFLE)
+    rng = np.random.default_rng(5)
+    a = rng.standard_normal((520, 200)).astype(np.float32)
+    b = rng.standard_normal((200, 300)).astype(np.float32)
+    ca = ia.Container.from_numpy(ctx, a, (128, 100), (64, 50))
+    cb = ia.Container.from_numpy(ctx, b, (100, 150), (50, 75))
+    monkeypatch.setenv("IARRAY_GEMM_PANEL_ROWS", "1")  # one chunk row a panel: 5 panels
+    monkeypatch.setenv("IARRAY_GPU_ENCODE", "0")
+    ref = ia.matmul(ca, cb, (128, 150), (64, 50)).to_numpy()
+    monkeypatch.setenv("IARRAY_GPU_ENCODE", share)
+    z = ia.matmul(ca, cb, (128, 150), (64, 50))
+    got = z.to_numpy()
+    assert got.tobytes() == ref.tobytes()
+    chunks = z.chunks()
+    nbytes = 128 * 150 * 4
+    for ch in chunks:
+        _check_chunk(ch, nbytes)
+    dense = orc.Array(chunks, z.shape, (128, 150), (64, 50), 4).to_dense()
+    assert dense.tobytes() == got.tobytes()
