@@ -58,15 +58,17 @@ __device__ __forceinline__ uint32_t rdlane(uint32_t v, int32_t l) {
 //           back when off >= 64), so a step never reads what it writes.
 //           Farther matches read the flushed stream back from global memory,
 //           after a workgroup fence if it was stored since the last fence.
-//   fast path  a token below 15 (no literals, match length 4..18) whose match
-//           does not overlap itself is one ring step with all 64 lanes.
+//   fast path  the headers of the next 64 input positions are decoded one
+//           per lane; the walk over them reads each sequence's fields with
+//           readlane and copies its literals and match with all 64 lanes.
 // Ring steps store all 64 lanes: bytes past the sequence land on positions
 // that are written again before anything reads them, and the slots they take
 // belong to positions more than RING-64 bytes back, which no later match
-// reads and which were flushed (at most RING/2 + 64 bytes are ever pending).
+// reads and which were flushed (at most RING/2 + 400 bytes are ever pending).
+// With a 4 KiB ring the kernel fits 54 VGPRs: 8 wavefronts per SIMD, 32 per CU.
 template <int RING>
-__global__ __launch_bounds__(64) void k_lz4_streams(const iamx_lz4_job_t *__restrict__ jobs, int njobs,
-                                                    uint8_t *__restrict__ scratch, int *status) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RING <= 4096 ? 8 : 1, 8))) void k_lz4_streams(
+    const iamx_lz4_job_t *__restrict__ jobs, int njobs, uint8_t *__restrict__ scratch, int *status) {
   __shared__ __align__(16) uint8_t ring[RING];
   constexpr int32_t RM = RING - 1;
   constexpr int32_t NEAR = RING - 64;
@@ -132,79 +134,100 @@ __global__ __launch_bounds__(64) void k_lz4_streams(const iamx_lz4_job_t *__rest
   int bad = 0;
   for (;;) {
     refill(ip);
-    // Fast section.  For the 64 input positions from the parse position on,
-    // lane l decodes the header of the sequence that would start at position
-    // ip + l (the window bytes are gathered with ds_bpermute) into one word:
-    // offset (bits 0-15), literal count (16-19), match length (20-28), a
-    // length continuation byte (29) and a fast flag (31): 0..14 literals, a
-    // match of 4..273 bytes (at most one continuation byte), the header inside
-    // the stream and starting at most 236 bytes into w0 (so its literals lie
-    // in w0).  The parse then costs one readlane per sequence.  It runs while
-    // the output position stays 288 bytes short of the slot end and below the
-    // next flush; each test is a difference whose sign bit flags failure.
-    if (ip - wb <= 236) {
-      const int32_t olim = min(oend - 288, flush_at - 1);
-      int32_t qs = ip - wb;  // window position of lane 0's candidate
-      uint32_t info = 0;
-      auto decode_headers = [&]() {
-        const int32_t r = qs + lane;  // <= 299
+    // Fast section (refill() above keeps ip - wb < 256).  Lane l decodes the
+    // header of the sequence that would start at window position qs + l (the
+    // window bytes are gathered with ds_bpermute): literal count, offset,
+    // match length, and a control word: the header length plus literals
+    // (`adv`) when the sequence can be decoded here, bit 31 when the walk
+    // stops before it, bit 30 when the general path below must decode it.
+    // Decoded here: up to 59 literals (read from w0/w1) and a match of
+    // 4..273 bytes (at most one length byte each), an offset within the
+    // output so far, the sequence inside the stream and at most 63 bytes
+    // long, and the next header still among the 64 positions.  The walk
+    // then costs one readlane per field, and it stops once the output
+    // position passes `olim`: 333 bytes (the most one sequence here writes)
+    // short of the slot end, and below the next flush.
+    {
+      const int32_t olim = min(oend - 333, flush_at - 1);
+      while (op <= olim) {
+        const int32_t qs = ip - wb;
+        const int32_t r = qs + lane;  // <= 318
         auto gather = [&](int32_t d) -> uint32_t {  // window dword d < 128
           const uint32_t a0 = (uint32_t)__shfl((int)w0, d & 63, 64);
           const uint32_t a1 = (uint32_t)__shfl((int)w1, d & 63, 64);
           return d < 64 ? a0 : a1;
         };
         const uint32_t h = __builtin_amdgcn_alignbyte(gather((r >> 2) + 1), gather(r >> 2), (uint32_t)(r & 3));
-        const uint32_t hl = (h >> 4) & 15u, hm = h & 15u;
-        const int32_t r2 = r + 1 + (int32_t)hl;  // offset field, <= 315
+        const uint32_t hl = (h >> 4) & 15u, hm = h & 15u, lb = (h >> 8) & 255u;
+        const uint32_t lext = hl == 15 ? 1u : 0u;  // one literal length byte
+        const uint32_t hlit = hl + (lext ? lb : 0u);
+        const uint32_t hlsrc = 1 + lext;  // literals start this far after the token
+        // offset field: <= 318 + 2 + 58 when the sequence is decoded here, and
+        // only read through lane indexes taken mod 64 otherwise
+        const int32_t r2 = r + (int32_t)(hlsrc + hlit);
         const uint32_t h2 = __builtin_amdgcn_alignbyte(gather((r2 >> 2) + 1), gather(r2 >> 2), (uint32_t)(r2 & 3));
         const uint32_t hoff = h2 & 0xffffu, he = (h2 >> 16) & 255u;
         const uint32_t ext = hm == 15 ? 1u : 0u;
         const uint32_t hml = hm + 4 + (ext ? he : 0u);
-        const int32_t hdr = 3 + (int32_t)hl + (int32_t)ext;
-        const bool hfast = hl < 15 && !(ext && he == 255) && hoff >= 1 && r <= 236 && hdr <= iend - wb - r;
-        info = hoff | (hl << 16) | (hml << 20) | (ext << 29) | (hfast ? 0x80000000u : 0u);
-      };
-      for (;;) {
-        decode_headers();
-        // one exit: the next header is read at the bottom of the body
-        int32_t l = ip - wb - qs;
-        uint32_t inf = rdlane(info, l & 63);
-        int32_t lit = (int32_t)((inf >> 16) & 15u);
-        int32_t off = (int32_t)(inf & 0xffffu);
-        while (((int32_t)~inf | (63 - l) | (min(op + lit, NEAR) - off) | (olim - op)) >= 0) {
-          const int32_t ml = (int32_t)((inf >> 20) & 511u);
+        const int32_t adv = 3 + (int32_t)(lext + hlit + ext);
+        const bool ok = !(lext && lb == 255) && !(ext && he == 255) && hoff >= 1 && (int32_t)hoff <= op && adv <= 63 &&
+                        adv <= iend - wb - r;
+        const uint32_t ctl = ok ? (lane + adv <= 63 ? (uint32_t)adv : 0x80000000u) : 0xC0000000u;
+        // 1/off rounded up by 16 ulps: floor((lane + off) * rcp) is exact for
+        // lane < 64, off < 64
+        const uint32_t hrcp = __float_as_uint(__builtin_amdgcn_rcpf((float)hoff)) + 16u;
+        int32_t l = 0;
+        uint32_t c = rdlane(ctl, 0);
+        while ((int32_t)(c | (uint32_t)(olim - op)) >= 0) {
+          const int32_t lit = (int32_t)rdlane(hlit, l);
+          const int32_t off = (int32_t)rdlane(hoff, l);
+          const int32_t ml = (int32_t)rdlane(hml, l);
           const int32_t o2 = op + lit;
           if (lit) {
-            const int32_t ql = ip - wb + 1 + lane;  // < 256 + 64
-            const uint32_t a = (uint32_t)__shfl((int)w0, (ql >> 2) & 63, 64);
-            ring[(op + lane) & RM] = (uint8_t)(a >> ((ql & 3) << 3));
+            const int32_t q = r + l + (int32_t)rdlane(hlsrc, l);  // window position of literal byte `lane`, <= 383
+            const int32_t d = (q >> 2) & 63;
+            const uint32_t a0 = (uint32_t)__shfl((int)w0, d, 64), a1 = (uint32_t)__shfl((int)w1, d, 64);
+            ring[(op + lane) & RM] = (uint8_t)((q < 256 ? a0 : a1) >> ((q & 3) << 3));
             __builtin_amdgcn_wave_barrier();
           }
-          int32_t back = off;
-          if (off < 64) {
-            const int32_t num = lane + off;
-            int32_t qq = (int32_t)((float)num * __builtin_amdgcn_rcpf((float)off));
-            const int32_t rr = num - qq * off;
-            qq += (rr >= off) - (rr < 0);
-            back = off * qq;
+          if (off > NEAR) {
+            // every source byte lies more than RING - 64 bytes back, and at
+            // most RING/2 + 395 bytes are pending, so all of them were flushed
+            const int32_t gend = o2 + ((ml + 63) & ~63) - off;
+            if (gend > fenced) {
+              __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+              __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+              fenced = flushed;
+            }
+            int32_t k = 0;
+            do {
+              const uint8_t v = dst[o2 + k + lane - off];
+              ring[(o2 + k + lane) & RM] = v;
+              __builtin_amdgcn_wave_barrier();
+              k += 64;
+            } while (k < ml);
+          } else {
+            int32_t back = off;  // off * ceil((lane + 1) / off)
+            if (off < 64) {
+              const float rc = __uint_as_float(rdlane(hrcp, l));
+              back = (int32_t)__umul24((uint32_t)((float)(lane + off) * rc), (uint32_t)off);
+            }
+            int32_t k = 0;
+            do {
+              const uint8_t v = ring[(o2 + k + lane - back) & RM];
+              __builtin_amdgcn_wave_barrier();
+              ring[(o2 + k + lane) & RM] = v;
+              __builtin_amdgcn_wave_barrier();
+              k += 64;
+            } while (k < ml);
           }
-          int32_t k = 0;
-          do {
-            const uint8_t v = ring[(o2 + k + lane - back) & RM];
-            __builtin_amdgcn_wave_barrier();
-            ring[(o2 + k + lane) & RM] = v;
-            __builtin_amdgcn_wave_barrier();
-            k += 64;
-          } while (k < ml);
-          ip += 3 + lit + (int32_t)((inf >> 29) & 1u);
           op = o2 + ml;
-          l = ip - wb - qs;
-          inf = rdlane(info, l & 63);
-          lit = (int32_t)((inf >> 16) & 15u);
-          off = (int32_t)(inf & 0xffffu);
+          l += (int32_t)c;
+          c = rdlane(ctl, l);
         }
-        if (l < 64 || ip - wb > 236 || op > olim) break;
-        qs = ip - wb;  // the next 64 positions
+        ip = wb + qs + l;
+        if (op > olim || (c & 0x40000000u)) break;  // a flush, or a sequence for the general path
+        refill(ip);  // the walk left the 64 positions: the next ones
       }
     }
     if (op >= flush_at) {
@@ -338,14 +361,34 @@ __global__ __launch_bounds__(64) void k_lz4_streams(const iamx_lz4_job_t *__rest
       back = off * qq;
     }
     if (off <= NEAR) {
-      for (int32_t k = 0; k < ml; k += 64) {
+      auto step = [&](int32_t k) {
         const uint8_t v = ring[(op + k + lane - back) & RM];
         __builtin_amdgcn_wave_barrier();
         ring[(op + k + lane) & RM] = v;
         __builtin_amdgcn_wave_barrier();
         const int32_t cur = op + min(ml, k + 64);
         if (cur >= flush_at) flush(cur);
+      };
+      // The match is periodic with period off, so position p may also copy
+      // from p - backU, the first multiple of off at least 256 back (at most
+      // 511 or off, both within the ring).  Once 256 bytes of that history
+      // exist, four 64-byte steps read nothing the same four steps write and
+      // go out together.
+      constexpr int32_t U = 4;
+      const int32_t backU = off >= 64 * U ? off : off * ((64 * U + off - 1) / off);
+      int32_t k = 0;
+      for (; k < ml && k < backU - off; k += 64) step(k);
+      for (; k + 64 * U <= ml; k += 64 * U) {
+        uint8_t v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) v[u] = ring[(op + k + 64 * u + lane - backU) & RM];
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int u = 0; u < U; u++) ring[(op + k + 64 * u + lane) & RM] = v[u];
+        __builtin_amdgcn_wave_barrier();
+        if (op + k + 64 * U >= flush_at) flush(op + k + 64 * U);
       }
+      for (; k < ml; k += 64) step(k);
     } else {
       for (int32_t k = 0; k < ml; k += 64) {
         // positions at or after `lo` are still in the ring; older ones were
@@ -577,16 +620,16 @@ namespace iamx {
 static int lz4_ring_kib() {
   static const int kib = [] {
     const char *e = getenv("IAMX_LZ4_RING_KIB");
-    const int v = e ? atoi(e) : 8;
-    return (v == 4 || v == 16) ? v : 8;
+    const int v = e ? atoi(e) : 4;
+    return (v == 8 || v == 16) ? v : 4;
   }();
   return kib;
 }
 
 int launch_lz4(const char *name, const iamx_lz4_job_t *d_lz4, int n_lz4, void *d_scratch) {
   switch (lz4_ring_kib()) {
-    case 4:
-      IAMX_LAUNCH(name, k_lz4_streams<4096>, dim3(n_lz4), dim3(64), 0, d_lz4, n_lz4, (uint8_t *)d_scratch,
+    case 8:
+      IAMX_LAUNCH(name, k_lz4_streams<8192>, dim3(n_lz4), dim3(64), 0, d_lz4, n_lz4, (uint8_t *)d_scratch,
                   status_word());
       break;
     case 16:
@@ -594,7 +637,7 @@ int launch_lz4(const char *name, const iamx_lz4_job_t *d_lz4, int n_lz4, void *d
                   status_word());
       break;
     default:
-      IAMX_LAUNCH(name, k_lz4_streams<8192>, dim3(n_lz4), dim3(64), 0, d_lz4, n_lz4, (uint8_t *)d_scratch,
+      IAMX_LAUNCH(name, k_lz4_streams<4096>, dim3(n_lz4), dim3(64), 0, d_lz4, n_lz4, (uint8_t *)d_scratch,
                   status_word());
       break;
   }

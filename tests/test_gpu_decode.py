@@ -15,6 +15,7 @@ import hashlib
 import json
 import os
 import struct
+import zlib
 
 import numpy as np
 import pytest
@@ -190,3 +191,100 @@ def test_corrupted_stream_reports_error(ctx):
     assert e.value.rc == ia.ERR_BLOSC_FAILED
     # the library keeps working afterwards
     assert ia.decode_chunks_flat(ctx, c.chunks(), x.nbytes) == x.tobytes()
+
+
+def _lenbytes(v):
+    """LZ4 length continuation bytes for a 4-bit field that holds 15."""
+    v -= 15
+    b = bytearray()
+    while v >= 255:
+        b.append(255)
+        v -= 255
+    b.append(v)
+    return b
+
+
+def _craft_lz4(rng, n, pick):
+    """An LZ4 block of n decoded bytes whose sequences come from pick(rng) ->
+    (literals, offset, match length), kept within the format's end rules.
+    Returns (block bytes, decoded bytes)."""
+    out = bytearray()
+    s = bytearray()
+    misses = 0
+    while misses < 50:
+        lit, off, ml = pick(rng)
+        if not out and lit == 0:
+            lit = 1
+        if len(out) + lit + ml > n - 12:
+            misses += 1
+            continue
+        off = max(1, min(off, len(out) + lit, 65535))
+        s.append((min(lit, 15) << 4) | min(ml - 4, 15))
+        if lit >= 15:
+            s += _lenbytes(lit)
+        lits = rng.integers(0, 256, lit, dtype=np.uint8).tobytes()
+        s += lits
+        out += lits
+        s += struct.pack("<H", off)
+        if ml - 4 >= 15:
+            s += _lenbytes(ml - 4)
+        for _ in range(ml):
+            out.append(out[-off])
+    lit = n - len(out)
+    s.append(min(lit, 15) << 4)
+    if lit >= 15:
+        s += _lenbytes(lit)
+    lits = rng.integers(0, 256, lit, dtype=np.uint8).tobytes()
+    s += lits
+    out += lits
+    return bytes(s), bytes(out)
+
+
+def _one_stream_chunk(ctx, stream, n):
+    """A Blosc2 chunk of one uint8 block whose single stream is `stream`."""
+    x = (np.arange(n) % 7).astype(np.uint8)
+    ch = bytearray(ia.Container.from_numpy(ctx, x, (n,), (n,)).chunks()[0][:32])
+    assert ch[3] == 1 and struct.unpack("<i", ch[8:12])[0] == n
+    body = struct.pack("<i", 36) + struct.pack("<i", len(stream)) + stream
+    ch[12:16] = struct.pack("<i", 32 + len(body))
+    return bytes(ch) + body
+
+
+def _choice(rng, ranges):
+    lo, hi = ranges[rng.integers(0, len(ranges))]
+    return int(rng.integers(lo, hi + 1))
+
+
+# (literals, offset, match length) ranges per plan; every path of the LZ4
+# kernel: the per-lane header walk (literals up to 59 with one length byte,
+# matches up to 273 with one, offsets within and beyond the LDS history ring,
+# periodic matches with offsets below 64), and the general path (longer
+# literal runs copied global to global, matches with two or more length bytes
+# copied four 64-byte steps at a time, offsets beyond the ring read back from
+# global memory).
+LZ4_PLANS = {
+    "short": ([(0, 0), (1, 14)], [(1, 63), (64, 4000)], [(4, 18)]),
+    "literal-byte": ([(15, 59), (0, 3)], [(1, 3000)], [(4, 40)]),
+    "long-literals": ([(60, 300), (300, 3000), (0, 5)], [(1, 3000)], [(4, 30)]),
+    "long-matches": ([(0, 3)], [(1, 8), (9, 63), (64, 255), (256, 300), (1000, 4000)], [(274, 6000), (19, 273)]),
+    "far": ([(0, 20)], [(4033, 8127), (8128, 65535), (100, 200)], [(4, 300)]),
+    "far-long": ([(0, 3)], [(4033, 65535)], [(274, 2000)]),
+    "mixed": ([(0, 0), (1, 14), (15, 59), (60, 700)], [(1, 7), (8, 63), (64, 4031), (4032, 65535)],
+              [(4, 18), (19, 273), (274, 3000)]),
+}
+
+
+@pytest.mark.parametrize("n", [200_000, 70_000, 3000])
+@pytest.mark.parametrize("plan", sorted(LZ4_PLANS))
+def test_crafted_lz4_streams(ctx, orc, plan, n):
+    print(f"case: crafted lz4 {plan} n={n}", flush=True)
+    lits, offs, mls = LZ4_PLANS[plan]
+    rng = np.random.default_rng(zlib.crc32(plan.encode()) + n)
+    stream, want = _craft_lz4(rng, n, lambda r: (_choice(r, lits), _choice(r, offs), _choice(r, mls)))
+    assert len(stream) < n
+    assert orc.lz4_decode(stream, n) == want  # the generator itself, against the oracle decoder
+    c = ia.Context(clevel=5, filters=0)
+    chunk = _one_stream_chunk(c, stream, n)
+    assert orc.decompress_chunk(chunk, n) == want
+    got = ia.decode_chunks_flat(c, [chunk], n)
+    assert got == want
