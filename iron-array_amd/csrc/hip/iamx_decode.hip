@@ -131,6 +131,30 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RING <= 4096
     flush_at = flushed + FLUSH_AT;
   };
 
+  // Adds the length continuation bytes from ip on (255s, then one below 255)
+  // to v, 64 input bytes a step (a ballot finds the last one); false when the
+  // stream ends first or the length exceeds the block.
+  auto read_len = [&](int32_t &v) -> bool {
+    for (;;) {
+      refill(ip);
+      const int32_t q = ip - wb + lane;  // < 320
+      const int32_t d = (q >> 2) & 63;
+      const uint32_t a0 = (uint32_t)__shfl((int)w0, d, 64), a1 = (uint32_t)__shfl((int)w1, d, 64);
+      const uint32_t b = ((q < 256 ? a0 : a1) >> ((q & 3) << 3)) & 255u;
+      const uint64_t stop = __ballot(ip + lane < iend && b != 255u);
+      if (stop) {
+        const int32_t t = (int32_t)__builtin_ctzll(stop);
+        v += 255 * t + (int32_t)rdlane(b, t);
+        ip += t + 1;
+        return true;
+      }
+      if (iend - ip < 64) return false;
+      v += 255 * 64;
+      ip += 64;
+      if (v > oend) return false;
+    }
+  };
+
   int bad = 0;
   for (;;) {
     refill(ip);
@@ -170,8 +194,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RING <= 4096
         const uint32_t ext = hm == 15 ? 1u : 0u;
         const uint32_t hml = hm + 4 + (ext ? he : 0u);
         const int32_t adv = 3 + (int32_t)(lext + hlit + ext);
-        const bool ok = !(lext && lb == 255) && !(ext && he == 255) && hoff >= 1 && (int32_t)hoff <= op && adv <= 63 &&
-                        adv <= iend - wb - r;
+        // (bitwise: one vector compare chain, no branches)
+        const bool ok = (!(lext & (lb == 255)) & !(ext & (he == 255)) & (hoff >= 1) & ((int32_t)hoff <= op) &
+                         (adv <= 63) & (adv <= iend - wb - r)) != 0;
         const uint32_t ctl = ok ? (lane + adv <= 63 ? (uint32_t)adv : 0x80000000u) : 0xC0000000u;
         // 1/off rounded up by 16 ulps: floor((lane + off) * rcp) is exact for
         // lane < 64, off < 64
@@ -190,6 +215,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RING <= 4096
             ring[(op + lane) & RM] = (uint8_t)((q < 256 ? a0 : a1) >> ((q & 3) << 3));
             __builtin_amdgcn_wave_barrier();
           }
+          // the first 64-byte step is peeled: most matches are one step
           if (off > NEAR) {
             // every source byte lies more than RING - 64 bytes back, and at
             // most RING/2 + 395 bytes are pending, so all of them were flushed
@@ -199,27 +225,31 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RING <= 4096
               __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
               fenced = flushed;
             }
-            int32_t k = 0;
-            do {
-              const uint8_t v = dst[o2 + k + lane - off];
-              ring[(o2 + k + lane) & RM] = v;
+            const int32_t d0 = o2 + lane;
+            ring[d0 & RM] = dst[d0 - off];
+            __builtin_amdgcn_wave_barrier();
+            for (int32_t k = 64; k < ml; k += 64) {
+              ring[(d0 + k) & RM] = dst[d0 + k - off];
               __builtin_amdgcn_wave_barrier();
-              k += 64;
-            } while (k < ml);
-          } else {
+            }
+          }
+          if (off <= NEAR) {
             int32_t back = off;  // off * ceil((lane + 1) / off)
             if (off < 64) {
               const float rc = __uint_as_float(rdlane(hrcp, l));
               back = (int32_t)__umul24((uint32_t)((float)(lane + off) * rc), (uint32_t)off);
             }
-            int32_t k = 0;
-            do {
-              const uint8_t v = ring[(o2 + k + lane - back) & RM];
+            const int32_t d0 = o2 + lane, s0 = d0 - back;
+            const uint8_t v = ring[s0 & RM];
+            __builtin_amdgcn_wave_barrier();
+            ring[d0 & RM] = v;
+            __builtin_amdgcn_wave_barrier();
+            for (int32_t k = 64; k < ml; k += 64) {
+              const uint8_t v2 = ring[(s0 + k) & RM];
               __builtin_amdgcn_wave_barrier();
-              ring[(o2 + k + lane) & RM] = v;
+              ring[(d0 + k) & RM] = v2;
               __builtin_amdgcn_wave_barrier();
-              k += 64;
-            } while (k < ml);
+            }
           }
           op = o2 + ml;
           l += (int32_t)c;
@@ -239,17 +269,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RING <= 4096
     const uint32_t token = rbyte(ip);
     ++ip;
     int32_t lit = (int32_t)(token >> 4);
-    if (lit == 15) {
-      uint32_t b;
-      do {
-        if (ip >= iend) { bad = 2; break; }
-        refill(ip);
-        b = rbyte(ip);
-        ++ip;
-        lit += (int32_t)b;
-      } while (b == 255);
-      if (bad) break;
-    }
+    if (lit == 15 && !read_len(lit)) { bad = 2; break; }
     if (lit > iend - ip || lit > oend - op) { bad = 3; break; }
     if (lit > 0) {
       if (lit <= 64) {
@@ -338,17 +358,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RING <= 4096
     const int32_t off = (int32_t)(rbyte(ip) | (rbyte(ip + 1) << 8));
     ip += 2;
     int32_t ml = (int32_t)(token & 15u);
-    if (ml == 15) {
-      uint32_t b;
-      do {
-        if (ip >= iend) { bad = 5; break; }
-        refill(ip);
-        b = rbyte(ip);
-        ++ip;
-        ml += (int32_t)b;
-      } while (b == 255);
-      if (bad) break;
-    }
+    if (ml == 15 && !read_len(ml)) { bad = 5; break; }
     ml += 4;
     if (off == 0 || off > op || ml > oend - op) { bad = 6; break; }
     // back distance of this lane's source: off * ceil((lane + 1) / off) for off < 64

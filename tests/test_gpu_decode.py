@@ -269,6 +269,7 @@ LZ4_PLANS = {
     "long-matches": ([(0, 3)], [(1, 8), (9, 63), (64, 255), (256, 300), (1000, 4000)], [(274, 6000), (19, 273)]),
     "far": ([(0, 20)], [(4033, 8127), (8128, 65535), (100, 200)], [(4, 300)]),
     "far-long": ([(0, 3)], [(4033, 65535)], [(274, 2000)]),
+    "huge-lengths": ([(0, 3), (17000, 20000)], [(1, 300), (5000, 9000)], [(20000, 60000), (4, 30)]),
     "mixed": ([(0, 0), (1, 14), (15, 59), (60, 700)], [(1, 7), (8, 63), (64, 4031), (4032, 65535)],
               [(4, 18), (19, 273), (274, 3000)]),
 }
