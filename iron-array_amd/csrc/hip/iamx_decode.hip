@@ -67,17 +67,13 @@ __device__ __forceinline__ uint32_t rdlane(uint32_t v, int32_t l) {
 // reads and which were flushed (at most RING/2 + 400 bytes are ever pending).
 // With a 4 KiB ring the kernel fits 54 VGPRs: 8 wavefronts per SIMD, 32 per CU.
 template <int RING>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RING <= 4096 ? 8 : 1, 8))) void k_lz4_streams(
-    const iamx_lz4_job_t *__restrict__ jobs, int njobs, uint8_t *__restrict__ scratch, int *status) {
-  __shared__ __align__(16) uint8_t ring[RING];
+__device__ __forceinline__ void lz4_stream(const iamx_lz4_job_t job, uint8_t *__restrict__ scratch, int *status,
+                                           uint8_t *ring) {
   constexpr int32_t RM = RING - 1;
   constexpr int32_t NEAR = RING - 64;
   constexpr int32_t FLUSH_AT = RING / 2;
   constexpr int kRunUnroll = 4;  // 16-byte groups in flight per lane in a long literal run
-  const int j = blockIdx.x;
-  if (j >= njobs) return;
   const int lane = threadIdx.x;
-  const iamx_lz4_job_t job = jobs[j];
   const uint64_t abase = job.src & ~3ull;  // input positions are relative to this aligned address
   const IAMX_GLOBAL uint32_t *__restrict__ sw = iamx::gptr<const uint32_t>(abase);
   const IAMX_GLOBAL uint8_t *__restrict__ sb = iamx::gptr<const uint8_t>(abase);
@@ -426,6 +422,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RING <= 4096
   if (!bad && op != oend) bad = 7;
   if (!bad) flush((oend + 3) & ~3);  // the slot is padded to 16 bytes
   if (bad && lane == 0) atomicOr(status, 1 << bad);
+}
+
+template <int RING>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RING <= 4096 ? 8 : 1, 8))) void k_lz4_streams(
+    const iamx_lz4_job_t *__restrict__ jobs, int njobs, uint8_t *__restrict__ scratch, int *status) {
+  __shared__ __align__(16) uint8_t ring[RING];
+  const int j = blockIdx.x;
+  if (j >= njobs) return;
+  lz4_stream<RING>(jobs[j], scratch, status, ring);
 }
 
 struct Planes {
