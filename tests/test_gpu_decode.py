@@ -7,7 +7,10 @@
 * every tiling / dtype / filter / split mode the host writes round-trips
   through the GPU decoder bit-exactly, checked against the oracle decoder;
 * a corrupted LZ4 stream is reported as IARRAY_ERR_BLOSC_FAILED (the kernel
-  bounds-checks every read and write, so no fault is possible).
+  bounds-checks every read and write, so no fault is possible);
+* LZ4 blocks built sequence by sequence (literal and match lengths with zero,
+  one and many length bytes, offsets from 1 to 65535, periodic matches) decode
+  to the bytes they were built from, and the oracle's decoder agrees.
 """
 import ctypes
 import glob
