@@ -47,6 +47,10 @@ IAMX_API int iamx_sync(void);
  * iamx_event_sync blocks the host until that event has completed. */
 #define IAMX_COPY_EVENTS 64
 IAMX_API void *iamx_host_buffer(size_t bytes);
+/* Pinned, mapped, fine-grained host buffer (grows, kept until iamx_shutdown)
+ * that kernels write directly through *dev: small results skip the copy.
+ * Its bytes are complete once the stream has been synchronised. */
+IAMX_API void *iamx_host_direct(size_t bytes, void **dev);
 IAMX_API int iamx_d2h_async(void *dst, const void *src, size_t bytes, int event);
 IAMX_API int iamx_event_sync(int event);
 /* A download queued on the copy stream after the ones before it, without
@@ -236,7 +240,12 @@ IAMX_API int iamx_capture_end(void **graph_exec);
 IAMX_API int iamx_graph_launch(void *graph_exec);
 IAMX_API void iamx_graph_destroy(void *graph_exec);
 
-/* Error word written by the decode kernels (0 = no error since last reset). */
+/* Error word written by the decode kernels (0 = no error since last reset).
+ * iamx_status_snapshot queues its copy to the host behind the work queued so
+ * far; iamx_decode_status waits for the stream and returns it (queueing the
+ * copy itself when kernels were launched after the last snapshot) and resets
+ * a non-zero word. */
+IAMX_API int iamx_status_snapshot(void);
 IAMX_API int iamx_decode_status(void);
 
 #ifdef __cplusplus

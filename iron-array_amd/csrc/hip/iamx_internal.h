@@ -12,6 +12,8 @@ extern hipStream_t g_stream;
 // The stream launches go to: the library stream, or the side stream between
 // iamx_use_stream(1) and iamx_use_stream(0).
 extern hipStream_t g_cur;
+// kernels and graph replays queued so far (a status snapshot is current while this is unchanged)
+extern uint64_t g_launch_seq;
 extern int g_num_cus;
 
 int set_error(int code, const char *fmt, ...);
@@ -56,6 +58,7 @@ __device__ __forceinline__ IAMX_GLOBAL T *gptr(uint64_t a) {
 #define IAMX_LAUNCH(name, kernel, grid, block, shmem, ...)                          \
   do {                                                                              \
     iamx::ProfScope _ps(name);                                                      \
+    ++iamx::g_launch_seq;                                                           \
     hipLaunchKernelGGL(kernel, grid, block, shmem, iamx::g_cur, __VA_ARGS__);       \
     hipError_t _e = hipGetLastError();                                              \
     if (_e != hipSuccess) return iamx::check_hip(_e, name);                         \

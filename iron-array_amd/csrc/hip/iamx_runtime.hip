@@ -17,12 +17,19 @@ namespace iamx {
 
 hipStream_t g_stream = nullptr;
 hipStream_t g_cur = nullptr;
+uint64_t g_launch_seq = 0;
 static hipStream_t g_side = nullptr;  // second compute stream, default priority
 int g_num_cus = 0;
 static int g_init_state = 0;  // 0 not tried, 1 ok, <0 failed
 static char g_err[512] = "";
 static std::mutex g_mu;
 static int *g_status = nullptr;
+static int *g_status_host = nullptr, *g_status_hdev = nullptr;  // mapped host copy of the status word
+
+// lane 0 copies the status word into mapped host memory (a vector store)
+__global__ void k_status_out(const int *__restrict__ status, int *out) {
+  if (threadIdx.x == 0) out[0] = __atomic_load_n(status, __ATOMIC_RELAXED);
+}
 
 int set_error(int code, const char *fmt, ...) {
   va_list ap;
@@ -238,6 +245,40 @@ IAMX_API void *iamx_host_buffer(size_t bytes) {
   return g_host_buf;
 }
 
+// Host memory the GPU writes directly: fine-grained (coherent) and mapped,
+// so kernels store into it through its device pointer and the bytes are
+// visible to the host once the stream is synchronised.  Grows like the
+// staging buffer.
+static void *g_direct_buf = nullptr, *g_direct_dev = nullptr;
+static size_t g_direct_bytes = 0;
+
+IAMX_API void *iamx_host_direct(size_t bytes, void **dev) {
+  if (ensure_init()) return nullptr;
+  if (!(g_direct_buf && g_direct_bytes >= bytes)) {
+    if (g_direct_buf) {
+      sync_streams();
+      (void)hipHostFree(g_direct_buf);
+      g_direct_buf = g_direct_dev = nullptr;
+      g_direct_bytes = 0;
+    }
+    const size_t want = bytes + bytes / 8 + 4096;
+    if (hipHostMalloc(&g_direct_buf, want, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+      g_direct_buf = nullptr;
+      set_error(IAMX_ENOMEM, "hipHostMalloc(%zu, mapped) failed", want);
+      return nullptr;
+    }
+    if (hipHostGetDevicePointer(&g_direct_dev, g_direct_buf, 0) != hipSuccess || !g_direct_dev) {
+      (void)hipHostFree(g_direct_buf);
+      g_direct_buf = g_direct_dev = nullptr;
+      set_error(IAMX_ENOMEM, "hipHostGetDevicePointer failed");
+      return nullptr;
+    }
+    g_direct_bytes = want;
+  }
+  *dev = g_direct_dev;
+  return g_direct_buf;
+}
+
 IAMX_API int iamx_d2h_async(void *dst, const void *src, size_t bytes, int event) {
   int rc = ensure_init();
   if (rc) return rc;
@@ -351,6 +392,11 @@ IAMX_API void iamx_shutdown(void) {
   if (g_host_buf) (void)hipHostFree(g_host_buf);
   g_host_buf = nullptr;
   g_host_bytes = 0;
+  if (g_direct_buf) (void)hipHostFree(g_direct_buf);
+  g_direct_buf = g_direct_dev = nullptr;
+  g_direct_bytes = 0;
+  if (g_status_host) (void)hipHostFree(g_status_host);
+  g_status_host = g_status_hdev = nullptr;
 }
 
 IAMX_API void iamx_prof_enable(int on) { g_prof_on = on != 0; }
@@ -402,6 +448,7 @@ IAMX_API int iamx_graph_launch(void *graph_exec) {
   int rc = ensure_init();
   if (rc) return rc;
   ProfScope ps("graph_launch");
+  ++g_launch_seq;
   IAMX_HIP_CHECK(hipGraphLaunch((hipGraphExec_t)graph_exec, g_stream));
   return IAMX_OK;
 }
@@ -410,12 +457,39 @@ IAMX_API void iamx_graph_destroy(void *graph_exec) {
   if (graph_exec && g_init_state == 1) (void)hipGraphExecDestroy((hipGraphExec_t)graph_exec);
 }
 
+// The status word reaches the host through a one-thread kernel storing it
+// into mapped host memory, behind the work queued so far: one launch and one
+// stream synchronisation, no staged copy.  A snapshot queued after the last
+// launch (iamx_status_snapshot) is read without a new launch.
+static uint64_t g_snap_seq = ~0ull;
+
+IAMX_API int iamx_status_snapshot(void) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (!g_status_host) {
+    void *h = nullptr, *d = nullptr;
+    IAMX_HIP_CHECK(hipHostMalloc(&h, 64, hipHostMallocMapped | hipHostMallocCoherent));
+    if (hipHostGetDevicePointer(&d, h, 0) != hipSuccess || !d) {
+      (void)hipHostFree(h);
+      return set_error(IAMX_ENOMEM, "hipHostGetDevicePointer(status) failed");
+    }
+    g_status_host = static_cast<int *>(h);
+    g_status_hdev = static_cast<int *>(d);
+  }
+  *(volatile int *)g_status_host = 0;
+  hipLaunchKernelGGL(k_status_out, dim3(1), dim3(64), 0, g_stream, (const int *)g_status, g_status_hdev);
+  IAMX_HIP_CHECK(hipGetLastError());
+  g_snap_seq = g_launch_seq;
+  return IAMX_OK;
+}
+
 IAMX_API int iamx_decode_status(void) {
   int rc = ensure_init();
   if (rc) return rc;
-  int v = 0;
-  IAMX_HIP_CHECK(hipMemcpyAsync(&v, g_status, sizeof(int), hipMemcpyDeviceToHost, g_stream));
+  if (g_snap_seq != g_launch_seq && (rc = iamx_status_snapshot()) != 0) return rc;
   IAMX_HIP_CHECK(hipStreamSynchronize(g_stream));
+  const int v = *(volatile int *)g_status_host;
+  g_snap_seq = ~0ull;
   if (v) IAMX_HIP_CHECK(hipMemsetAsync(g_status, 0, sizeof(int), g_stream));
   return v;
 }

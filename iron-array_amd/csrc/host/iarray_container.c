@@ -893,6 +893,15 @@ static double gpu_encode_share(const iarray_container_t *c, int npanels) {
   return v > 1.0 ? 1.0 : v;
 }
 
+/* Outputs up to this size are packed straight into mapped host memory (the
+ * pack kernels' byte stores then cross the host link); IARRAY_DIRECT_WRITE=0
+ * sends every output through the pipelined downloads. */
+#define IAX_DIRECT_WRITE_MAX ((size_t)2 << 20)
+static int direct_write_on(void) {
+  const char *e = getenv("IARRAY_DIRECT_WRITE");
+  return !(e && e[0] == '0');
+}
+
 ina_rc_t iax_write_from_dense_rows(iarray_container_t *c, const void *d_dense, int64_t panel_rows,
                                    iax_rows_fn rows, void *rctx) {
   iamx_tiling_t t;
@@ -930,6 +939,29 @@ ina_rc_t iax_write_from_dense_rows(iarray_container_t *c, const void *d_dense, i
     if (INA_FAILED(rc)) return rc;
     drop_device(c);
     return INA_SUCCESS;
+  }
+  if (total <= IAX_DIRECT_WRITE_MAX && direct_write_on()) {
+    /* small C: the pack kernels write straight into mapped host memory, so
+     * the host encodes after one stream synchronisation and no copy */
+    void *hdev = NULL;
+    uint8_t *h = (uint8_t *)iamx_host_direct(total + 16, &hdev);
+    if (h) {
+      const double t0 = iax_now();
+      panel_ctx_t pd = {c, &t, d_dense, hdev, rows, rctx};
+      int e = 0;
+      for (int k = 0; k < npanels && !e; k++) e = produce_panel(&pd, k, pf[k], pf[k + 1]);
+      if (!e) e = iamx_status_snapshot(); /* read by the caller's status check without another launch */
+      if (!e) e = iamx_sync();
+      free(ends);
+      free(pf);
+      if (e) return iax_dev_error(e, "panel");
+      wctx_t xd = {c, h, filtered, 0};
+      iax_parallel_for(iax_nthreads(&c->cfg), c->nchunks, write_chunk_task, &xd);
+      iax_last_encode_s = iax_now() - t0;
+      drop_device(c);
+      if (xd.err) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "chunk encode failed");
+      return INA_SUCCESS;
+    }
   }
   panel_ctx_t px = {c, &t, d_dense, d_packed, rows, rctx};
   const double t0 = iax_now();

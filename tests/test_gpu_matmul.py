@@ -177,6 +177,38 @@ def test_row_panels_bit_exact(ctx, monkeypatch, form, panel_rows):
     assert got.tobytes() == one.tobytes()
 
 
+@pytest.mark.parametrize("kind", ["gemv", "gemm-f32", "gemm-f64-panels"])
+def test_direct_and_pipelined_writes_agree(monkeypatch, kind):
+    """Small C packed straight into mapped host memory and C downloaded
+    through the copy stream give the same chunks, byte for byte."""
+    print(f"case: direct vs pipelined write {kind}", flush=True)
+    rng = np.random.default_rng(11)
+    ctx = ia.Context(clevel=5, filters=ia.COMP_BITSHUFFLE if kind == "gemv" else ia.COMP_SHUFFLE)
+    if kind == "gemv":
+        a = rng.standard_normal((700, 500)).astype(np.float32)
+        x = rng.standard_normal(500).astype(np.float32)
+        ca = ia.Container.from_numpy(ctx, a, (256, 250), (128, 125))
+        cx = ia.Container.from_numpy(ctx, x, (250,), (125,))
+        run = lambda: ia.matmul(ca, cx, (256,), (64,))
+    else:
+        dt = np.float32 if kind == "gemm-f32" else np.float64
+        a = rng.standard_normal((300, 200)).astype(dt)
+        b = rng.standard_normal((200, 260)).astype(dt)
+        ca = ia.Container.from_numpy(ctx, a, (128, 100), (64, 50))
+        cb = ia.Container.from_numpy(ctx, b, (100, 130), (50, 65))
+        if kind == "gemm-f64-panels":
+            monkeypatch.setenv("IARRAY_GEMM_PANEL_ROWS", "1")
+        run = lambda: ia.matmul(ca, cb, (128, 130), (64, 65))
+    monkeypatch.setenv("IARRAY_DIRECT_WRITE", "0")
+    piped = run()
+    monkeypatch.setenv("IARRAY_DIRECT_WRITE", "1")
+    direct = run()
+    assert direct.chunks() == piped.chunks()
+    want = a.astype(np.float64) @ (x if kind == "gemv" else b).astype(np.float64)
+    np.testing.assert_allclose(direct.to_numpy(), want, rtol=1e-4 if a.dtype == np.float32 else 1e-12,
+                               atol=1e-3 if a.dtype == np.float32 else 1e-10)
+
+
 def test_bitshuffled_gemv_bit_exact(ctx):
     """The headline GEMV layout (float32, LZ4 + bitshuffle, 250x250 blocks,
     62500 elements per block so every block has a 4-element tail)."""
