@@ -378,8 +378,18 @@ __global__ __launch_bounds__(256) void k_reduce_cols(const T *__restrict__ parti
                                                      T *__restrict__ y) {
   const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (r >= M) return;
+  // the sum runs over the block columns in order (bit-identical run to run);
+  // eight loads go out before their adds
   T s = 0;
-  for (int c = 0; c < nbc; c++) s += partial[(int64_t)c * M + r];
+  int c = 0;
+  for (; c + 8 <= nbc; c += 8) {
+    T v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = partial[(int64_t)(c + k) * M + r];
+#pragma unroll
+    for (int k = 0; k < 8; k++) s += v[k];
+  }
+  for (; c < nbc; c++) s += partial[(int64_t)c * M + r];
   y[r] = s;
 }
 
