@@ -247,6 +247,15 @@ INA_API(ina_rc_t) iarray_opt_gemv(iarray_context_t *ctx, iarray_container_t *a, 
                                   iarray_storage_t *storage, iarray_container_t **c);
 INA_API(ina_rc_t) iarray_opt_gemm(iarray_context_t *ctx, iarray_container_t *a, iarray_container_t *b,
                                   iarray_storage_t *storage, iarray_container_t **c);
+/* iarray.h:872-882.  iarray_opt_gemm_a: src/linalg/iarray_gemm_a.c:191-358 (A one
+ * chunk column, fetched whole per C chunk); iarray_opt_gemm_b:
+ * src/linalg/iarray_gemm_b.c:191-358 (B one chunk row, the mirror).  The same
+ * partition checks in the same order with the same error codes; the product is
+ * the GPU GEMM of iarray_linalg_matmul. */
+INA_API(ina_rc_t) iarray_opt_gemm_a(iarray_context_t *ctx, iarray_container_t *a, iarray_container_t *b,
+                                    iarray_storage_t *storage, iarray_container_t **c);
+INA_API(ina_rc_t) iarray_opt_gemm_b(iarray_context_t *ctx, iarray_container_t *a, iarray_container_t *b,
+                                    iarray_storage_t *storage, iarray_container_t **c);
 
 #ifdef __cplusplus
 }

@@ -1,16 +1,18 @@
 /*
- * iarray_linalg.c -- iarray_linalg_matmul, iarray_opt_gemm, iarray_opt_gemv
- * on the GPU.
+ * iarray_linalg.c -- iarray_linalg_matmul, iarray_opt_gemm, iarray_opt_gemm_a,
+ * iarray_opt_gemm_b, iarray_opt_gemv on the GPU.
  *
  * Validation, error codes and output allocation follow the reference:
  *   iarray_linalg_matmul  src/linalg/matmul.c:19-68
  *   iarray_opt_gemm       src/linalg/iarray_gemm.c:315-364 (partition rules)
+ *   iarray_opt_gemm_a/_b  src/linalg/iarray_gemm_a.c:191-262, src/linalg/iarray_gemm_b.c:191-263
  *   iarray_opt_gemv       src/linalg/iarray_gemv.c:249-296
  * The arithmetic is re-planned for one GPU instead of the reference's
  * per-C-chunk panel decode + per-C-block MKL call: every A and B block is
  * decoded exactly once into a dense operand in HBM, one MFMA GEMM launch
  * computes all of C, and one launch re-shuffles the C blocks for the Blosc2
- * write (LZ4 encode of C stays on the host).  The GEMV branch never builds a
+ * write (LZ4 encode of C on the host threads, part of the panels of large
+ * outputs on the GPU).  The GEMV branch never builds a
  * dense A: blocks are decoded and multiplied in one fused pass.
  */
 #include <string.h>
@@ -298,6 +300,60 @@ INA_API(ina_rc_t) iarray_opt_gemm(iarray_context_t *ctx, iarray_container_t *a, 
     return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_BLOCKSHAPE), "a->blockshape[1] != b->blockshape[0]");
   if (b->block[1] != storage->blockshape[1])
     return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_BLOCKSHAPE), "b->blockshape[1] != c->blockshape[1]");
+  return matmul_impl(ctx, a, b, storage, NULL, c);
+}
+
+#define PART_ERR(code, msg) return iax_set_error(INA_ERROR(code), msg)
+
+/* Checks shared by opt_gemm_a and opt_gemm_b, src/linalg/iarray_gemm_a.c:198-230
+ * and src/linalg/iarray_gemm_b.c:198-230, in their order. */
+static ina_rc_t opt_ab_head(iarray_context_t *ctx, iarray_container_t *a, iarray_container_t *b,
+                            iarray_storage_t *storage, iarray_container_t **c, const char *name) {
+  if (!ctx || !a || !b || !storage || !c) return INA_ERROR(INA_ERR_NULL_POINTER);
+  if (a->viewed || b->viewed) return iax_set_error(INA_ERROR(INA_ERR_NOT_SUPPORTED), "%s reads chunks directly", name);
+  if (a->ndim != 2) PART_ERR(IARRAY_ERR_INVALID_NDIM, "The a dimension must be 2");
+  if (b->ndim != 2) PART_ERR(IARRAY_ERR_INVALID_NDIM, "The b dimension must be 2");
+  if (a->shape[1] != b->shape[0]) PART_ERR(IARRAY_ERR_INVALID_SHAPE, "a->shape[1] != b->shape[0]");
+  if (a->chunk[1] < a->shape[1]) PART_ERR(IARRAY_ERR_INVALID_CHUNKSHAPE, "a->chunkshape[1]  < a->shape[1]");
+  if (b->chunk[0] < b->shape[0]) PART_ERR(IARRAY_ERR_INVALID_CHUNKSHAPE, "b->chunkshape[0] < c->chunkshape[0]");
+  if (a->chunk[1] != b->chunk[0]) PART_ERR(IARRAY_ERR_INVALID_CHUNKSHAPE, "a->chunkshape[1] != b->chunkshape[0]");
+  return INA_SUCCESS;
+}
+
+/* Block checks shared by both, src/linalg/iarray_gemm_a.c:241-262 and
+ * src/linalg/iarray_gemm_b.c:241-263. */
+static ina_rc_t opt_ab_blocks(const iarray_container_t *a, const iarray_container_t *b, const iarray_storage_t *storage) {
+  if (a->chunk[0] != storage->blockshape[0]) PART_ERR(IARRAY_ERR_INVALID_BLOCKSHAPE, "a->chunkshape[0] != c->blockshape[0]");
+  if (b->chunk[1] != storage->blockshape[1]) PART_ERR(IARRAY_ERR_INVALID_BLOCKSHAPE, "b->chunkshape[1] != c->blockshape[1]");
+  if (a->chunk[0] != a->block[0]) PART_ERR(IARRAY_ERR_INVALID_BLOCKSHAPE, "a->chunkshape[0] != a->blockshape[0]");
+  if (b->chunk[1] != b->block[1]) PART_ERR(IARRAY_ERR_INVALID_BLOCKSHAPE, "b->chunkshape[1] != b->blockshape[1]");
+  if (a->block[1] != b->block[0]) PART_ERR(IARRAY_ERR_INVALID_BLOCKSHAPE, "a->blockshape[1] != b->blockshape[0]");
+  return INA_SUCCESS;
+}
+
+/* The reference computes each C block as the sum over K blocks of A block x
+ * B block (one cblas_?gemm each); here the whole product is the GPU GEMM. */
+INA_API(ina_rc_t) iarray_opt_gemm_a(iarray_context_t *ctx, iarray_container_t *a, iarray_container_t *b,
+                                    iarray_storage_t *storage, iarray_container_t **c) {
+  ina_rc_t rc = opt_ab_head(ctx, a, b, storage, c, "opt_gemm_a");
+  if (INA_FAILED(rc)) return rc;
+  /* src/linalg/iarray_gemm_a.c:232-239 */
+  if (a->chunk[0] != storage->chunkshape[0]) PART_ERR(IARRAY_ERR_INVALID_CHUNKSHAPE, "a->chunkshape[0] = c->chunkshape[0]");
+  if (b->shape[1] > storage->chunkshape[1]) PART_ERR(IARRAY_ERR_INVALID_CHUNKSHAPE, "b->shape[1] > c->chunkshape[1]");
+  rc = opt_ab_blocks(a, b, storage);
+  if (INA_FAILED(rc)) return rc;
+  return matmul_impl(ctx, a, b, storage, NULL, c);
+}
+
+INA_API(ina_rc_t) iarray_opt_gemm_b(iarray_context_t *ctx, iarray_container_t *a, iarray_container_t *b,
+                                    iarray_storage_t *storage, iarray_container_t **c) {
+  ina_rc_t rc = opt_ab_head(ctx, a, b, storage, c, "opt_gemm_b");
+  if (INA_FAILED(rc)) return rc;
+  /* src/linalg/iarray_gemm_b.c:232-239 */
+  if (a->shape[0] > storage->chunkshape[0]) PART_ERR(IARRAY_ERR_INVALID_CHUNKSHAPE, "a->shape[0] > c->chunkshape[0]");
+  if (b->chunk[1] != storage->chunkshape[1]) PART_ERR(IARRAY_ERR_INVALID_CHUNKSHAPE, "b->chunkshape[1] != c->chunkshape[1]");
+  rc = opt_ab_blocks(a, b, storage);
+  if (INA_FAILED(rc)) return rc;
   return matmul_impl(ctx, a, b, storage, NULL, c);
 }
 

@@ -107,6 +107,8 @@ def lib():
         "iarray_linalg_transpose": ([vp, vp, P(vp)], rc_t),
         "iarray_opt_gemv": ([vp, vp, vp, P(Storage), P(vp)], rc_t),
         "iarray_opt_gemm": ([vp, vp, vp, P(Storage), P(vp)], rc_t),
+        "iarray_opt_gemm_a": ([vp, vp, vp, P(Storage), P(vp)], rc_t),
+        "iarray_opt_gemm_b": ([vp, vp, vp, P(Storage), P(vp)], rc_t),
         "iarray_container_chunk": ([vp, i64, P(ctypes.POINTER(ctypes.c_uint8)), P(ctypes.c_int32)], rc_t),
         "iarray_container_nchunks": ([vp, P(i64)], rc_t),
         "iarray_from_chunks": ([vp, P(DTShape), P(Storage), i64, P(vp), P(ctypes.c_int32), P(vp)], rc_t),
@@ -131,7 +133,8 @@ def exported_symbols():
             "iarray_empty", "iarray_zeros", "iarray_arange", "iarray_linspace", "iarray_from_buffer",
             "iarray_to_buffer", "iarray_get_slice_buffer", "iarray_get_dtshape", "iarray_get_storage",
             "iarray_is_view", "iarray_container_info", "iarray_container_free", "iarray_container_almost_equal",
-            "iarray_linalg_matmul", "iarray_linalg_transpose", "iarray_opt_gemv", "iarray_opt_gemm"]
+            "iarray_linalg_matmul", "iarray_linalg_transpose", "iarray_opt_gemv", "iarray_opt_gemm",
+            "iarray_opt_gemm_a", "iarray_opt_gemm_b"]
 
 
 def _check(rc):
@@ -304,6 +307,20 @@ def opt_gemm(a, b, chunks, blocks):
     p = ctypes.c_void_p()
     s = _storage(chunks, blocks)
     _check(lib().iarray_opt_gemm(a.ctx.ptr, a.ptr, b.ptr, ctypes.byref(s), ctypes.byref(p)))
+    return Container(p, a.ctx, a.dtype, _out_shape(a, b))
+
+
+def opt_gemm_a(a, b, chunks, blocks):
+    p = ctypes.c_void_p()
+    s = _storage(chunks, blocks)
+    _check(lib().iarray_opt_gemm_a(a.ctx.ptr, a.ptr, b.ptr, ctypes.byref(s), ctypes.byref(p)))
+    return Container(p, a.ctx, a.dtype, _out_shape(a, b))
+
+
+def opt_gemm_b(a, b, chunks, blocks):
+    p = ctypes.c_void_p()
+    s = _storage(chunks, blocks)
+    _check(lib().iarray_opt_gemm_b(a.ctx.ptr, a.ptr, b.ptr, ctypes.byref(s), ctypes.byref(p)))
     return Container(p, a.ctx, a.dtype, _out_shape(a, b))
 
 

@@ -27,7 +27,8 @@
  *   - orc_array_to_dense .... closed-form tiling (Caterva, restated from
  *                              src/iarray_zproxy.c:46-85 and src/linalg/matmul/gemm.c:25-69),
  *                              brute-force checked on tiny shapes in tests.
- *   - orc_gemm_blosc / orc_gemv_blosc / orc_opt_gemm / orc_opt_gemv ... integer-valued
+ *   - orc_gemm_blosc / orc_gemv_blosc / orc_opt_gemm / orc_opt_gemv / orc_opt_gemm_a /
+ *     orc_opt_gemm_b ......... integer-valued
  *                              inputs (exact in any summation order) against a dense
  *                              product, the reference tests' arange/linspace fixtures at
  *                              their tolerances (tests/test_linalg_gemm.c:128-147).
@@ -116,6 +117,10 @@ int orc_set_lz4(int which);
 int orc_gemm_blosc(const orc_array_t *a, const orc_array_t *b, const int64_t *c_chunkshape,
                    const int64_t *c_blockshape, void *c, int nthreads);
 int orc_gemv_blosc(const orc_array_t *a, const orc_array_t *b, const int64_t *c_chunkshape,
+                   const int64_t *c_blockshape, void *c, int nthreads);
+int orc_opt_gemm_a(const orc_array_t *a, const orc_array_t *b, const int64_t *c_chunkshape,
+                   const int64_t *c_blockshape, void *c, int nthreads);
+int orc_opt_gemm_b(const orc_array_t *a, const orc_array_t *b, const int64_t *c_chunkshape,
                    const int64_t *c_blockshape, void *c, int nthreads);
 int orc_opt_gemm(const orc_array_t *a, const orc_array_t *b, const int64_t *c_chunkshape,
                  const int64_t *c_blockshape, void *c, int nthreads);

@@ -55,7 +55,7 @@ def lib():
             f.restype = None
         L.orc_array_to_dense.argtypes = [ctypes.POINTER(OrcArray), vp, ctypes.c_int]
         L.orc_get_slice.argtypes = [ctypes.POINTER(OrcArray), vp, vp, vp, vp, ctypes.c_int]
-        for f in (L.orc_gemm_blosc, L.orc_gemv_blosc, L.orc_opt_gemm, L.orc_opt_gemv):
+        for f in (L.orc_gemm_blosc, L.orc_gemv_blosc, L.orc_opt_gemm, L.orc_opt_gemv, L.orc_opt_gemm_a, L.orc_opt_gemm_b):
             f.argtypes = [ctypes.POINTER(OrcArray), ctypes.POINTER(OrcArray), vp, vp, vp, ctypes.c_int]
         L.orc_set_blas.argtypes = [ctypes.c_int]
         L.orc_blas_name.restype = ctypes.c_char_p
@@ -175,6 +175,14 @@ def gemv_blosc(a, b, cchunk, cblock, nthreads=4):
 
 def opt_gemm(a, b, cchunk, cblock, nthreads=4):
     return _matmul(lib().orc_opt_gemm, a, b, cchunk, cblock, nthreads)
+
+
+def opt_gemm_a(a, b, cchunk, cblock, nthreads=4):
+    return _matmul(lib().orc_opt_gemm_a, a, b, cchunk, cblock, nthreads)
+
+
+def opt_gemm_b(a, b, cchunk, cblock, nthreads=4):
+    return _matmul(lib().orc_opt_gemm_b, a, b, cchunk, cblock, nthreads)
 
 
 def opt_gemv(a, b, cchunk, cblock, nthreads=4):

@@ -1,7 +1,7 @@
 /*
  * tiling_linalg.c -- TEST INFRASTRUCTURE (oracle).
  *
- * Caterva two-level tiling and the reference's four matmul algorithms,
+ * Caterva two-level tiling and the reference's six matmul algorithms,
  * restated on the CPU from:
  *   - tiling: src/iarray_zproxy.c:46-85 (chunk/block coordinates, padding
  *     trims), src/linalg/matmul/gemm.c:25-69 (block number -> start),
@@ -15,7 +15,10 @@
  *   - iarray_opt_gemm: src/linalg/iarray_gemm.c:91-432 (per C block, K-chunk
  *     and K-block loops, zero-chunk/zero-block skips, edge trimming only in
  *     padded chunks, beta 1);
- *   - iarray_opt_gemv: src/linalg/iarray_gemv.c:88-351.
+ *   - iarray_opt_gemv: src/linalg/iarray_gemv.c:88-351;
+ *   - iarray_opt_gemm_a / _b: src/linalg/iarray_gemm_a.c:81-358 and
+ *     src/linalg/iarray_gemm_b.c:81-358 (one operand fetched a chunk at a
+ *     time, per C block a K-block loop, zero-chunk/zero-block skips, beta 1).
  * The reference computes each C block inside a Blosc2 prefilter and then
  * compresses it; here the block is written straight into a dense C (the
  * values C decodes to).  Per-block BLAS runs on a pthread pool, as Blosc2's
@@ -536,6 +539,104 @@ int orc_opt_gemm(const orc_array_t *a, const orc_array_t *b, const int64_t *cchu
 int orc_opt_gemv(const orc_array_t *a, const orc_array_t *b, const int64_t *cchunk, const int64_t *cblock,
                  void *c, int nthreads) {
   return opt_common(a, b, cchunk, cblock, c, nthreads, 1);
+}
+
+/* ---- iarray_opt_gemm_a / _b (src/linalg/iarray_gemm_a.c:81-358, iarray_gemm_b.c:81-358) ----
+ * gemm_a: A has one chunk column; C chunk i is A's chunk row i, and block j of
+ * it is B's chunk j (C blocks are A chunk rows x B chunk columns).  Each C
+ * block starts at zero and adds, for every block k along K, A block k of A's
+ * chunk i times B block k of B's chunk j (cblas_?gemm, beta 1), skipping
+ * zero blocks, and stays zero when B's chunk is a zero chunk
+ * (iarray_gemm_a.c:100-175).  gemm_b is the mirror: B has one chunk row, C
+ * chunk j is B's chunk column j and block i of it is A's chunk i; the chunk
+ * tested for zeros is A's (iarray_gemm_b.c:100-175). */
+typedef struct {
+  const orc_array_t *a, *b;
+  tiling_t at, bt;
+  int64_t c_chunk;
+  int variant_b;
+  uint8_t *c;
+  int64_t M, N;
+  atomic_int err;
+} ab_ctx_t;
+
+static void ab_block_task(void *vctx, int64_t nb, uint8_t *scratch) {
+  ab_ctx_t *x = (ab_ctx_t *)vctx;
+  const tiling_t *at = &x->at, *bt = &x->bt;
+  const int is = at->itemsize;
+  const int64_t a_nc = x->variant_b ? nb : x->c_chunk, b_nc = x->variant_b ? x->c_chunk : nb;
+  const int64_t Mb = at->block[0], Nb = bt->block[1], Kb = at->block[1];
+  uint8_t *out = scratch;
+  uint8_t *a_block = out + Mb * Nb * is;
+  uint8_t *b_block = a_block + at->blocksize;
+  memset(out, 0, (size_t)(Mb * Nb * is));
+  const uint8_t *ach = x->a->chunks[a_nc], *bch = x->b->chunks[b_nc];
+  const int32_t acs = x->a->csizes[a_nc], bcs = x->b->csizes[b_nc];
+  if (!orc_chunk_is_zeros(x->variant_b ? ach : bch)) {
+    const int64_t nk = x->variant_b ? at->extchunk[1] / at->block[1] : bt->extchunk[0] / bt->block[0];
+    for (int64_t k = 0; k < nk; k++) {
+      if (orc_block_is_zeros(ach, acs, (int)k) || orc_block_is_zeros(bch, bcs, (int)k)) continue;
+      const int64_t a_items = at->block[0] * at->block[1], b_items = bt->block[0] * bt->block[1];
+      if (orc_getitem(ach, acs, (int)(k * a_items), (int)a_items, a_block) < 0 ||
+          orc_getitem(bch, bcs, (int)(k * b_items), (int)b_items, b_block) < 0) {
+        atomic_store(&x->err, 1);
+        return;
+      }
+      gemm_any(is, (int)Mb, (int)Nb, (int)Kb, a_block, (int)Kb, b_block, (int)Nb, 1, out, (int)Nb);
+    }
+  }
+  for (int64_t r = 0; r < Mb; r++) {
+    const int64_t gr = a_nc * at->chunk[0] + r;
+    if (gr >= x->M) break;
+    for (int64_t cc = 0; cc < Nb; cc++) {
+      const int64_t gc = b_nc * bt->chunk[1] + cc;
+      if (gc >= x->N) break;
+      memcpy(x->c + (gr * x->N + gc) * is, out + (r * Nb + cc) * is, is);
+    }
+  }
+}
+
+static int opt_ab(const orc_array_t *a, const orc_array_t *b, const int64_t *cchunk, const int64_t *cblock, void *c,
+                  int nthreads, int variant_b) {
+  if (a->ndim != 2 || b->ndim != 2 || a->transposed || b->transposed) return -1;
+  if (a->itemsize != b->itemsize || a->shape[1] != b->shape[0]) return -1;
+  /* partition constraints (iarray_gemm_a.c:213-262, iarray_gemm_b.c:213-263) */
+  if (a->chunkshape[1] < a->shape[1] || b->chunkshape[0] < b->shape[0] || a->chunkshape[1] != b->chunkshape[0])
+    return -2;
+  if (!variant_b && (a->chunkshape[0] != cchunk[0] || b->shape[1] > cchunk[1])) return -2;
+  if (variant_b && (a->shape[0] > cchunk[0] || b->chunkshape[1] != cchunk[1])) return -2;
+  if (a->chunkshape[0] != cblock[0] || b->chunkshape[1] != cblock[1] || a->chunkshape[0] != a->blockshape[0] ||
+      b->chunkshape[1] != b->blockshape[1] || a->blockshape[1] != b->blockshape[0])
+    return -3;
+  ab_ctx_t x;
+  memset(&x, 0, sizeof(x));
+  x.a = a;
+  x.b = b;
+  x.variant_b = variant_b;
+  array_tiling(a, &x.at);
+  array_tiling(b, &x.bt);
+  x.M = a->shape[0];
+  x.N = b->shape[1];
+  x.c = (uint8_t *)c;
+  atomic_init(&x.err, 0);
+  const size_t scratch = (size_t)(x.at.block[0] * x.bt.block[1] * x.at.itemsize + x.at.blocksize + x.bt.blocksize);
+  const int64_t n_c = variant_b ? x.bt.nchunks_ax[1] : x.at.nchunks_ax[0];
+  const int64_t n_blocks = variant_b ? x.at.nchunks_ax[0] : x.bt.nchunks_ax[1];
+  for (int64_t ci = 0; ci < n_c; ci++) {
+    x.c_chunk = ci;
+    parallel_for(nthreads, n_blocks, scratch, ab_block_task, &x);
+  }
+  return atomic_load(&x.err) ? -4 : 0;
+}
+
+int orc_opt_gemm_a(const orc_array_t *a, const orc_array_t *b, const int64_t *cchunk, const int64_t *cblock,
+                   void *c, int nthreads) {
+  return opt_ab(a, b, cchunk, cblock, c, nthreads, 0);
+}
+
+int orc_opt_gemm_b(const orc_array_t *a, const orc_array_t *b, const int64_t *cchunk, const int64_t *cblock,
+                   void *c, int nthreads) {
+  return opt_ab(a, b, cchunk, cblock, c, nthreads, 1);
 }
 
 /* ---- C write stage for the CPU baseline ------------------------------------------------ */

@@ -1,4 +1,5 @@
-"""GPU parity of iarray_linalg_matmul / iarray_opt_gemm / iarray_opt_gemv.
+"""GPU parity of iarray_linalg_matmul / iarray_opt_gemm / iarray_opt_gemm_a /
+iarray_opt_gemm_b / iarray_opt_gemv.
 
 Oracle: oracle/ (gemm_blosc, gemv_blosc, opt_gemm, opt_gemv restated on the
 CPU, plain-C BLAS), run on the same compressed containers.  Bars:
@@ -130,6 +131,39 @@ def test_opt_gemm_matches_oracle(ctx, orc):
     z = ia.opt_gemm(x, y, (100, 60), (50, 20))
     o = orc.opt_gemm(_oracle(orc, x), _oracle(orc, y), (100, 60), (50, 20))
     _rel_ok(z.to_numpy(), o, 1e-14)
+
+
+# (variant, M, K, N, A chunk/block, B chunk/block, C chunk/block); padded edges
+AB_CASES = [
+    ("a", 300, 250, 170, ((100, 250), (100, 50)), ((250, 60), (50, 60)), ((100, 170), (100, 60))),
+    ("a", 128, 96, 200, ((64, 96), (64, 32)), ((96, 100), (32, 100)), ((64, 200), (64, 100))),
+    ("b", 300, 250, 170, ((100, 250), (100, 50)), ((250, 60), (50, 60)), ((300, 60), (100, 60))),
+    ("b", 128, 96, 200, ((32, 96), (32, 48)), ((96, 80), (48, 80)), ((128, 80), (32, 80))),
+]
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("case", AB_CASES, ids=[f"gemm_{c[0]}-{c[1]}x{c[2]}x{c[3]}" for c in AB_CASES])
+def test_opt_gemm_ab_matches_oracle(orc, dt, case):
+    """iarray_opt_gemm_a / _b against the oracle's restatement of the
+    reference's block loops: linspace operands at the reference tests'
+    tolerances, integer-valued operands exactly."""
+    var, M, K, N, (acs, abs_), (bcs, bbs), (ccs, cbs) = case
+    print(f"case: opt_gemm_{var} {M}x{K}x{N} {np.dtype(dt).name}", flush=True)
+    ctx = ia.Context(clevel=5, filters=ia.COMP_SHUFFLE)
+    x = _mk(ctx, "linspace", dt, (M, K), acs, abs_, start=-1.0, stop=2.0)
+    y = _mk(ctx, "linspace", dt, (K, N), bcs, bbs, start=0.5, stop=3.0)
+    f = ia.opt_gemm_a if var == "a" else ia.opt_gemm_b
+    of = orc.opt_gemm_a if var == "a" else orc.opt_gemm_b
+    z = f(x, y, ccs, cbs)
+    _rel_ok(z.to_numpy(), of(_oracle(orc, x), _oracle(orc, y), ccs, cbs), TOL[np.dtype(dt)])
+    rng = np.random.default_rng(M * 5 + N)
+    a = rng.integers(-8, 9, (M, K)).astype(dt)
+    b = rng.integers(-8, 9, (K, N)).astype(dt)
+    ca = ia.Container.from_numpy(ctx, a, acs, abs_)
+    cb = ia.Container.from_numpy(ctx, b, bcs, bbs)
+    got = f(ca, cb, ccs, cbs).to_numpy()
+    assert np.array_equal(got, (a.astype(np.float64) @ b.astype(np.float64)).astype(dt))
 
 
 @pytest.mark.parametrize("dt", [np.float32, np.float64])

@@ -155,6 +155,98 @@ def test_opt_partition_errors():
     assert _err(lambda: ia.opt_gemv(a, x2, (20,), (10,))) == ia.ERR_INVALID_BLOCKSHAPE
 
 
+def test_opt_gemm_a_partition_errors():
+    """The checks of src/linalg/iarray_gemm_a.c:198-262, one at a time in their
+    order; a valid partition gets past them (and then needs the GPU)."""
+    ctx = _ctx()
+    f = np.float64
+    A = ia.Container.arange(ctx, f, (40, 30), (20, 30), (20, 10))
+    B = ia.Container.arange(ctx, f, (30, 24), (30, 12), (10, 12))
+    ok_c, ok_b = (20, 24), (20, 12)
+    v1 = ia.Container.arange(ctx, f, (30,), (30,), (10,))
+    assert _err(lambda: ia.opt_gemm_a(v1, B, ok_c, ok_b)) == ia.ERR_INVALID_NDIM
+    assert _err(lambda: ia.opt_gemm_a(A, v1, ok_c, ok_b)) == ia.ERR_INVALID_NDIM
+    B31 = ia.Container.arange(ctx, f, (31, 24), (31, 12), (10, 12))
+    assert _err(lambda: ia.opt_gemm_a(A, B31, ok_c, ok_b)) == ia.ERR_INVALID_SHAPE
+    A_c = ia.Container.arange(ctx, f, (40, 30), (20, 10), (20, 10))  # a chunk narrower than a
+    assert _err(lambda: ia.opt_gemm_a(A_c, B, ok_c, ok_b)) == ia.ERR_INVALID_CHUNKSHAPE
+    B_c = ia.Container.arange(ctx, f, (30, 24), (10, 12), (10, 12))  # b chunk shorter than b
+    assert _err(lambda: ia.opt_gemm_a(A, B_c, ok_c, ok_b)) == ia.ERR_INVALID_CHUNKSHAPE
+    B_k = ia.Container.arange(ctx, f, (30, 24), (40, 12), (10, 12))  # b chunk rows != a chunk cols
+    assert _err(lambda: ia.opt_gemm_a(A, B_k, ok_c, ok_b)) == ia.ERR_INVALID_CHUNKSHAPE
+    assert _err(lambda: ia.opt_gemm_a(A, B, (10, 24), ok_b)) == ia.ERR_INVALID_CHUNKSHAPE  # c chunk rows
+    assert _err(lambda: ia.opt_gemm_a(A, B, (20, 12), ok_b)) == ia.ERR_INVALID_CHUNKSHAPE  # c chunk too narrow
+    assert _err(lambda: ia.opt_gemm_a(A, B, ok_c, (10, 12))) == ia.ERR_INVALID_BLOCKSHAPE
+    assert _err(lambda: ia.opt_gemm_a(A, B, ok_c, (20, 6))) == ia.ERR_INVALID_BLOCKSHAPE
+    A_b = ia.Container.arange(ctx, f, (40, 30), (20, 30), (10, 10))  # a block rows != a chunk rows
+    assert _err(lambda: ia.opt_gemm_a(A_b, B, ok_c, ok_b)) == ia.ERR_INVALID_BLOCKSHAPE
+    B_b = ia.Container.arange(ctx, f, (30, 24), (30, 12), (10, 6))  # b block cols != b chunk cols
+    assert _err(lambda: ia.opt_gemm_a(A, B_b, ok_c, ok_b)) == ia.ERR_INVALID_BLOCKSHAPE
+    A_k = ia.Container.arange(ctx, f, (40, 30), (20, 30), (20, 15))  # a block cols != b block rows
+    assert _err(lambda: ia.opt_gemm_a(A_k, B, ok_c, ok_b)) == ia.ERR_INVALID_BLOCKSHAPE
+    if not _has_gpu():
+        assert _err(lambda: ia.opt_gemm_a(A, B, ok_c, ok_b)) == ia.ERR_NO_DEVICE
+
+
+def test_opt_gemm_b_partition_errors():
+    """The checks of src/linalg/iarray_gemm_b.c:198-263 in their order."""
+    ctx = _ctx()
+    f = np.float64
+    A = ia.Container.arange(ctx, f, (40, 30), (10, 30), (10, 10))
+    B = ia.Container.arange(ctx, f, (30, 24), (30, 12), (10, 12))
+    ok_c, ok_b = (40, 12), (10, 12)
+    A_c = ia.Container.arange(ctx, f, (40, 30), (10, 15), (10, 15))
+    assert _err(lambda: ia.opt_gemm_b(A_c, B, ok_c, ok_b)) == ia.ERR_INVALID_CHUNKSHAPE
+    assert _err(lambda: ia.opt_gemm_b(A, B, (20, 12), ok_b)) == ia.ERR_INVALID_CHUNKSHAPE  # c chunk shorter than a
+    assert _err(lambda: ia.opt_gemm_b(A, B, (40, 24), ok_b)) == ia.ERR_INVALID_CHUNKSHAPE  # c chunk cols != b's
+    assert _err(lambda: ia.opt_gemm_b(A, B, ok_c, (20, 12))) == ia.ERR_INVALID_BLOCKSHAPE
+    assert _err(lambda: ia.opt_gemm_b(A, B, ok_c, (10, 6))) == ia.ERR_INVALID_BLOCKSHAPE
+    A_b = ia.Container.arange(ctx, f, (40, 30), (10, 30), (5, 10))
+    assert _err(lambda: ia.opt_gemm_b(A_b, B, ok_c, ok_b)) == ia.ERR_INVALID_BLOCKSHAPE
+    if not _has_gpu():
+        assert _err(lambda: ia.opt_gemm_b(A, B, ok_c, ok_b)) == ia.ERR_NO_DEVICE
+
+
+# (variant, M, K, N, A chunk/block, B chunk/block, C chunk/block): gemm_a wants A
+# in one chunk column with block rows = chunk rows, B in one chunk row with
+# block columns = chunk columns; gemm_b the mirror of the C tiling.  Padded
+# edges on every axis.
+AB_CASES = [
+    ("a", 70, 45, 50, ((30, 45), (30, 20)), ((45, 20), (20, 20)), ((30, 50), (30, 20))),
+    ("a", 64, 64, 64, ((32, 64), (32, 16)), ((64, 32), (16, 32)), ((32, 64), (32, 32))),
+    ("b", 70, 45, 50, ((30, 45), (30, 20)), ((45, 20), (20, 20)), ((70, 20), (30, 20))),
+    ("b", 64, 64, 96, ((16, 64), (16, 32)), ((64, 48), (32, 48)), ((64, 48), (16, 48))),
+]
+
+
+@pytest.mark.parametrize("case", AB_CASES, ids=[f"gemm_{c[0]}-{c[1]}x{c[2]}x{c[3]}" for c in AB_CASES])
+def test_oracle_opt_gemm_ab_exact(orc, case):
+    """The oracle's restatement of opt_gemm_a / opt_gemm_b on integer-valued
+    operands (exact in any summation order) equals the dense product, with
+    a zero chunk and zero blocks planted in the operand each variant fetches
+    block by block; a transposed operand would not."""
+    var, M, K, N, (acs, abs_), (bcs, bbs), (ccs, cbs) = case
+    ctx = ia.Context(clevel=5, filters=ia.COMP_SHUFFLE)
+    rng = np.random.default_rng(M + 3 * K + 7 * N)
+    a = rng.integers(-8, 9, (M, K)).astype(np.float64)
+    b = rng.integers(-8, 9, (K, N)).astype(np.float64)
+    if var == "a":
+        b[:, :bcs[1]] = 0                      # B's first chunk is all zeros
+        b[bbs[0]:2 * bbs[0], bcs[1]:] = 0      # a zero block row inside the others
+    else:
+        a[:acs[0], :] = 0
+        a[acs[0]:, :abs_[1]] = 0
+    ca = ia.Container.from_numpy(ctx, a, acs, abs_)
+    cb = ia.Container.from_numpy(ctx, b, bcs, bbs)
+    oa = orc.Array(ca.chunks(), a.shape, acs, abs_, 8)
+    ob = orc.Array(cb.chunks(), b.shape, bcs, bbs, 8)
+    fn = orc.opt_gemm_a if var == "a" else orc.opt_gemm_b
+    got = fn(oa, ob, ccs, cbs)
+    assert np.array_equal(got, a @ b)
+    if M == K:
+        assert not np.array_equal(got, a.T @ b)
+
+
 def test_constructor_errors():
     ctx = _ctx()
     small = np.zeros(10)
