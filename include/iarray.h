@@ -158,7 +158,7 @@ typedef struct iarray_dtshape_s {
 
 /* include/libiarray/iarray.h:235-240 */
 typedef struct iarray_storage_s {
-  char *urlpath; /* on-disk frames are not supported by this build: must be NULL */
+  char *urlpath; /* must be NULL: containers live in memory (frames are read by iarray_container_load) */
   bool contiguous;
   int64_t chunkshape[IARRAY_DIMENSION_MAX];
   int64_t blockshape[IARRAY_DIMENSION_MAX];
@@ -238,6 +238,16 @@ INA_API(ina_rc_t) iarray_linalg_matmul(iarray_context_t *ctx, iarray_container_t
                                        iarray_storage_t *storage, iarray_container_t **c);
 /* src/linalg/transpose.c:18-45 */
 INA_API(ina_rc_t) iarray_linalg_transpose(iarray_context_t *ctx, iarray_container_t *a, iarray_container_t **b);
+
+/* ---- frames: iarray.h:593-599 ------------------------------------------------
+ * A container read from a contiguous Blosc2 frame file written by the
+ * reference (src/iarray_container.c:79-218): shape, chunk and block shapes
+ * from the "caterva" metalayer, dtype from the "iarray" one, the chunks copied
+ * in.  iarray_container_open reads the chunks at open as well (the reference
+ * reads them lazily).  Frames whose offsets chunk is compressed, or that hold
+ * special-value chunks, return INA_ERR_NOT_SUPPORTED. */
+INA_API(ina_rc_t) iarray_container_load(iarray_context_t *ctx, char *urlpath, iarray_container_t **container);
+INA_API(ina_rc_t) iarray_container_open(iarray_context_t *ctx, char *urlpath, iarray_container_t **container);
 
 /* ---- block-wise entry points: iarray.h:860-870 ------------------------------
  * iarray_opt_gemv: src/linalg/iarray_gemv.c:249-351; iarray_opt_gemm:

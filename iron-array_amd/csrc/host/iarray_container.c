@@ -1159,12 +1159,16 @@ INA_API(ina_rc_t) iarray_from_chunks(iarray_context_t *ctx, iarray_dtshape_t *dt
   if (INA_FAILED(rc)) return rc;
   iarray_container_t *c = *container;
   if (nchunks != c->nchunks) {
+    const long long want = (long long)c->nchunks;
     iarray_container_free(ctx, container);
-    return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_CHUNKSHAPE), "expected %lld chunks", (long long)c->nchunks);
+    return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_CHUNKSHAPE), "expected %lld chunks", want);
   }
   for (int64_t i = 0; i < nchunks; i++) {
     c->chunks[i] = (uint8_t *)malloc((size_t)csizes[i]);
-    if (!c->chunks[i]) return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+    if (!c->chunks[i]) {
+      iarray_container_free(ctx, container);
+      return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+    }
     memcpy(c->chunks[i], chunks[i], (size_t)csizes[i]);
     c->csizes[i] = csizes[i];
   }

@@ -1,0 +1,296 @@
+/*
+ * iarray_frame.c -- iarray_container_load / iarray_container_open: a
+ * container read from a contiguous Blosc2 frame file (b2frame), the on-disk
+ * form the reference writes (src/iarray_container.c:79-218: caterva_open on
+ * the file, the "iarray" metalayer for the dtype, the "caterva" metalayer for
+ * the shape, chunk shape and block shape).
+ *
+ * The frame is read as c-blosc2 lays it out: a msgpack header (a 14-element
+ * array: magic "b2frame\0", header size, frame size, flags, nbytes, cbytes,
+ * typesize, blocksize, chunksize, two thread counts, a flag, the coding
+ * parameters, then [index size, {metalayer name: header offset}, [contents]]),
+ * the data chunks from the end of the header on, then the offsets chunk (one
+ * int64 per chunk, relative to the end of the header) at header size +
+ * cbytes, then a trailer that is not needed here.  The caterva metalayer is
+ * [version, ndim, shape[ndim], chunkshape[ndim], blockshape[ndim]], the
+ * iarray metalayer [version, dtype, flags] (src/iarray_private.h:364-419).
+ *
+ * The chunks are copied into an in-memory container; the GPU decodes them as
+ * it does every other container.  Not read here (an error says so): an
+ * offsets chunk that is compressed rather than stored, and special-value
+ * chunks (negative offsets).
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "iarray_private.h"
+
+/* ---- a minimal msgpack reader over the types c-blosc2's frame header uses ---- */
+typedef struct {
+  const uint8_t *p, *end;
+  int bad;
+} mp_t;
+
+static uint64_t be(const uint8_t *p, int n) {
+  uint64_t v = 0;
+  for (int i = 0; i < n; i++) v = (v << 8) | p[i];
+  return v;
+}
+
+static const uint8_t *mp_take(mp_t *m, size_t n) {
+  if (m->bad || (size_t)(m->end - m->p) < n) {
+    m->bad = 1;
+    return NULL;
+  }
+  const uint8_t *q = m->p;
+  m->p += n;
+  return q;
+}
+
+static int64_t mp_int(mp_t *m) {
+  const uint8_t *t = mp_take(m, 1);
+  if (!t) return 0;
+  const uint8_t c = *t;
+  if (c <= 0x7f) return c;
+  if (c >= 0xe0) return (int8_t)c;
+  static const int width[8] = {1, 2, 4, 8, 1, 2, 4, 8}; /* cc..cf unsigned, d0..d3 signed */
+  if (c < 0xcc || c > 0xd3) {
+    m->bad = 1;
+    return 0;
+  }
+  const int k = c - 0xcc, n = width[k];
+  const uint8_t *q = mp_take(m, (size_t)n);
+  if (!q) return 0;
+  const uint64_t u = be(q, n);
+  if (k < 4) return (int64_t)u;
+  if (n == 1) return (int8_t)u;
+  if (n == 2) return (int16_t)u;
+  if (n == 4) return (int32_t)u;
+  return (int64_t)u;
+}
+
+/* element count of an array (want_map = 0) or a map (1) */
+static int64_t mp_len(mp_t *m, int want_map) {
+  const uint8_t *t = mp_take(m, 1);
+  if (!t) return 0;
+  const uint8_t c = *t, fix = want_map ? 0x80 : 0x90, c16 = want_map ? 0xde : 0xdc;
+  if ((c & 0xf0) == fix) return c & 0x0f;
+  if (c == c16 || c == c16 + 1) {
+    const int n = c == c16 ? 2 : 4;
+    const uint8_t *q = mp_take(m, (size_t)n);
+    return q ? (int64_t)be(q, n) : 0;
+  }
+  m->bad = 1;
+  return 0;
+}
+
+/* a str (want_bin = 0) or bin (1) element: its bytes */
+static const uint8_t *mp_bytes(mp_t *m, int want_bin, int64_t *len) {
+  const uint8_t *t = mp_take(m, 1);
+  *len = 0;
+  if (!t) return NULL;
+  const uint8_t c = *t;
+  int n = -1;
+  if (!want_bin && (c & 0xe0) == 0xa0) {
+    *len = c & 0x1f;
+  } else if (c == (want_bin ? 0xc4 : 0xd9)) {
+    n = 1;
+  } else if (c == (want_bin ? 0xc5 : 0xda)) {
+    n = 2;
+  } else if (c == (want_bin ? 0xc6 : 0xdb)) {
+    n = 4;
+  } else {
+    m->bad = 1;
+    return NULL;
+  }
+  if (n > 0) {
+    const uint8_t *q = mp_take(m, (size_t)n);
+    if (!q) return NULL;
+    *len = (int64_t)be(q, n);
+  }
+  return mp_take(m, (size_t)*len);
+}
+
+/* skips one scalar element of the kinds the header holds between chunksize
+ * and the metalayers (ints, booleans, nil, fixext) */
+static void mp_skip(mp_t *m) {
+  if (m->bad || m->p >= m->end) {
+    m->bad = 1;
+    return;
+  }
+  const uint8_t c = *m->p;
+  if (c == 0xc0 || c == 0xc2 || c == 0xc3) {
+    m->p++;
+  } else if (c >= 0xd4 && c <= 0xd8) { /* fixext 1, 2, 4, 8, 16: type byte + data */
+    static const int data[5] = {1, 2, 4, 8, 16};
+    (void)mp_take(m, 2 + (size_t)data[c - 0xd4]);
+  } else {
+    (void)mp_int(m);
+  }
+}
+
+/* ---- frame ------------------------------------------------------------------- */
+static ina_rc_t frame_error(const char *what) {
+  return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "not a readable Blosc2 frame: %s", what);
+}
+
+static ina_rc_t read_meta(const uint8_t *d, int64_t hsize, int64_t off, const uint8_t **content, int64_t *len) {
+  if (off <= 0 || off >= hsize) return frame_error("metalayer offset");
+  mp_t m = {d + off, d + hsize, 0};
+  *content = mp_bytes(&m, 1, len);
+  return m.bad ? frame_error("metalayer") : INA_SUCCESS;
+}
+
+static ina_rc_t container_from_frame(iarray_context_t *ctx, const uint8_t *d, int64_t size,
+                                     iarray_container_t **container) {
+  static const uint8_t magic[10] = {0x9e, 0xa8, 'b', '2', 'f', 'r', 'a', 'm', 'e', 0};
+  if (size < 64 || memcmp(d, magic, sizeof(magic)) != 0) return frame_error("no b2frame header");
+  mp_t m = {d + 10, d + size, 0};
+  const int64_t hsize = mp_int(&m);
+  const int64_t fsize = mp_int(&m);
+  int64_t flen;
+  (void)mp_bytes(&m, 0, &flen); /* general flags */
+  const int64_t nbytes = mp_int(&m), cbytes = mp_int(&m);
+  const int64_t typesize = mp_int(&m), blocksize = mp_int(&m), chunksize = mp_int(&m);
+  for (int i = 0; i < 4; i++) mp_skip(&m); /* thread counts, a flag, the coding parameters */
+  if (m.bad || hsize <= 0 || hsize > size || fsize > size || nbytes < 0 || cbytes < 0 || typesize <= 0 ||
+      blocksize <= 0 || chunksize < 0)
+    return frame_error("header fields");
+  m.end = d + hsize;
+  if (mp_len(&m, 0) < 2) return frame_error("metalayer section");
+  (void)mp_int(&m); /* index size */
+  const int64_t nmeta = mp_len(&m, 1);
+  int64_t cat_off = -1, ia_off = -1;
+  for (int64_t i = 0; i < nmeta && !m.bad; i++) {
+    int64_t nl;
+    const uint8_t *name = mp_bytes(&m, 0, &nl);
+    const int64_t off = mp_int(&m);
+    if (!name) break;
+    if (nl == 7 && memcmp(name, "caterva", 7) == 0) cat_off = off;
+    if (nl == 6 && memcmp(name, "iarray", 6) == 0) ia_off = off;
+  }
+  if (m.bad) return frame_error("metalayer map");
+  if (cat_off < 0) return iax_set_error(INA_ERROR(IARRAY_ERR_CATERVA_FAILED), "frame has no caterva metalayer");
+  if (ia_off < 0) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "frame has no iarray metalayer");
+
+  /* caterva: [version, ndim, shape, chunkshape, blockshape] */
+  const uint8_t *cm;
+  int64_t cl;
+  ina_rc_t rc = read_meta(d, hsize, cat_off, &cm, &cl);
+  if (INA_FAILED(rc)) return rc;
+  mp_t c = {cm, cm + cl, 0};
+  iarray_dtshape_t dts;
+  iarray_storage_t st;
+  memset(&dts, 0, sizeof(dts));
+  memset(&st, 0, sizeof(st));
+  if (mp_len(&c, 0) != 5) return frame_error("caterva metalayer");
+  (void)mp_int(&c);
+  const int64_t ndim = mp_int(&c);
+  if (c.bad || ndim < 1 || ndim > IARRAY_DIMENSION_MAX) return frame_error("caterva ndim");
+  int64_t *dst[3] = {dts.shape, st.chunkshape, st.blockshape};
+  for (int k = 0; k < 3; k++) {
+    if (mp_len(&c, 0) != ndim) return frame_error("caterva shapes");
+    for (int64_t i = 0; i < ndim; i++) dst[k][i] = mp_int(&c);
+  }
+  if (c.bad) return frame_error("caterva shapes");
+  dts.ndim = (int8_t)ndim;
+
+  /* iarray: [version, dtype, flags] */
+  const uint8_t *im;
+  int64_t il;
+  rc = read_meta(d, hsize, ia_off, &im, &il);
+  if (INA_FAILED(rc)) return rc;
+  if (il < 3 || im[0] != 0x93) return frame_error("iarray metalayer");
+  if (im[2] >= IARRAY_DATA_TYPE_MAX) return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_DTYPE), "The data type is invalid");
+  dts.dtype = (iarray_data_type_t)im[2];
+  if (iax_dtype_size(dts.dtype) != typesize) return frame_error("typesize disagrees with the dtype");
+
+  /* the metalayers must describe the frame's own chunks: a Caterva chunk is
+   * its chunk shape rounded up to whole blocks, and there is one per cell of
+   * the chunk grid */
+  int64_t ext = typesize, ncells = 1;
+  for (int64_t i = 0; i < ndim; i++) {
+    const int64_t s = dts.shape[i], ch = st.chunkshape[i], bl = st.blockshape[i];
+    if (s < 0 || ch < 1 || bl < 1 || bl > ch || ch > ((int64_t)1 << 31) || s > ((int64_t)1 << 48))
+      return frame_error("caterva shapes");
+    ext *= (ch + bl - 1) / bl * bl;
+    ncells *= s == 0 ? 0 : (s + ch - 1) / ch;
+    if (ext > ((int64_t)1 << 31) || ncells > ((int64_t)1 << 31)) return frame_error("caterva shapes");
+  }
+  const int64_t nchunks = chunksize > 0 ? (nbytes + chunksize - 1) / chunksize : 0;
+  if (ext != chunksize || ncells != nchunks) return frame_error("caterva metalayer disagrees with the chunks");
+
+  /* chunk offsets: the chunk after the data, stored (memcpyed) */
+  const int64_t opos = hsize + cbytes;
+  if (opos < hsize || opos + 32 > size) return frame_error("offsets chunk position");
+  const uint8_t *oc = d + opos;
+  int32_t onb, ocb;
+  memcpy(&onb, oc + 4, 4);
+  memcpy(&ocb, oc + 12, 4);
+  if (oc[0] < 1 || onb != 8 * nchunks || ocb < 32 || opos + ocb > size) return frame_error("offsets chunk");
+  if (!(oc[2] & 0x02) || ocb < 32 + onb)
+    return iax_set_error(INA_ERROR(INA_ERR_NOT_SUPPORTED), "frames with a compressed offsets chunk are not read");
+  const uint8_t **ptrs = (const uint8_t **)malloc(sizeof(*ptrs) * (size_t)(nchunks > 0 ? nchunks : 1));
+  int32_t *csz = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nchunks > 0 ? nchunks : 1));
+  if (!ptrs || !csz) {
+    free(ptrs);
+    free(csz);
+    return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+  }
+  rc = INA_SUCCESS;
+  for (int64_t i = 0; i < nchunks && INA_SUCCEED(rc); i++) {
+    int64_t off;
+    memcpy(&off, oc + 32 + 8 * i, 8);
+    if (off < 0) {
+      rc = iax_set_error(INA_ERROR(INA_ERR_NOT_SUPPORTED), "special-value chunks in frames are not read");
+      break;
+    }
+    const int64_t pos = hsize + off;
+    if (pos + 32 > opos) {
+      rc = frame_error("chunk offset");
+      break;
+    }
+    int32_t cs;
+    memcpy(&cs, d + pos + 12, 4);
+    if (cs < 32 || pos + cs > opos) {
+      rc = frame_error("chunk size");
+      break;
+    }
+    ptrs[i] = d + pos;
+    csz[i] = cs;
+  }
+  if (INA_SUCCEED(rc)) rc = iarray_from_chunks(ctx, &dts, &st, nchunks, ptrs, csz, container);
+  free(ptrs);
+  free(csz);
+  return rc;
+}
+
+static ina_rc_t load(iarray_context_t *ctx, const char *urlpath, iarray_container_t **container) {
+  if (!ctx || !urlpath || !container) return INA_ERROR(INA_ERR_NULL_POINTER);
+  FILE *f = fopen(urlpath, "rb");
+  if (!f) return iax_set_error(INA_ERROR(IARRAY_ERR_CATERVA_FAILED), "cannot open %s", urlpath);
+  uint8_t *d = NULL;
+  int64_t size = -1;
+  if (fseek(f, 0, SEEK_END) == 0) size = ftell(f);
+  if (size > 0 && fseek(f, 0, SEEK_SET) == 0 && (d = (uint8_t *)malloc((size_t)size)) != NULL &&
+      fread(d, 1, (size_t)size, f) != (size_t)size) {
+    free(d);
+    d = NULL;
+  }
+  fclose(f);
+  if (!d) return iax_set_error(INA_ERROR(IARRAY_ERR_CATERVA_FAILED), "cannot read %s", urlpath);
+  const ina_rc_t rc = container_from_frame(ctx, d, size, container);
+  free(d);
+  return rc;
+}
+
+INA_API(ina_rc_t) iarray_container_load(iarray_context_t *ctx, char *urlpath, iarray_container_t **container) {
+  return load(ctx, urlpath, container);
+}
+
+/* The reference keeps an opened frame on disk and reads chunks lazily; here
+ * the chunks are read at open, as for load. */
+INA_API(ina_rc_t) iarray_container_open(iarray_context_t *ctx, char *urlpath, iarray_container_t **container) {
+  return load(ctx, urlpath, container);
+}

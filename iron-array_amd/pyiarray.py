@@ -56,7 +56,7 @@ class IArrayError(RuntimeError):
 # error symbols (values of include/iarray.h)
 ERR_BIT = 1 << 63
 _K = {"INVALID": 0x01, "FAILED": 0x02, "EMPTY": 0x03, "TOO_SMALL": 0x05, "NOT_SUPPORTED": 0x06}
-_S = {"DTSHAPE": 2, "SHAPE": 3, "CHUNKSHAPE": 4, "NDIM": 5, "DTYPE": 6, "STORAGE": 7, "BUFFER": 9, "BLOSC": 11,
+_S = {"DTSHAPE": 2, "SHAPE": 3, "CHUNKSHAPE": 4, "NDIM": 5, "DTYPE": 6, "STORAGE": 7, "BUFFER": 9, "CATERVA": 10, "BLOSC": 11,
       "BLOCKSHAPE": 13, "DEVICE": 64}
 
 
@@ -72,6 +72,7 @@ ERR_INVALID_BLOCKSHAPE = err_code("INVALID", "BLOCKSHAPE")
 ERR_INVALID_STORAGE = err_code("INVALID", "STORAGE")
 ERR_TOO_SMALL_BUFFER = err_code("TOO_SMALL", "BUFFER")
 ERR_BLOSC_FAILED = err_code("FAILED", "BLOSC")
+ERR_CATERVA_FAILED = err_code("FAILED", "CATERVA")
 ERR_NO_DEVICE = err_code("NOT_SUPPORTED", "DEVICE")
 
 _lib = None
@@ -108,6 +109,8 @@ def lib():
         "iarray_opt_gemv": ([vp, vp, vp, P(Storage), P(vp)], rc_t),
         "iarray_opt_gemm": ([vp, vp, vp, P(Storage), P(vp)], rc_t),
         "iarray_opt_gemm_a": ([vp, vp, vp, P(Storage), P(vp)], rc_t),
+        "iarray_container_load": ([vp, ctypes.c_char_p, P(vp)], rc_t),
+        "iarray_container_open": ([vp, ctypes.c_char_p, P(vp)], rc_t),
         "iarray_opt_gemm_b": ([vp, vp, vp, P(Storage), P(vp)], rc_t),
         "iarray_container_chunk": ([vp, i64, P(ctypes.POINTER(ctypes.c_uint8)), P(ctypes.c_int32)], rc_t),
         "iarray_container_nchunks": ([vp, P(i64)], rc_t),
@@ -250,6 +253,18 @@ class Container:
         _check(lib().iarray_from_chunks(ctx.ptr, ctypes.byref(d), ctypes.byref(s), len(bufs), ptrs, sizes,
                                         ctypes.byref(p)))
         return cls(p, ctx, dtype, shape)
+
+    @classmethod
+    def load(cls, ctx, path, lazy=False):
+        """A container read from a Blosc2 frame file (iarray_container_load /
+        iarray_container_open)."""
+        p = ctypes.c_void_p()
+        fn = lib().iarray_container_open if lazy else lib().iarray_container_load
+        _check(fn(ctx.ptr, os.fsencode(path), ctypes.byref(p)))
+        d = DTShape()
+        _check(lib().iarray_get_dtshape(ctx.ptr, p, ctypes.byref(d)))
+        inv = {v: k for k, v in _DT.items()}
+        return cls(p, ctx, inv[d.dtype], tuple(d.shape[i] for i in range(d.ndim)))
 
     def to_numpy(self):
         out = np.empty(self.shape, dtype=self.dtype)

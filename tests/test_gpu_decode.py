@@ -1,7 +1,8 @@
 """GPU parity of the block decode stage (bit-exact).
 
 * the reference's Blosc2 frame fixtures (written by c-blosc2) decode on the GPU
-  to the SHA-256s of tests/golden/fixture_hashes.json;
+  to the SHA-256s of tests/golden/fixture_hashes.json, as raw chunks and as
+  containers read by iarray_container_load;
 * chunks from c-blosc 1.21 (byte shuffle, bitshuffle, ragged element counts,
   format-2 rules) decode on the GPU to their inputs, when that library exists;
 * every tiling / dtype / filter / split mode the host writes round-trips
@@ -42,6 +43,15 @@ def test_fixture_frames_decode_on_gpu(ctx, orc, path):
     chunks, nbytes = orc.frame_chunks(open(path, "rb").read())
     out = ia.decode_chunks_flat(ctx, chunks, nbytes)
     assert hashlib.sha256(out).hexdigest() == HASHES[os.path.basename(path)]["sha256"]
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLD, "frames", "*.iarray"))), ids=os.path.basename)
+def test_loaded_frame_decodes_to_known_answer(ctx, path):
+    """iarray_container_load reads the reference's frame file; the container
+    decodes on the GPU to the fixture's SHA-256."""
+    print(f"case: load {os.path.basename(path)}", flush=True)
+    c = ia.Container.load(ctx, path)
+    assert hashlib.sha256(c.to_numpy().tobytes()).hexdigest() == HASHES[os.path.basename(path)]["sha256"]
 
 
 def test_fixture_frame_as_container(ctx, orc):

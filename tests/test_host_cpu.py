@@ -11,6 +11,7 @@
   GPU is present -- there is no CPU fallback.
 """
 import ctypes
+import glob
 import os
 import re
 
@@ -245,6 +246,64 @@ def test_oracle_opt_gemm_ab_exact(orc, case):
     assert np.array_equal(got, a @ b)
     if M == K:
         assert not np.array_equal(got, a.T @ b)
+
+
+FRAMES = sorted(glob.glob(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "frames", "*.iarray")))
+
+
+@pytest.mark.parametrize("path", FRAMES, ids=os.path.basename)
+def test_frame_load_reads_fixture(orc, path):
+    """iarray_container_load on the reference's frame fixtures: dtype and shape
+    from the metalayers (tests/generator.py:23-38 writes 1e6 elements), and
+    the same chunk bytes the oracle's independent frame parser finds."""
+    ctx = _ctx()
+    c = ia.Container.load(ctx, path)
+    assert c.shape == (1000000,)
+    assert c.dtype == (np.int32 if "int32" in path else np.float32)
+    want, nbytes = orc.frame_chunks(open(path, "rb").read())
+    assert c.chunks() == want
+    nb, cb = c.info()
+    assert nb == nbytes and cb == sum(len(w) for w in want)
+    assert ia.Container.load(ctx, path, lazy=True).chunks() == want
+
+
+def test_frame_load_errors(tmp_path):
+    ctx = _ctx()
+    assert _err(lambda: ia.Container.load(ctx, str(tmp_path / "missing.iarray"))) == ia.ERR_CATERVA_FAILED
+    bad = tmp_path / "bad.iarray"
+    bad.write_bytes(b"not a frame" * 20)
+    assert _err(lambda: ia.Container.load(ctx, str(bad))) == ia.ERR_BLOSC_FAILED
+    data = open(FRAMES[0], "rb").read()
+    short = tmp_path / "short.iarray"
+    short.write_bytes(data[: len(data) // 2])  # the offsets chunk is gone
+    assert _err(lambda: ia.Container.load(ctx, str(short))) == ia.ERR_BLOSC_FAILED
+
+
+def test_frame_load_survives_damage(tmp_path):
+    """Truncated frames and frames with flipped header or offset bytes load
+    or fail with an error; the reader bounds-checks every field."""
+    ctx = _ctx()
+    data = bytearray(open(FRAMES[0], "rb").read())
+    hl = int.from_bytes(data[11:15], "big")
+    rng = np.random.default_rng(3)
+    f = tmp_path / "damaged.iarray"
+    outcomes = set()
+    for i in range(150):
+        d = bytearray(data)
+        if i < 30:
+            d = d[: int(rng.integers(1, len(d)))]
+        else:
+            lo, hi = (0, hl) if i < 110 else (len(d) - 120, len(d))
+            for _ in range(int(rng.integers(1, 4))):
+                d[int(rng.integers(lo, hi))] = int(rng.integers(0, 256))
+        f.write_bytes(bytes(d))
+        try:
+            c = ia.Container.load(ctx, str(f))
+            outcomes.add("loaded")
+            del c
+        except ia.IArrayError:
+            outcomes.add("error")
+    assert "error" in outcomes
 
 
 def test_constructor_errors():
