@@ -73,8 +73,8 @@ def stream_bytes(chunks):
 def main():
     args = parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
-              "the sharding design is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU (asked for --gpus {args.gpus}): the library runs one product on one "
+              "GPU and has no path over several", file=sys.stderr)
         sys.exit(2)
     if args.workload == "hbm_copy":
         hbm_copy(args)

@@ -5,8 +5,8 @@
  *
  * Every declaration mirrors the reference's public header
  * include/libiarray/iarray.h (cited per item) with the same names, argument
- * meaning, ownership and error behaviour.  libinac is not part of this build,
- * so the pieces of its RC convention the API exposes (ina_rc_t, INA_ERROR,
+ * meaning, ownership and error behaviour.  libinac is not used, so the pieces
+ * of its RC convention the API exposes (ina_rc_t, INA_ERROR,
  * INA_SUCCEED/INA_FAILED) are re-declared here; callers compare against the
  * IARRAY_ERR_* symbols, not bit patterns.
  */
@@ -57,7 +57,7 @@ typedef int64_t ina_rc_t;
 #define IARRAY_ES_BLOSC (INA_ES_USER_DEFINED + 11)
 #define IARRAY_ES_ASSERTION (INA_ES_USER_DEFINED + 12)
 #define IARRAY_ES_BLOCKSHAPE (INA_ES_USER_DEFINED + 13)
-/* subject of this build: the GPU (HIP runtime / kernels) */
+/* error subject of the GPU (HIP runtime / kernels) */
 #define IARRAY_ES_DEVICE (INA_ES_USER_DEFINED + 64)
 
 /* errors, include/libiarray/iarray.h:53-90 */
@@ -144,7 +144,7 @@ typedef struct iarray_config_s {
   unsigned int eval_method;
   int max_num_threads; /* host threads for the Blosc2 write stage */
   uint8_t fp_mantissa_bits;
-  bool btune; /* accepted; this build never re-tunes codecs (operands stay LZ4) */
+  bool btune; /* accepted; codecs are never re-tuned (operands stay LZ4) */
   uint8_t compression_meta;
 } iarray_config_t;
 

@@ -5,7 +5,7 @@ The oracle is a double-double (compensated) dot product per output element
 once.  Metrics per result C against C*:
   componentwise  max_ij |C - C*|_ij / (|A||B|)_ij
   normwise       ||C - C*||_F / ||C*||_F
-Bars (written here, derived in DESIGN.md):
+Bars (written here):
   split-bf16 (float32): componentwise <= 6e-5 (3 bf16 products drop lo*lo ~
     2^-18 and accumulate in FP32 over K), and never worse than 4x the plain
     FP32 GPU result on the same inputs plus that bound;
