@@ -42,6 +42,10 @@ typedef int64_t ina_rc_t;
 #define INA_ERR_INVALID_ARGUMENT ((uint64_t)0x08 << 32)
 #define INA_ERR_OUT_OF_MEMORY ((uint64_t)0x09 << 32)
 #define INA_ES_USER_DEFINED ((uint64_t)0x100)
+/* libinac's file subject and the error _iarray_container_load returns for a
+ * path that does not exist (src/iarray_container.c:86-89) */
+#define INA_ES_FILE ((uint64_t)0x05)
+#define INA_ERR_FILE_OPEN (INA_ERR_FAILED | INA_ES_FILE)
 
 /* subjects, include/libiarray/iarray.h:28-50 */
 #define IARRAY_ES_CONTAINER (INA_ES_USER_DEFINED + 1)
@@ -244,10 +248,27 @@ INA_API(ina_rc_t) iarray_linalg_transpose(iarray_context_t *ctx, iarray_containe
  * reference (src/iarray_container.c:79-218): shape, chunk and block shapes
  * from the "caterva" metalayer, dtype from the "iarray" one, the chunks copied
  * in.  iarray_container_open reads the chunks at open as well (the reference
- * reads them lazily).  Frames whose offsets chunk is compressed, or that hold
- * special-value chunks, return INA_ERR_NOT_SUPPORTED. */
+ * reads them lazily).  A path that does not exist returns INA_ERR_FILE_OPEN;
+ * frames whose offsets chunk is compressed, or that hold special-value
+ * chunks, return INA_ERR_NOT_SUPPORTED. */
 INA_API(ina_rc_t) iarray_container_load(iarray_context_t *ctx, char *urlpath, iarray_container_t **container);
 INA_API(ina_rc_t) iarray_container_open(iarray_context_t *ctx, char *urlpath, iarray_container_t **container);
+
+/* iarray.h:601-617, src/iarray_container.c:50-76,219-300.
+ * iarray_container_save writes a base container as one contiguous frame file
+ * (views and opened containers: IARRAY_ERR_INVALID_STORAGE); iarray_to_cframe
+ * returns the same bytes in a malloc'ed buffer (*needs_free is always true);
+ * iarray_from_cframe reads a frame from memory (the chunks are always
+ * copied); iarray_container_remove deletes a frame file, or a frame directory
+ * and the files in it (a path that does not exist is not an error).  A frame
+ * read and saved again keeps its header fields, so the file is rewritten byte
+ * for byte when it holds no variable-length metalayers. */
+INA_API(ina_rc_t) iarray_container_save(iarray_context_t *ctx, iarray_container_t *container, char *urlpath);
+INA_API(ina_rc_t) iarray_to_cframe(iarray_context_t *ctx, iarray_container_t *src, uint8_t **frame,
+                                   int64_t *frame_len, bool *needs_free);
+INA_API(ina_rc_t) iarray_from_cframe(iarray_context_t *ctx, uint8_t *frame, int64_t frame_len, bool copy,
+                                     iarray_container_t **container);
+INA_API(ina_rc_t) iarray_container_remove(char *urlpath);
 
 /* ---- block-wise entry points: iarray.h:860-870 ------------------------------
  * iarray_opt_gemv: src/linalg/iarray_gemv.c:249-351; iarray_opt_gemm:

@@ -1,9 +1,13 @@
 /*
- * iarray_frame.c -- iarray_container_load / iarray_container_open: a
- * container read from a contiguous Blosc2 frame file (b2frame), the on-disk
- * form the reference writes (src/iarray_container.c:79-218: caterva_open on
- * the file, the "iarray" metalayer for the dtype, the "caterva" metalayer for
- * the shape, chunk shape and block shape).
+ * iarray_frame.c -- containers to and from contiguous Blosc2 frames (b2frame),
+ * the on-disk form the reference reads and writes:
+ *   iarray_container_load / _open (src/iarray_container.c:79-218: caterva_open
+ *     on the file, the "iarray" metalayer for the dtype, the "caterva"
+ *     metalayer for the shape, chunk shape and block shape),
+ *   iarray_from_cframe / iarray_to_cframe (src/iarray_container.c:228-300),
+ *   iarray_container_save (src/iarray_container.c:50-76: a copy of the
+ *     super-chunk into a contiguous frame file),
+ *   iarray_container_remove (src/iarray_container.c:219-225).
  *
  * The frame is read as c-blosc2 lays it out: a msgpack header (a 14-element
  * array: magic "b2frame\0", header size, frame size, flags, nbytes, cbytes,
@@ -20,9 +24,13 @@
  * offsets chunk that is compressed rather than stored, and special-value
  * chunks (negative offsets).
  */
+#include <dirent.h>
+#include <errno.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include "iarray_private.h"
 
@@ -131,6 +139,9 @@ static void mp_skip(mp_t *m) {
 }
 
 /* ---- frame ------------------------------------------------------------------- */
+/* a 14-element array whose first element is the 8-byte string "b2frame\0" */
+static const uint8_t frame_magic[10] = {0x9e, 0xa8, 'b', '2', 'f', 'r', 'a', 'm', 'e', 0};
+
 static ina_rc_t frame_error(const char *what) {
   return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "not a readable Blosc2 frame: %s", what);
 }
@@ -142,18 +153,43 @@ static ina_rc_t read_meta(const uint8_t *d, int64_t hsize, int64_t off, const ui
   return m.bad ? frame_error("metalayer") : INA_SUCCESS;
 }
 
+/* the header fields between chunksize and the metalayers, kept for a later save
+ * when they have the form c-blosc2 writes (two int16 thread counts, a boolean,
+ * a fixext16 of coding parameters) */
+static void read_params(mp_t *m, iax_frame_params_t *fp) {
+  memset(fp, 0, sizeof(*fp));
+  const uint8_t *p0 = m->p;
+  int form = 1;
+  for (int i = 0; i < 2; i++) {
+    form &= m->p < m->end && *m->p == 0xd1;
+    fp->nthreads[i] = mp_int(m);
+  }
+  form &= m->p < m->end && (*m->p == 0xc2 || *m->p == 0xc3);
+  if (m->p < m->end) fp->vlmeta = *m->p;
+  mp_skip(m);
+  form &= m->p + 18 <= m->end && *m->p == 0xd8;
+  if (form) {
+    fp->coding_type = m->p[1];
+    memcpy(fp->coding, m->p + 2, 16);
+  }
+  mp_skip(m);
+  fp->valid = form && !m->bad && m->p - p0 == 3 + 3 + 1 + 18;
+}
+
 static ina_rc_t container_from_frame(iarray_context_t *ctx, const uint8_t *d, int64_t size,
                                      iarray_container_t **container) {
-  static const uint8_t magic[10] = {0x9e, 0xa8, 'b', '2', 'f', 'r', 'a', 'm', 'e', 0};
-  if (size < 64 || memcmp(d, magic, sizeof(magic)) != 0) return frame_error("no b2frame header");
+  if (size < 64 || memcmp(d, frame_magic, sizeof(frame_magic)) != 0) return frame_error("no b2frame header");
   mp_t m = {d + 10, d + size, 0};
   const int64_t hsize = mp_int(&m);
   const int64_t fsize = mp_int(&m);
   int64_t flen;
-  (void)mp_bytes(&m, 0, &flen); /* general flags */
+  const uint8_t *gflags = mp_bytes(&m, 0, &flen); /* general flags */
   const int64_t nbytes = mp_int(&m), cbytes = mp_int(&m);
   const int64_t typesize = mp_int(&m), blocksize = mp_int(&m), chunksize = mp_int(&m);
-  for (int i = 0; i < 4; i++) mp_skip(&m); /* thread counts, a flag, the coding parameters */
+  iax_frame_params_t fp;
+  read_params(&m, &fp); /* thread counts, a flag, the coding parameters */
+  if (gflags && flen == 4) memcpy(fp.flags, gflags, 4);
+  else fp.valid = 0;
   if (m.bad || hsize <= 0 || hsize > size || fsize > size || nbytes < 0 || cbytes < 0 || typesize <= 0 ||
       blocksize <= 0 || chunksize < 0)
     return frame_error("header fields");
@@ -261,13 +297,15 @@ static ina_rc_t container_from_frame(iarray_context_t *ctx, const uint8_t *d, in
     csz[i] = cs;
   }
   if (INA_SUCCEED(rc)) rc = iarray_from_chunks(ctx, &dts, &st, nchunks, ptrs, csz, container);
+  if (INA_SUCCEED(rc)) (*container)->frame = fp;
   free(ptrs);
   free(csz);
   return rc;
 }
 
-static ina_rc_t load(iarray_context_t *ctx, const char *urlpath, iarray_container_t **container) {
+static ina_rc_t load(iarray_context_t *ctx, const char *urlpath, bool on_disk, iarray_container_t **container) {
   if (!ctx || !urlpath || !container) return INA_ERROR(INA_ERR_NULL_POINTER);
+  if (access(urlpath, F_OK) != 0) return iax_set_error(INA_ERROR(INA_ERR_FILE_OPEN), "%s does not exist", urlpath);
   FILE *f = fopen(urlpath, "rb");
   if (!f) return iax_set_error(INA_ERROR(IARRAY_ERR_CATERVA_FAILED), "cannot open %s", urlpath);
   uint8_t *d = NULL;
@@ -282,15 +320,257 @@ static ina_rc_t load(iarray_context_t *ctx, const char *urlpath, iarray_containe
   if (!d) return iax_set_error(INA_ERROR(IARRAY_ERR_CATERVA_FAILED), "cannot read %s", urlpath);
   const ina_rc_t rc = container_from_frame(ctx, d, size, container);
   free(d);
+  if (INA_SUCCEED(rc)) (*container)->on_disk = on_disk;
   return rc;
 }
 
 INA_API(ina_rc_t) iarray_container_load(iarray_context_t *ctx, char *urlpath, iarray_container_t **container) {
-  return load(ctx, urlpath, container);
+  return load(ctx, urlpath, false, container);
 }
 
 /* The reference keeps an opened frame on disk and reads chunks lazily; here
- * the chunks are read at open, as for load. */
+ * the chunks are read at open, as for load, and the container counts as
+ * stored on disk (it cannot be saved). */
 INA_API(ina_rc_t) iarray_container_open(iarray_context_t *ctx, char *urlpath, iarray_container_t **container) {
-  return load(ctx, urlpath, container);
+  return load(ctx, urlpath, true, container);
+}
+
+INA_API(ina_rc_t) iarray_from_cframe(iarray_context_t *ctx, uint8_t *frame, int64_t frame_len, bool copy,
+                                     iarray_container_t **container) {
+  (void)copy; /* the chunks are always copied into the container */
+  if (!ctx || !frame || !container) return INA_ERROR(INA_ERR_NULL_POINTER);
+  return container_from_frame(ctx, frame, frame_len, container);
+}
+
+/* ---- writing a frame ------------------------------------------------------------
+ * The layout the reader above takes, written in c-blosc2's encodings (checked
+ * against the reference's frame files, which a load and a save reproduce byte
+ * for byte):
+ *   header   14-element array: "b2frame\0", int32 header size, uint64 frame
+ *            size, the 4 flag bytes (format version with the 64-bit-offsets
+ *            bit, frame type, codec | clevel << 4, 0), int64 nbytes and
+ *            cbytes, int32 typesize, blocksize and chunksize, int16 thread
+ *            counts, a boolean (no variable-length metalayers), a fixext16 of
+ *            coding parameters (filters[6], codec, codec meta, filters_meta[6],
+ *            2 reserved), then [uint16 index size, map16 {name: int32 offset},
+ *            array16 [bin32 content]] with the caterva metalayer before the
+ *            iarray one; the index size counts from the array's first byte to
+ *            the end of the map
+ *   chunks   back to back from the end of the header
+ *   offsets  a memcpyed Blosc2 chunk of int64 chunk offsets (from the end of
+ *            the header)
+ *   trailer  [1, [uint16 6, {}, []], uint32 trailer size, fixext16 of zeros]
+ */
+static uint8_t *put(uint8_t *p, uint64_t v, int n) {
+  for (int i = n - 1; i >= 0; i--) *p++ = (uint8_t)(v >> (8 * i));
+  return p;
+}
+
+static uint8_t *put_le(uint8_t *p, uint64_t v, int n) {
+  for (int i = 0; i < n; i++) *p++ = (uint8_t)(v >> (8 * i));
+  return p;
+}
+
+static uint8_t *put_str(uint8_t *p, const char *s) {
+  const size_t n = strlen(s);
+  *p++ = (uint8_t)(0xa0 | n);
+  memcpy(p, s, n);
+  return p + n;
+}
+
+static ina_rc_t frame_write(const iarray_container_t *c, uint8_t **out, int64_t *out_len) {
+  if (c->viewed)
+    return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_STORAGE),
+                         "Container must be stored on a blosc schunk and must not be a view");
+  const int64_t n = c->nchunks;
+  int64_t cbytes = 0;
+  for (int64_t i = 0; i < n; i++) {
+    if (!c->chunks[i] || c->csizes[i] < 32) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "chunk %lld is missing", (long long)i);
+    cbytes += c->csizes[i];
+  }
+  if (8 * n > INT32_MAX - 32) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "too many chunks for one frame");
+
+  /* metalayer contents */
+  const int nd = c->dtshape.ndim;
+  uint8_t cat[3 + 3 * (1 + 9 * IARRAY_DIMENSION_MAX)], *q = cat;
+  *q++ = 0x95;
+  *q++ = 0; /* version */
+  *q++ = (uint8_t)nd;
+  *q++ = (uint8_t)(0x90 | nd);
+  for (int i = 0; i < nd; i++) {
+    *q++ = 0xd3;
+    q = put(q, (uint64_t)c->dtshape.shape[i], 8);
+  }
+  const int64_t *tiles[2] = {c->storage.chunkshape, c->storage.blockshape};
+  for (int k = 0; k < 2; k++) {
+    *q++ = (uint8_t)(0x90 | nd);
+    for (int i = 0; i < nd; i++) {
+      *q++ = 0xd2;
+      q = put(q, (uint64_t)(uint32_t)tiles[k][i], 4);
+    }
+  }
+  const int64_t catlen = q - cat;
+  const uint8_t ia[4] = {0x93, IARRAY_METALAYER_VERSION, (uint8_t)c->dtshape.dtype, 0};
+
+  /* header fields the chunks do not fix */
+  iax_frame_params_t fp = c->frame;
+  if (!fp.valid) {
+    memset(&fp, 0, sizeof(fp));
+    const int codec = (int)c->cfg.compression_codec, clevel = c->cfg.compression_level;
+    fp.flags[0] = 0x12; /* frame format 2, 64-bit offsets */
+    fp.flags[2] = (uint8_t)((codec & 0x0f) | (clevel << 4));
+    fp.nthreads[0] = fp.nthreads[1] = c->cfg.max_num_threads;
+    fp.vlmeta = 0xc2;
+    fp.coding_type = 6; /* filter slots */
+    const int f = iax_filter_code(&c->cfg);
+    fp.coding[5] = (uint8_t)(f == IAMX_FILTER_SHUFFLE ? 1 : f == IAMX_FILTER_BITSHUFFLE ? 2 : 0);
+    fp.coding[6] = (uint8_t)codec;
+  }
+
+  const int64_t meta_at = 87, cat_off = meta_at + 32 + 3, ia_off = cat_off + 5 + catlen;
+  const int64_t hsize = ia_off + 5 + (int64_t)sizeof(ia);
+  const int64_t olen = 32 + 8 * n, tlen = 35;
+  const int64_t fsize = hsize + cbytes + olen + tlen;
+  uint8_t *d = (uint8_t *)malloc((size_t)fsize);
+  if (!d) return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+  uint8_t *p = d;
+  memcpy(p, frame_magic, sizeof(frame_magic));
+  p += sizeof(frame_magic);
+  *p++ = 0xd2;
+  p = put(p, (uint64_t)hsize, 4);
+  *p++ = 0xcf;
+  p = put(p, (uint64_t)fsize, 8);
+  *p++ = 0xa4;
+  memcpy(p, fp.flags, 4);
+  p += 4;
+  *p++ = 0xd3;
+  p = put(p, (uint64_t)(n * c->chunk_nbytes), 8);
+  *p++ = 0xd3;
+  p = put(p, (uint64_t)cbytes, 8);
+  const int64_t i32s[3] = {c->itemsize, c->blocksize, c->chunk_nbytes};
+  for (int i = 0; i < 3; i++) {
+    *p++ = 0xd2;
+    p = put(p, (uint64_t)(uint32_t)i32s[i], 4);
+  }
+  for (int i = 0; i < 2; i++) {
+    *p++ = 0xd1;
+    p = put(p, (uint64_t)(uint16_t)fp.nthreads[i], 2);
+  }
+  *p++ = fp.vlmeta;
+  *p++ = 0xd8;
+  *p++ = fp.coding_type;
+  memcpy(p, fp.coding, 16);
+  p += 16;
+  /* metalayers */
+  *p++ = 0x93;
+  *p++ = 0xcd;
+  p = put(p, 32, 2);
+  *p++ = 0xde;
+  p = put(p, 2, 2);
+  p = put_str(p, "caterva");
+  *p++ = 0xd2;
+  p = put(p, (uint64_t)cat_off, 4);
+  p = put_str(p, "iarray");
+  *p++ = 0xd2;
+  p = put(p, (uint64_t)ia_off, 4);
+  *p++ = 0xdc;
+  p = put(p, 2, 2);
+  *p++ = 0xc6;
+  p = put(p, (uint64_t)catlen, 4);
+  memcpy(p, cat, (size_t)catlen);
+  p += catlen;
+  *p++ = 0xc6;
+  p = put(p, sizeof(ia), 4);
+  memcpy(p, ia, sizeof(ia));
+  p += sizeof(ia);
+  /* chunks */
+  uint8_t *offs = d + hsize + cbytes + 32;
+  int64_t at = 0;
+  for (int64_t i = 0; i < n; i++) {
+    memcpy(p, c->chunks[i], (size_t)c->csizes[i]);
+    p += c->csizes[i];
+    put_le(offs + 8 * i, (uint64_t)at, 8);
+    at += c->csizes[i];
+  }
+  /* offsets chunk: version 5, LZ4 format 1, memcpyed + shuffle + extended header flags, typesize 8 */
+  memset(p, 0, 32);
+  p[0] = 5;
+  p[1] = 1;
+  p[2] = 0x07;
+  p[3] = 8;
+  put_le(p + 4, (uint64_t)(8 * n), 4);
+  put_le(p + 8, (uint64_t)(8 * n), 4);
+  put_le(p + 12, (uint64_t)olen, 4);
+  p[21] = 1;
+  p += olen;
+  /* trailer */
+  *p++ = 0x94;
+  *p++ = 0x01;
+  *p++ = 0x93;
+  *p++ = 0xcd;
+  p = put(p, 6, 2);
+  *p++ = 0xde;
+  p = put(p, 0, 2);
+  *p++ = 0xdc;
+  p = put(p, 0, 2);
+  *p++ = 0xce;
+  p = put(p, (uint64_t)tlen, 4);
+  *p++ = 0xd8;
+  *p++ = 0;
+  memset(p, 0, 16);
+  p += 16;
+  if (p - d != fsize) {
+    free(d);
+    return iax_set_error(INA_ERROR(IARRAY_ERR_ASSERTION_FAILED), "frame layout");
+  }
+  *out = d;
+  *out_len = fsize;
+  return INA_SUCCESS;
+}
+
+INA_API(ina_rc_t) iarray_to_cframe(iarray_context_t *ctx, iarray_container_t *src, uint8_t **frame,
+                                   int64_t *frame_len, bool *needs_free) {
+  if (!ctx || !src || !frame || !frame_len || !needs_free) return INA_ERROR(INA_ERR_NULL_POINTER);
+  const ina_rc_t rc = frame_write(src, frame, frame_len);
+  *needs_free = INA_SUCCEED(rc);
+  return rc;
+}
+
+INA_API(ina_rc_t) iarray_container_save(iarray_context_t *ctx, iarray_container_t *container, char *urlpath) {
+  if (!ctx || !container || !urlpath) return INA_ERROR(INA_ERR_NULL_POINTER);
+  if (container->on_disk && !container->viewed) return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_STORAGE), "Container is already on disk");
+  uint8_t *d;
+  int64_t len;
+  ina_rc_t rc = frame_write(container, &d, &len);
+  if (INA_FAILED(rc)) return rc;
+  FILE *f = fopen(urlpath, "wb");
+  int ok = f && fwrite(d, 1, (size_t)len, f) == (size_t)len;
+  if (f && fclose(f) != 0) ok = 0;
+  free(d);
+  if (!ok) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "cannot write %s", urlpath);
+  return INA_SUCCESS;
+}
+
+INA_API(ina_rc_t) iarray_container_remove(char *urlpath) {
+  if (!urlpath) return INA_SUCCESS;
+  struct stat st;
+  if (stat(urlpath, &st) != 0)
+    return errno == ENOENT ? INA_SUCCESS : iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "cannot access %s", urlpath);
+  if (S_ISDIR(st.st_mode)) {
+    /* a frame directory holds its index and chunk files, no subdirectories */
+    DIR *dir = opendir(urlpath);
+    if (!dir) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "cannot open %s", urlpath);
+    struct dirent *e;
+    int ok = 1;
+    char path[4096];
+    while ((e = readdir(dir)) != NULL) {
+      if (strcmp(e->d_name, ".") == 0 || strcmp(e->d_name, "..") == 0) continue;
+      if (snprintf(path, sizeof(path), "%s/%s", urlpath, e->d_name) >= (int)sizeof(path) || remove(path) != 0) ok = 0;
+    }
+    closedir(dir);
+    if (!ok || rmdir(urlpath) != 0) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "cannot remove %s", urlpath);
+    return INA_SUCCESS;
+  }
+  if (remove(urlpath) != 0) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "cannot remove %s", urlpath);
+  return INA_SUCCESS;
 }

@@ -35,6 +35,19 @@ typedef struct iax_plan_s {
   iamx_block_job_t *h_blocks;
 } iax_plan_t;
 
+/* Blosc2 frame header fields that a container's chunks do not determine
+ * (general flags, thread counts, the variable-length-metalayer flag, the
+ * coding parameters).  A container read from a frame keeps them, so saving it
+ * again writes the same header; other containers take them from their config. */
+typedef struct {
+  int valid;
+  uint8_t flags[4];
+  int64_t nthreads[2];
+  uint8_t vlmeta; /* msgpack boolean byte */
+  uint8_t coding_type;
+  uint8_t coding[16];
+} iax_frame_params_t;
+
 struct iarray_container_s {
   iarray_dtshape_t dtshape; /* logical shape (a transposed view swaps it) */
   iarray_storage_t storage;
@@ -60,6 +73,10 @@ struct iarray_container_s {
   /* views */
   iarray_container_t *viewed;
   bool transposed;
+  /* frames: header fields of the frame it was read from; opened (rather than
+   * loaded) containers count as stored on disk */
+  iax_frame_params_t frame;
+  bool on_disk;
   /* Ozaki slice count picked for this container as A against one B: valid
    * while both generations are unchanged */
   const iarray_container_t *tuned_b;
@@ -93,13 +110,13 @@ typedef void (*iax_task_fn)(void *ctx, int64_t i, int worker);
 int iax_nthreads(const iarray_config_t *cfg);
 void iax_parallel_for(int nthreads, int64_t n, iax_task_fn fn, void *ctx);
 double iax_now(void);
+/* Order LZ4 decode jobs by compressed size, largest first. */
+void iax_sort_lz4_jobs(iamx_lz4_job_t *jobs, int64_t n);
 /* Download `total` bytes of d_src into the pinned staging buffer in pieces
  * and run fn(ctx, task, worker) for each of `ntasks` tasks as soon as the
  * piece holding its bytes (up to task_end[task]) has landed, so the copy of
  * later pieces overlaps the work on earlier ones.  *host receives the
  * staging buffer (owned by the device runtime). */
-/* Order LZ4 decode jobs by compressed size, largest first. */
-void iax_sort_lz4_jobs(iamx_lz4_job_t *jobs, int64_t n);
 ina_rc_t iax_download_pipelined(const void *d_src, size_t total, const int64_t *task_end, int64_t ntasks,
                                 int nthreads, iax_task_fn fn, void *ctx, uint8_t **host);
 

@@ -74,6 +74,7 @@ ERR_TOO_SMALL_BUFFER = err_code("TOO_SMALL", "BUFFER")
 ERR_BLOSC_FAILED = err_code("FAILED", "BLOSC")
 ERR_CATERVA_FAILED = err_code("FAILED", "CATERVA")
 ERR_NO_DEVICE = err_code("NOT_SUPPORTED", "DEVICE")
+ERR_FILE_OPEN = (_K["FAILED"] << 32) | 0x05  # libinac's file subject, not a user-defined one
 
 _lib = None
 
@@ -111,6 +112,10 @@ def lib():
         "iarray_opt_gemm_a": ([vp, vp, vp, P(Storage), P(vp)], rc_t),
         "iarray_container_load": ([vp, ctypes.c_char_p, P(vp)], rc_t),
         "iarray_container_open": ([vp, ctypes.c_char_p, P(vp)], rc_t),
+        "iarray_container_save": ([vp, vp, ctypes.c_char_p], rc_t),
+        "iarray_to_cframe": ([vp, vp, P(ctypes.POINTER(ctypes.c_uint8)), P(i64), P(ctypes.c_bool)], rc_t),
+        "iarray_from_cframe": ([vp, vp, i64, ctypes.c_bool, P(vp)], rc_t),
+        "iarray_container_remove": ([ctypes.c_char_p], rc_t),
         "iarray_opt_gemm_b": ([vp, vp, vp, P(Storage), P(vp)], rc_t),
         "iarray_container_chunk": ([vp, i64, P(ctypes.POINTER(ctypes.c_uint8)), P(ctypes.c_int32)], rc_t),
         "iarray_container_nchunks": ([vp, P(i64)], rc_t),
@@ -129,6 +134,17 @@ def lib():
     _lib = L
     _check(L.iarray_init())
     return L
+
+
+def _libc_free(ptr):
+    libc = ctypes.CDLL(None)
+    libc.free.argtypes = [ctypes.c_void_p]
+    libc.free(ctypes.cast(ptr, ctypes.c_void_p))
+
+
+def remove(path):
+    """Deletes a frame file or frame directory (iarray_container_remove)."""
+    _check(lib().iarray_container_remove(os.fsencode(path)))
 
 
 def exported_symbols():
@@ -265,6 +281,33 @@ class Container:
         _check(lib().iarray_get_dtshape(ctx.ptr, p, ctypes.byref(d)))
         inv = {v: k for k, v in _DT.items()}
         return cls(p, ctx, inv[d.dtype], tuple(d.shape[i] for i in range(d.ndim)))
+
+    @classmethod
+    def from_cframe(cls, ctx, frame):
+        """A container read from the bytes of a Blosc2 frame (iarray_from_cframe)."""
+        buf = ctypes.create_string_buffer(bytes(frame), len(frame))
+        p = ctypes.c_void_p()
+        _check(lib().iarray_from_cframe(ctx.ptr, buf, len(frame), True, ctypes.byref(p)))
+        d = DTShape()
+        _check(lib().iarray_get_dtshape(ctx.ptr, p, ctypes.byref(d)))
+        inv = {v: k for k, v in _DT.items()}
+        return cls(p, ctx, inv[d.dtype], tuple(d.shape[i] for i in range(d.ndim)))
+
+    def save(self, path):
+        """Writes the container as a contiguous frame file (iarray_container_save)."""
+        _check(lib().iarray_container_save(self.ctx.ptr, self.ptr, os.fsencode(path)))
+
+    def to_cframe(self):
+        """The container's frame as bytes (iarray_to_cframe)."""
+        buf = ctypes.POINTER(ctypes.c_uint8)()
+        n = ctypes.c_int64()
+        needs_free = ctypes.c_bool()
+        _check(lib().iarray_to_cframe(self.ctx.ptr, self.ptr, ctypes.byref(buf), ctypes.byref(n),
+                                      ctypes.byref(needs_free)))
+        out = ctypes.string_at(buf, n.value)
+        if needs_free.value:
+            _libc_free(buf)
+        return out
 
     def to_numpy(self):
         out = np.empty(self.shape, dtype=self.dtype)

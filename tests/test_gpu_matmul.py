@@ -282,6 +282,37 @@ def test_bitshuffled_gemv_block_shapes(dt, shape, cs, bs):
     assert np.array_equal(y.to_numpy(), (a.astype(np.float64) @ x.astype(np.float64)).astype(dt))
 
 
+@pytest.mark.parametrize("share", ["0", "1"])
+def test_matmul_through_frames(ctx, tmp_path, monkeypatch, share):
+    """The reference's persistent gemm fixtures (test_linalg_gemm.c:227-248,
+    x, y and z stored as frames): the operands are saved and loaded back,
+    multiplied, and the product goes through a frame file and an in-memory
+    frame; every copy decodes to the product of the in-memory operands, with
+    C's chunks written by the host encoder (share 0) or the GPU encoder
+    (share 1)."""
+    monkeypatch.setenv("IARRAY_GPU_ENCODE", share)
+    name, dt, (xs, xcs, xbs), (ys, ycs, ybs), (zcs, zbs) = GEMM_FIXTURES[2]
+    print(f"case: frames {name} share {share}", flush=True)
+    x = _mk(ctx, "arange", dt, xs, xcs, xbs)
+    y = _mk(ctx, "arange", dt, ys, ycs, ybs)
+    x.save(str(tmp_path / "xarr.iarray"))
+    y.save(str(tmp_path / "yarr.iarray"))
+    xl = ia.Container.load(ctx, str(tmp_path / "xarr.iarray"))
+    yl = ia.Container.load(ctx, str(tmp_path / "yarr.iarray"), lazy=True)
+    assert xl.chunks() == x.chunks() and yl.chunks() == y.chunks()
+    z = ia.matmul(x, y, zcs, zbs).to_numpy()
+    zc = ia.matmul(xl, yl, zcs, zbs)
+    zc.save(str(tmp_path / "zarr.iarray"))
+    zl = ia.Container.load(ctx, str(tmp_path / "zarr.iarray"))
+    zf = ia.Container.from_cframe(ctx, zc.to_cframe())
+    for got in (zc, zl, zf):
+        assert got.to_numpy().tobytes() == z.tobytes()
+    o = np.arange(np.prod(xs), dtype=np.float64).reshape(xs) @ np.arange(np.prod(ys), dtype=np.float64).reshape(ys)
+    _rel_ok(z, o, TOL[np.dtype(dt)])
+    ia.remove(str(tmp_path / "zarr.iarray"))
+    assert not (tmp_path / "zarr.iarray").exists()
+
+
 def test_matmul_deterministic(ctx):
     rng = np.random.default_rng(2)
     a = rng.standard_normal((400, 300))

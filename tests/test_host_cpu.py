@@ -269,7 +269,7 @@ def test_frame_load_reads_fixture(orc, path):
 
 def test_frame_load_errors(tmp_path):
     ctx = _ctx()
-    assert _err(lambda: ia.Container.load(ctx, str(tmp_path / "missing.iarray"))) == ia.ERR_CATERVA_FAILED
+    assert _err(lambda: ia.Container.load(ctx, str(tmp_path / "missing.iarray"))) == ia.ERR_FILE_OPEN
     bad = tmp_path / "bad.iarray"
     bad.write_bytes(b"not a frame" * 20)
     assert _err(lambda: ia.Container.load(ctx, str(bad))) == ia.ERR_BLOSC_FAILED
@@ -304,6 +304,74 @@ def test_frame_load_survives_damage(tmp_path):
         except ia.IArrayError:
             outcomes.add("error")
     assert "error" in outcomes
+
+
+@pytest.mark.parametrize("path", FRAMES, ids=os.path.basename)
+def test_frame_save_rewrites_fixture(path, tmp_path):
+    """A frame the reference wrote, loaded and saved again, is the same file
+    byte for byte (header, metalayers, chunks, offsets chunk, trailer); so is
+    the frame of a container built from the same chunks with the writer's
+    settings (LZ4 clevel 5, shuffle, 8 threads), which pins the header fields
+    a new container takes from its config."""
+    data = open(path, "rb").read()
+    c = ia.Container.load(_ctx(), path)
+    out = tmp_path / "saved.iarray"
+    c.save(str(out))
+    assert out.read_bytes() == data
+    assert c.to_cframe() == data
+    ctx8 = ia.Context(clevel=5, filters=ia.COMP_SHUFFLE, nthreads=8)
+    fresh = ia.Container.from_chunks(ctx8, c.dtype, c.shape, (100000,), (10000,), c.chunks())
+    assert fresh.to_cframe() == data
+    back = ia.Container.from_cframe(_ctx(), data)
+    assert back.shape == c.shape and back.dtype == c.dtype and back.chunks() == c.chunks()
+
+
+def test_frame_save_two_dimensional(tmp_path):
+    """A 2-D container goes through a frame and back with its shape, tiling
+    and chunks; saving the loaded copy writes the same bytes again."""
+    ctx = _ctx()
+    src = ia.Container.load(ctx, FRAMES[0])
+    c = ia.Container.from_chunks(ctx, src.dtype, (10, 100000), (1, 100000), (1, 10000), src.chunks())
+    f1 = tmp_path / "a.iarray"
+    c.save(str(f1))
+    d = ia.Container.load(ctx, str(f1))
+    assert d.shape == (10, 100000) and d.dtype == src.dtype and d.chunks() == src.chunks()
+    f2 = tmp_path / "b.iarray"
+    d.save(str(f2))
+    assert f2.read_bytes() == f1.read_bytes()
+    # the caterva metalayer: [version, ndim, shape (int64), chunkshape, blockshape (int32)]
+    raw = f1.read_bytes()
+    cat = bytes([0x95, 0, 2, 0x92, 0xd3]) + (10).to_bytes(8, "big") + b"\xd3" + (100000).to_bytes(8, "big")
+    cat += bytes([0x92, 0xd2]) + (1).to_bytes(4, "big") + b"\xd2" + (100000).to_bytes(4, "big")
+    cat += bytes([0x92, 0xd2]) + (1).to_bytes(4, "big") + b"\xd2" + (10000).to_bytes(4, "big")
+    assert raw.find(cat) > 0
+
+
+def test_frame_save_errors_and_remove(tmp_path):
+    """iarray_container_save refuses views and opened containers
+    (src/iarray_container.c:57-66); iarray_container_remove deletes files and
+    frame directories and accepts a path that does not exist."""
+    ctx = _ctx()
+    src = ia.Container.load(ctx, FRAMES[0])
+    c = ia.Container.from_chunks(ctx, src.dtype, (10, 100000), (1, 100000), (1, 10000), src.chunks())
+    t = c.transpose()
+    assert _err(lambda: t.save(str(tmp_path / "t.iarray"))) == ia.ERR_INVALID_STORAGE
+    assert _err(lambda: t.to_cframe()) == ia.ERR_INVALID_STORAGE
+    opened = ia.Container.load(ctx, FRAMES[0], lazy=True)
+    assert _err(lambda: opened.save(str(tmp_path / "o.iarray"))) == ia.ERR_INVALID_STORAGE
+    f = tmp_path / "s.iarray"
+    src.save(str(f))
+    assert f.exists()
+    ia.remove(str(f))
+    assert not f.exists()
+    ia.remove(str(f))
+    d = tmp_path / "dir.iarr"
+    d.mkdir()
+    (d / "chunks.b2frame").write_bytes(b"x")
+    (d / "00000000.chunk").write_bytes(b"y")
+    ia.remove(str(d))
+    assert not d.exists()
+    assert _err(lambda: src.save(str(tmp_path / "no" / "such" / "dir.iarray"))) == ia.ERR_BLOSC_FAILED
 
 
 def test_constructor_errors():
