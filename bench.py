@@ -209,6 +209,11 @@ def main():
             alg += li + lo
         alg /= per_step
         bound, unit, peak, achieved = "hbm", "GB/s", HBM_PEAK_GBS, alg / avg_s / 1e9
+    elif dname.startswith("lz4_encode"):
+        # the GPU share of the C write: filtered stream bytes read + finished chunk bytes written
+        gin, gout = ia.last_gpu_encode_bytes()
+        alg = (gin + gout) / per_step
+        bound, unit, peak, achieved = "hbm", "GB/s", HBM_PEAK_GBS, alg / avg_s / 1e9
     elif dname.startswith("gemv_fused"):
         lz_in, lz_out, raw = stream_bytes(cA.chunks())
         alg = (lz_out + raw + K * isz + (M // 250) * M * isz) / per_step

@@ -201,6 +201,19 @@ IAMX_API int64_t iamx_encode_scratch_bytes(const iamx_tiling_t *tiling, int spli
 IAMX_API int iamx_encode_chunks(const iamx_tiling_t *tiling, int split, const void *d_packed, int64_t chunk_begin,
                                 int64_t chunk_end, void *d_scratch, int32_t *d_meta, void *d_out,
                                 int64_t *d_chunk_size);
+/* The same stage for the outputs of many products at once (the C write of a
+ * batch of iarray_linalg_matmul calls, src/linalg/matmul/gemm.c:216-243 per
+ * product): one launch of each encoder kernel covers every chunk of the n
+ * tilings.  Tiling i's packed blocks start at d_packed + packed_off[i]; its
+ * chunk c is written at d_out + out_off[i] + c * iamx_chunk_capacity(tiling i)
+ * (out_off 16-byte aligned) and its size stored in d_chunk_size[first + c],
+ * first = the chunk count of tilings 0..i-1.  d_scratch holds
+ * iamx_encode_batch_scratch_bytes bytes; the segment table is copied into it
+ * synchronously before the kernels are queued on the current stream. */
+IAMX_API int64_t iamx_encode_batch_scratch_bytes(int n, const iamx_tiling_t *tilings, const int *split);
+IAMX_API int iamx_encode_batch(int n, const iamx_tiling_t *tilings, const int *split, const void *d_packed,
+                               const int64_t *packed_off, void *d_scratch, void *d_out, const int64_t *out_off,
+                               int64_t *d_chunk_size);
 
 /* Transpose a dense row-major (rows x cols) matrix: the omatcopy of a
  * transposed view (src/iarray_container.c:718-723). */

@@ -76,6 +76,11 @@ INA_API(bool) iarray_prof_entry(int i, const char **name, int64_t *launches, dou
  * the last matmul, and device milliseconds from the first to the last kernel. */
 INA_API(void) iarray_last_matmul_times(double *host_encode_s, double *device_ms);
 
+/* Bytes of filtered C streams the GPU encoder read, and bytes of finished
+ * chunks it wrote, in the last matmul's write stage (0 when every chunk was
+ * encoded on the host). */
+INA_API(void) iarray_last_gpu_encode_bytes(int64_t *stream_bytes, int64_t *chunk_bytes);
+
 /* Slice count used by the last OZAKI_INT8 product (0 if none ran) and, when
  * it was picked automatically (ozaki_slices = 0), the componentwise error
  * max |C_S - C| / (|A||B|) measured on the sampled rows (else -1).  The

@@ -8,8 +8,8 @@
 //                  the sub-block.  Candidates are found 64 positions at a
 //                  time: each lane hashes the 4 bytes at its position and
 //                  reads the hash table (LDS, 2^10 entries holding a position
-//                  + 1 and the 4 bytes found there); when those bytes are
-//                  equal it also compares the next 16 bytes at both places.
+//                  + 1); when the 4 bytes there are equal to its own it also
+//                  compares the next 16 bytes at both places.
 //                  Then every lane inserts its position with an atomic max,
 //                  so the latest position of a hash wins whatever the lane
 //                  order, and a candidate always comes from an earlier step.
@@ -36,6 +36,10 @@
 // Chunk c is assembled at c * cap of the output; stream E (chunk-major,
 // block-major, stream-minor) has an output slot at E * (neblock rounded up to
 // 16) and two int32 of metadata (csize, then offset in its chunk).
+#include <string.h>
+
+#include <vector>
+
 #include "iamx_internal.h"
 
 namespace {
@@ -95,13 +99,12 @@ constexpr int kSubSlot = kSub + 256;  // a sub-block's worst output, rounded
 constexpr int kInfo = 8;              // int32 per info record
 enum { I_HAS = 0, I_POS, I_OFF, I_ML, I_LEN, I_TRAIL, I_NZ, I_FAIL };
 
-__global__ __launch_bounds__(64) void k_lz4_encode_sub(EncGeom G, const uint8_t *__restrict__ packed, int64_t e0,
-                                                       int64_t nsub, int32_t K, uint8_t *__restrict__ subslots,
-                                                       int32_t *__restrict__ info) {
-  __shared__ uint64_t ht[1 << kSubHlog];
+// Sub-block sbid of the call (stream e0 + sbid / K, sub-block sbid % K).
+__device__ __forceinline__ void encode_sub(const EncGeom &G, const uint8_t *__restrict__ packed, int64_t e0,
+                                           int64_t sbid, int32_t K, uint8_t *__restrict__ subslots,
+                                           int32_t *__restrict__ info) {
+  __shared__ uint32_t ht[1 << kSubHlog];  // latest position + 1 of each hash
   __shared__ __align__(16) uint32_t stage[kSub / 4 + 16];
-  const int64_t sbid = blockIdx.x;
-  if (sbid >= nsub) return;
   const int lane = threadIdx.x;
   const int64_t E = e0 + sbid / K;
   const int32_t k = (int32_t)(sbid % K);
@@ -171,10 +174,9 @@ __global__ __launch_bounds__(64) void k_lz4_encode_sub(EncGeom G, const uint8_t 
     for (int t = 0; t < 5; t++) x[t] = word_at(lp + 4 * t);
     const uint32_t seq = x[0];
     const uint32_t h = (seq * 2654435761u) >> (32 - kSubHlog);
-    const uint64_t ent = act ? ht[h] : 0ull;
+    const int32_t cand = act ? (int32_t)ht[h] - 1 : -1;
     __builtin_amdgcn_wave_barrier();
-    const int32_t cand = (int32_t)(ent >> 32) - 1;
-    const bool ok = act && cand >= 0 && (uint32_t)ent == seq;
+    const bool ok = cand >= 0 && word_at(cand) == seq;
     int32_t mlen = 0;
     if (ok) {
       int32_t ext = 16;
@@ -185,7 +187,7 @@ __global__ __launch_bounds__(64) void k_lz4_encode_sub(EncGeom G, const uint8_t 
       }
       mlen = 4 + ext;
     }
-    if (act) atomicMax(&ht[h], ((uint64_t)(uint32_t)(pos + 1) << 32) | seq);
+    if (act) atomicMax(&ht[h], (uint32_t)(pos + 1));
     __builtin_amdgcn_wave_barrier();
     uint64_t mask = __ballot(ok);
     if (ip > base) mask = ip - base >= 64 ? 0ull : mask & (~0ull << (ip - base));
@@ -262,15 +264,15 @@ __global__ __launch_bounds__(64) void k_lz4_encode_sub(EncGeom G, const uint8_t 
 }
 
 // Merge: one wavefront per stream joins its sub-blocks into one LZ4 block in
-// the stream's slot (or records a zero / raw stream).
-__global__ __launch_bounds__(64) void k_lz4_merge(EncGeom G, const uint8_t *__restrict__ packed, int64_t e0,
-                                                  int njobs, int32_t K, const uint8_t *__restrict__ subslots,
-                                                  const int32_t *__restrict__ info, uint8_t *__restrict__ slots,
-                                                  int32_t *__restrict__ meta) {
-  if ((int)blockIdx.x >= njobs) return;
+// the stream's slot (or records a zero / raw stream).  Stream j of the call is
+// stream e0 + j.
+__device__ __forceinline__ void merge_stream(const EncGeom &G, const uint8_t *__restrict__ packed, int64_t e0,
+                                             int64_t j, int32_t K, const uint8_t *__restrict__ subslots,
+                                             const int32_t *__restrict__ info, uint8_t *__restrict__ slots,
+                                             int32_t *__restrict__ meta) {
   const int lane = threadIdx.x;
-  const int64_t E = e0 + blockIdx.x;
-  const int64_t sb0 = (int64_t)blockIdx.x * K;  // first sub-block record of this stream
+  const int64_t E = e0 + j;
+  const int64_t sb0 = j * K;  // first sub-block record of this stream
   const int64_t per_chunk = (int64_t)G.nblocks * G.nstreams;
   const int64_t c = E / per_chunk, r = E % per_chunk;
   const int64_t b = r / G.nstreams, s = r % G.nstreams;
@@ -330,10 +332,10 @@ __global__ __launch_bounds__(64) void k_lz4_merge(EncGeom G, const uint8_t *__re
   if (lane == 0) meta[2 * E] = cs;
 }
 
-__global__ __launch_bounds__(256) void k_chunk_offsets(EncGeom G, int64_t c0, int32_t *__restrict__ meta,
-                                                       uint8_t *__restrict__ out, int64_t *__restrict__ csizes) {
+// Chunk c: its streams' offsets, header and bstarts.
+__device__ __forceinline__ void chunk_offsets(const EncGeom &G, int64_t c, int32_t *__restrict__ meta,
+                                              uint8_t *__restrict__ out, int64_t *__restrict__ csizes) {
   __shared__ int32_t part[256];
-  const int64_t c = c0 + blockIdx.x;
   const int tid = threadIdx.x;
   const int32_t ne = G.nblocks * G.nstreams;
   const int64_t E0 = c * ne;
@@ -375,10 +377,10 @@ __global__ __launch_bounds__(256) void k_chunk_offsets(EncGeom G, int64_t c0, in
   }
 }
 
-__global__ __launch_bounds__(256) void k_chunk_copy(EncGeom G, int64_t e0, const uint8_t *__restrict__ packed,
-                                                    const uint8_t *__restrict__ slots,
-                                                    const int32_t *__restrict__ meta, uint8_t *__restrict__ out) {
-  const int64_t E = e0 + blockIdx.x;
+// Stream E: its csize word and bytes into its chunk.
+__device__ __forceinline__ void chunk_copy(const EncGeom &G, int64_t E, const uint8_t *__restrict__ packed,
+                                           const uint8_t *__restrict__ slots, const int32_t *__restrict__ meta,
+                                           uint8_t *__restrict__ out) {
   const int64_t per_chunk = (int64_t)G.nblocks * G.nstreams;
   const int64_t c = E / per_chunk, r = E % per_chunk;
   const int64_t b = r / G.nstreams, s = r % G.nstreams;
@@ -389,6 +391,101 @@ __global__ __launch_bounds__(256) void k_chunk_copy(EncGeom G, int64_t e0, const
   const uint8_t *sp = cs == G.neblock ? packed + c * G.chunk_nbytes + b * G.blocksize + s * (int64_t)G.neblock
                                       : slots + E * G.slot;
   wg_copy(d + 4, gptr<const uint8_t>((uint64_t)sp), cs, threadIdx.x, blockDim.x);
+}
+
+// ---- one tiling per call --------------------------------------------------------
+__global__ __launch_bounds__(64) void k_lz4_encode_sub(EncGeom G, const uint8_t *__restrict__ packed, int64_t e0,
+                                                       int64_t nsub, int32_t K, uint8_t *__restrict__ subslots,
+                                                       int32_t *__restrict__ info) {
+  if ((int64_t)blockIdx.x >= nsub) return;
+  encode_sub(G, packed, e0, blockIdx.x, K, subslots, info);
+}
+
+__global__ __launch_bounds__(64) void k_lz4_merge(EncGeom G, const uint8_t *__restrict__ packed, int64_t e0,
+                                                  int njobs, int32_t K, const uint8_t *__restrict__ subslots,
+                                                  const int32_t *__restrict__ info, uint8_t *__restrict__ slots,
+                                                  int32_t *__restrict__ meta) {
+  if ((int)blockIdx.x >= njobs) return;
+  merge_stream(G, packed, e0, blockIdx.x, K, subslots, info, slots, meta);
+}
+
+__global__ __launch_bounds__(256) void k_chunk_offsets(EncGeom G, int64_t c0, int32_t *__restrict__ meta,
+                                                       uint8_t *__restrict__ out, int64_t *__restrict__ csizes) {
+  chunk_offsets(G, c0 + blockIdx.x, meta, out, csizes);
+}
+
+__global__ __launch_bounds__(256) void k_chunk_copy(EncGeom G, int64_t e0, const uint8_t *__restrict__ packed,
+                                                    const uint8_t *__restrict__ slots,
+                                                    const int32_t *__restrict__ meta, uint8_t *__restrict__ out) {
+  chunk_copy(G, e0 + blockIdx.x, packed, slots, meta, out);
+}
+
+// ---- many tilings per call ------------------------------------------------------
+// Segment i of a batch: one tiling's chunks.  Its sub-blocks, streams and
+// chunks are numbered after those of segments 0..i-1 (sub0, st0, ch0); every
+// buffer it touches starts at its own offset, so the one-tiling code above
+// runs unchanged on segment-local indexes.
+struct EncSeg {
+  EncGeom G;
+  int32_t K;           // sub-blocks per stream
+  int32_t pad;
+  int64_t sub0, st0, ch0;
+  int64_t packed_off;  // bytes: its packed blocks in the packed buffer
+  int64_t out_off;     // bytes: its chunks in the output (chunk c at out_off + c * G.cap)
+  int64_t slot_off;    // bytes: its stream slots in the scratch
+};
+
+// the segment whose range of field F (0 sub-blocks, 1 streams, 2 chunks) holds i
+template <int F>
+__device__ __forceinline__ int seg_of(const EncSeg *__restrict__ S, int n, int64_t i) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    const int64_t v = F == 0 ? S[mid].sub0 : F == 1 ? S[mid].st0 : S[mid].ch0;
+    if (v <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(64) void k_lz4_encode_sub_batch(const EncSeg *__restrict__ S, int n,
+                                                             const uint8_t *__restrict__ packed,
+                                                             uint8_t *__restrict__ subslots,
+                                                             int32_t *__restrict__ info) {
+  const EncSeg &q = S[seg_of<0>(S, n, blockIdx.x)];
+  const EncGeom G = q.G;
+  encode_sub(G, packed + q.packed_off, 0, (int64_t)blockIdx.x - q.sub0, q.K, subslots + q.sub0 * kSubSlot,
+             info + q.sub0 * kInfo);
+}
+
+__global__ __launch_bounds__(64) void k_lz4_merge_batch(const EncSeg *__restrict__ S, int n,
+                                                        const uint8_t *__restrict__ packed,
+                                                        const uint8_t *__restrict__ subslots,
+                                                        const int32_t *__restrict__ info,
+                                                        uint8_t *__restrict__ slots, int32_t *__restrict__ meta) {
+  const EncSeg &q = S[seg_of<1>(S, n, blockIdx.x)];
+  const EncGeom G = q.G;
+  merge_stream(G, packed + q.packed_off, 0, (int64_t)blockIdx.x - q.st0, q.K, subslots + q.sub0 * kSubSlot,
+               info + q.sub0 * kInfo, slots + q.slot_off, meta + 2 * q.st0);
+}
+
+__global__ __launch_bounds__(256) void k_chunk_offsets_batch(const EncSeg *__restrict__ S, int n,
+                                                             int32_t *__restrict__ meta, uint8_t *__restrict__ out,
+                                                             int64_t *__restrict__ csizes) {
+  const EncSeg &q = S[seg_of<2>(S, n, blockIdx.x)];
+  const EncGeom G = q.G;
+  chunk_offsets(G, (int64_t)blockIdx.x - q.ch0, meta + 2 * q.st0, out + q.out_off, csizes + q.ch0);
+}
+
+__global__ __launch_bounds__(256) void k_chunk_copy_batch(const EncSeg *__restrict__ S, int n,
+                                                          const uint8_t *__restrict__ packed,
+                                                          const uint8_t *__restrict__ slots,
+                                                          const int32_t *__restrict__ meta,
+                                                          uint8_t *__restrict__ out) {
+  const EncSeg &q = S[seg_of<1>(S, n, blockIdx.x)];
+  const EncGeom G = q.G;
+  chunk_copy(G, (int64_t)blockIdx.x - q.st0, packed + q.packed_off, slots + q.slot_off, meta + 2 * q.st0,
+             out + q.out_off);
 }
 
 bool geom_of(const iamx_tiling_t *t, int split, EncGeom *G) {
@@ -425,9 +522,91 @@ ScratchLayout layout_of(const iamx_tiling_t *t, const EncGeom &G) {
   return L;
 }
 
+// batch scratch: segment table, stream metadata, stream slots, sub-block slots, info records
+struct BatchLayout {
+  int64_t nsub, nst, nch, meta_off, slots_off, sub_off, info_off, bytes;
+};
+
+bool batch_layout(int n, const iamx_tiling_t *tilings, const int *split, EncSeg *segs, BatchLayout *B) {
+  if (n < 1 || !tilings || !split) return false;
+  int64_t sub = 0, st = 0, ch = 0, slot = 0;
+  for (int i = 0; i < n; i++) {
+    EncSeg q;
+    memset(&q, 0, sizeof(q));
+    if (!geom_of(&tilings[i], split[i], &q.G)) return false;
+    const iamx_tiling_t *t = &tilings[i];
+    const int64_t nch = t->nchunks[0] * t->nchunks[1];
+    if (nch < 1) return false;
+    const int64_t nst = nch * q.G.nblocks * q.G.nstreams;
+    q.K = (q.G.neblock + kSub - 1) / kSub;
+    q.sub0 = sub;
+    q.st0 = st;
+    q.ch0 = ch;
+    q.slot_off = slot;
+    if (segs) segs[i] = q;
+    sub += nst * q.K;
+    st += nst;
+    ch += nch;
+    slot += nst * q.G.slot;
+  }
+  if (sub > INT32_MAX || st > INT32_MAX || ch > INT32_MAX) return false;
+  B->nsub = sub;
+  B->nst = st;
+  B->nch = ch;
+  auto up = [](int64_t v) { return (v + 255) & ~255ll; };
+  B->meta_off = up((int64_t)n * (int64_t)sizeof(EncSeg));
+  B->slots_off = B->meta_off + up(st * 8);
+  B->sub_off = B->slots_off + up(slot);
+  B->info_off = B->sub_off + sub * kSubSlot;
+  B->bytes = B->info_off + sub * kInfo * 4 + 256;
+  return true;
+}
+
 }  // namespace
 
 extern "C" {
+
+IAMX_API int64_t iamx_encode_batch_scratch_bytes(int n, const iamx_tiling_t *tilings, const int *split) {
+  BatchLayout B;
+  return batch_layout(n, tilings, split, nullptr, &B) ? B.bytes : -1;
+}
+
+IAMX_API int iamx_encode_batch(int n, const iamx_tiling_t *tilings, const int *split, const void *d_packed,
+                               const int64_t *packed_off, void *d_scratch, void *d_out, const int64_t *out_off,
+                               int64_t *d_chunk_size) {
+  int rc = iamx::ensure_init();
+  if (rc) return rc;
+  if (!d_packed || !packed_off || !d_scratch || !d_out || !out_off || !d_chunk_size)
+    return iamx::set_error(IAMX_EINVAL, "iamx_encode_batch: null buffer");
+  if (n < 1 || n > (1 << 20)) return iamx::set_error(IAMX_EINVAL, "iamx_encode_batch: %d tilings", n);
+  std::vector<EncSeg> segs((size_t)n);
+  BatchLayout B;
+  if (!batch_layout(n, tilings, split, segs.data(), &B))
+    return iamx::set_error(IAMX_EINVAL, "iamx_encode_batch: bad tiling or too many streams");
+  for (int i = 0; i < n; i++) {
+    if (packed_off[i] < 0 || out_off[i] < 0 || (out_off[i] & 15))
+      return iamx::set_error(IAMX_EINVAL, "iamx_encode_batch: offsets of tiling %d", i);
+    segs[(size_t)i].packed_off = packed_off[i];
+    segs[(size_t)i].out_off = out_off[i];
+  }
+  uint8_t *scr = (uint8_t *)d_scratch;
+  rc = iamx_h2d(scr, segs.data(), sizeof(EncSeg) * (size_t)n);
+  if (rc) return rc;
+  const EncSeg *d_segs = (const EncSeg *)scr;
+  int32_t *meta = (int32_t *)(scr + B.meta_off);
+  uint8_t *slots = scr + B.slots_off, *subslots = scr + B.sub_off;
+  int32_t *info = (int32_t *)(scr + B.info_off);
+  const uint8_t *pk = (const uint8_t *)d_packed;
+  IAMX_LAUNCH("lz4_encode_batch", k_lz4_encode_sub_batch, dim3((unsigned)B.nsub), dim3(64), 0, d_segs, n, pk,
+              subslots, info);
+  IAMX_LAUNCH("lz4_merge_batch", k_lz4_merge_batch, dim3((unsigned)B.nst), dim3(64), 0, d_segs, n, pk,
+              (const uint8_t *)subslots, (const int32_t *)info, slots, meta);
+  IAMX_LAUNCH("chunk_offsets_batch", k_chunk_offsets_batch, dim3((unsigned)B.nch), dim3(256), 0, d_segs, n, meta,
+              (uint8_t *)d_out, d_chunk_size);
+  IAMX_LAUNCH("chunk_copy_batch", k_chunk_copy_batch, dim3((unsigned)B.nst), dim3(256), 0, d_segs, n, pk,
+              (const uint8_t *)slots, (const int32_t *)meta, (uint8_t *)d_out);
+  return IAMX_OK;
+}
 
 IAMX_API int64_t iamx_chunk_capacity(const iamx_tiling_t *t, int split) {
   EncGeom G;

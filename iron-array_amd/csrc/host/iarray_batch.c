@@ -14,14 +14,23 @@
  *     work-queue GEMM;
  *   - every C block goes into ONE pack table;
  * and the launch sequence (decode, queue reset, GEMM, pack) is captured into
- * a hipGraph.  A call with the same signature replays the graph, copies the
- * packed C blocks back and LZ4-encodes all C chunks on the host threads.
+ * a hipGraph.  A call with the same signature replays the graph; then a
+ * suffix of the products is LZ4-encoded and assembled into chunks on the GPU
+ * (one encoder pass per group of products) while the host threads encode the
+ * others from their downloaded packed blocks.
  * Plain precision only; the mixed forms run product by product.
  */
+#include <stdatomic.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include "iarray_private.h"
+
+/* Default share of a batch's output bytes encoded on the GPU (outputs of 128
+ * MiB or more).  Measured on 512 ragged float32 products (1.3 GB of C, 16
+ * host threads), ms per call at shares 0 / 0.3 / 0.5 / 0.6 / 0.7 / 0.8 / 1:
+ * 99 / 73 / 63-72 / 62 / 60-61 / 65 / 70. */
+#define IAX_BATCH_GPU_SHARE 0.65
 
 typedef struct {
   int n;
@@ -39,6 +48,7 @@ typedef struct {
   iamx_batch_pack_t *d_packs;
   int64_t packed_bytes;
   int64_t *out_off; /* per product: byte offset of its packed blocks */
+  iamx_tiling_t *tilings; /* per product: C's tiling (for the GPU encoder) */
   /* the captured launches' arguments, for an eager replay */
   int n_lz4, n_blk, ntiles, npb;
   int64_t b_base; /* byte offset of the B images in d_dense */
@@ -56,6 +66,7 @@ static void free_batch_plan(batch_plan_t *p) {
   free(p->gen);
   free(p->store);
   free(p->out_off);
+  free(p->tilings);
   memset(p, 0, sizeof(*p));
 }
 
@@ -124,9 +135,10 @@ static ina_rc_t build_plan(batch_plan_t *p, int n, iarray_container_t **a, iarra
   p->gen = (uint64_t *)malloc(sizeof(uint64_t) * 2 * (size_t)n);
   p->store = (int64_t *)malloc(sizeof(int64_t) * 4 * (size_t)n);
   p->out_off = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+  p->tilings = (iamx_tiling_t *)calloc((size_t)n, sizeof(iamx_tiling_t));
   iamx_batch_prod_t *prods = (iamx_batch_prod_t *)calloc((size_t)n, sizeof(iamx_batch_prod_t));
   iamx_batch_pack_t *packs = (iamx_batch_pack_t *)calloc((size_t)n, sizeof(iamx_batch_pack_t));
-  if (!p->a || !p->b || !p->gen || !p->store || !p->out_off || !prods || !packs) {
+  if (!p->a || !p->b || !p->gen || !p->store || !p->out_off || !p->tilings || !prods || !packs) {
     free(prods);
     free(packs);
     return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
@@ -154,6 +166,7 @@ static ina_rc_t build_plan(batch_plan_t *p, int n, iarray_container_t **a, iarra
     }
     t->typesize = cc->itemsize;
     t->filter = cc->cfg.compression_level > 0 ? iax_filter_code(&cc->cfg) : IAMX_FILTER_NONE;
+    p->tilings[i] = *t;
     packs[i].c_off_bytes = prods[i].c_off * is;
     packs[i].out_off = packed;
     p->out_off[i] = packed;
@@ -244,32 +257,223 @@ static ina_rc_t build_plan(batch_plan_t *p, int n, iarray_container_t **a, iarra
   return rc;
 }
 
+/* The C write of a batch.  Products [0, hsplit) are LZ4-encoded on the host
+ * threads from their packed blocks; products [hsplit, n) are encoded and
+ * assembled into chunks on the GPU (on the side stream, one encoder pass per
+ * group of products) and only copied into place on the host.  Every
+ * download is queued up front on the copy stream, host pieces first; the
+ * host threads take each chunk as soon as its piece or group has landed.
+ * Staging: the host products' packed bytes at their packed offsets, then,
+ * from gbase, the GPU products' chunks at gout_off[i] + chunk * cap[i]. */
 typedef struct {
   iarray_container_t **c;
-  const int64_t *first; /* prefix sums of chunk counts */
-  int n;
-  uint8_t *packed;
-  const int64_t *out_off;
-  int err;
+  const int64_t *first; /* prefix sums of chunk counts, n + 1 entries */
+  int n, hsplit;
+  const uint8_t *h;
+  const int64_t *out_off, *gout_off, *cap;
+  int64_t gbase;
+  const int64_t *pfirst; /* piece k covers global chunks [pfirst[k], pfirst[k + 1]) */
+  int npieces;
+  int filtered;
+  atomic_int landed[IAMX_COPY_EVENTS];
+  atomic_int err;
 } wctx_t;
+
+static int find_le(const int64_t *v, int n, int64_t g) { /* last index i < n with v[i] <= g */
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (v[mid] <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
 
 static void write_task(void *vctx, int64_t g, int worker) {
   (void)worker;
   wctx_t *x = (wctx_t *)vctx;
-  int lo = 0, hi = x->n - 1; /* product holding global chunk g */
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) / 2;
-    if (x->first[mid] <= g) lo = mid;
-    else hi = mid - 1;
+  const int k = find_le(x->pfirst, x->npieces, g);
+  if (!atomic_load(&x->landed[k])) {
+    if (iamx_event_sync(k) != 0) {
+      atomic_store(&x->err, 1);
+      return;
+    }
+    atomic_store(&x->landed[k], 1);
   }
-  iarray_container_t *c = x->c[lo];
-  const int64_t nc = g - x->first[lo];
-  const int filtered = c->cfg.compression_level > 0;
+  const int i = find_le(x->first, x->n, g);
+  iarray_container_t *c = x->c[i];
+  const int64_t nc = g - x->first[i];
   free(c->chunks[nc]);
   c->chunks[nc] = NULL;
-  if (iax_encode_chunk(x->packed + x->out_off[lo] + nc * c->chunk_nbytes, (int32_t)c->chunk_nbytes, c->blocksize,
-                       c->itemsize, &c->cfg, filtered, &c->chunks[nc], &c->csizes[nc]) != 0)
-    x->err = 1;
+  if (i >= x->hsplit) {
+    const uint8_t *src = x->h + x->gbase + x->gout_off[i] + nc * x->cap[i];
+    int32_t sz;
+    memcpy(&sz, src + 12, 4); /* cbytes of the assembled chunk */
+    uint8_t *q = sz > 32 && sz <= x->cap[i] ? (uint8_t *)malloc((size_t)sz) : NULL;
+    if (!q) {
+      atomic_store(&x->err, 1);
+      return;
+    }
+    memcpy(q, src, (size_t)sz);
+    c->chunks[nc] = q;
+    c->csizes[nc] = sz;
+  } else if (iax_encode_chunk(x->h + x->out_off[i] + nc * c->chunk_nbytes, (int32_t)c->chunk_nbytes, c->blocksize,
+                              c->itemsize, &c->cfg, x->filtered, &c->chunks[nc], &c->csizes[nc]) != 0) {
+    atomic_store(&x->err, 1);
+  }
+}
+
+/* Share of the output bytes encoded on the GPU: IARRAY_GPU_ENCODE (0..1), or
+ * the default below; 0 when the codec settings are not the GPU encoder's. */
+static double batch_gpu_share(iarray_container_t **c, int n, int64_t total) {
+  const iarray_container_t *c0 = c[0];
+  const int ok = c0->cfg.compression_level > 0 &&
+                 (c0->cfg.compression_codec == IARRAY_COMPRESSION_LZ4 ||
+                  c0->cfg.compression_codec == IARRAY_COMPRESSION_LZ4HC) &&
+                 c0->itemsize <= IAMX_MAX_STREAMS;
+  if (!ok) return 0.0;
+  for (int i = 0; i < n; i++)
+    if (c[i]->chunk_nbytes % c[i]->blocksize != 0 || c[i]->chunk_nbytes > INT32_MAX / 2) return 0.0;
+  const char *env = getenv("IARRAY_GPU_ENCODE");
+  if (!env) return total >= ((int64_t)128 << 20) ? IAX_BATCH_GPU_SHARE : 0.0;
+  const double v = atof(env);
+  if (!(v > 0.0)) return 0.0;
+  return v > 1.0 ? 1.0 : v;
+}
+
+static ina_rc_t batch_write(iarray_context_t *ctx, int n, iarray_container_t **c) {
+  const batch_plan_t *p = &g_plan;
+  const int64_t total = p->packed_bytes;
+  const double share = batch_gpu_share(c, n, total);
+  /* the GPU takes a suffix of the products holding about `share` of the bytes */
+  int hsplit = n;
+  if (share > 0.0) {
+    int64_t acc = 0;
+    while (hsplit > 0 && acc < share * (double)total) {
+      hsplit--;
+      acc += c[hsplit]->nchunks * c[hsplit]->chunk_nbytes;
+    }
+  }
+  const int ngpu = n - hsplit;
+  int64_t *first = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n + 1));
+  int64_t *gout = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
+  int64_t *cap = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
+  int *split = (int *)calloc((size_t)n + 1, sizeof(int));
+  int64_t *pfirst = (int64_t *)malloc(sizeof(int64_t) * (IAMX_COPY_EVENTS + 1));
+  int *gprod = (int *)malloc(sizeof(int) * (IAMX_COPY_EVENTS + 1)); /* first product of each GPU group */
+  if (!first || !gout || !cap || !split || !pfirst || !gprod) {
+    free(first); free(gout); free(cap); free(split); free(pfirst); free(gprod);
+    return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+  }
+  first[0] = 0;
+  for (int i = 0; i < n; i++) first[i + 1] = first[i] + c[i]->nchunks;
+  ina_rc_t rc = INA_SUCCESS;
+  int64_t gbytes = 0, sizes_bytes = 0;
+  for (int i = hsplit; i < n && INA_SUCCEED(rc); i++) {
+    split[i] = iax_split(&c[i]->cfg, c[i]->itemsize, c[i]->blocksize);
+    cap[i] = iamx_chunk_capacity(&p->tilings[i], split[i]);
+    if (cap[i] < c[i]->chunk_nbytes) rc = iax_dev_error(IAMX_EINVAL, "chunk capacity");
+    gout[i] = gbytes;
+    gbytes += c[i]->nchunks * cap[i];
+    sizes_bytes += c[i]->nchunks * 8;
+  }
+  /* pieces: host products in downloads of about 4 MB or more (at most 40),
+   * then the GPU products in groups of about equal bytes (at most 16) */
+  const int64_t hbytes = hsplit < n ? p->out_off[hsplit] : total; /* packed bytes of the host products */
+  int np = 0;
+  pfirst[0] = 0;
+  if (hsplit > 0) {
+    int64_t piece = hbytes / 40 > ((int64_t)4 << 20) ? hbytes / 40 : ((int64_t)4 << 20);
+    int64_t at = 0;
+    for (int i = 0; i < hsplit; i++)
+      for (int64_t k = 0; k < c[i]->nchunks; k++) {
+        at += c[i]->chunk_nbytes;
+        const int last = i == hsplit - 1 && k == c[i]->nchunks - 1;
+        if ((at >= piece * (np + 1) && np < 39) || last) pfirst[++np] = first[i] + k + 1;
+      }
+  }
+  const int nhp = np;
+  if (ngpu > 0) {
+    const int ng = ngpu < 16 ? ngpu : 16;
+    int64_t acc = 0;
+    int g = 0;
+    gprod[0] = hsplit;
+    for (int i = hsplit; i < n; i++) {
+      acc += c[i]->nchunks * cap[i];
+      if ((acc * ng >= gbytes * (g + 1) && g < ng - 1) || i == n - 1) {
+        pfirst[++np] = first[i + 1];
+        gprod[++g] = i + 1;
+      }
+    }
+  }
+  const int64_t gbase = (hbytes + 255) / 256 * 256;
+  uint8_t *h = INA_SUCCEED(rc) ? (uint8_t *)iamx_host_buffer((size_t)(gbase + gbytes + 256)) : NULL;
+  if (INA_SUCCEED(rc) && !h) rc = iax_dev_error(IAMX_ENOMEM, "pinned staging buffer");
+  int e = 0;
+  /* host pieces first on the copy stream */
+  for (int k = 0; k < nhp && INA_SUCCEED(rc) && !e; k++) {
+    const int i0 = find_le(first, n, pfirst[k]), i1 = find_le(first, n, pfirst[k + 1] - 1);
+    const int64_t b0 = p->out_off[i0] + (pfirst[k] - first[i0]) * c[i0]->chunk_nbytes;
+    const int64_t b1 = p->out_off[i1] + (pfirst[k + 1] - first[i1]) * c[i1]->chunk_nbytes;
+    e = iamx_d2h_async(h + b0, (const uint8_t *)p->d_packed + b0, (size_t)(b1 - b0), k);
+  }
+  if (e) rc = iax_dev_error(e, "download");
+  /* then the GPU groups: encode on the side stream, download behind each */
+  if (INA_SUCCEED(rc) && ngpu > 0) {
+    int64_t scr = 0;
+    for (int g = 0; g < np - nhp && scr >= 0; g++) {
+      const int64_t b = iamx_encode_batch_scratch_bytes(gprod[g + 1] - gprod[g], p->tilings + gprod[g], split + gprod[g]);
+      scr = b < 0 ? -1 : scr + (b + 255) / 256 * 256;
+    }
+    uint8_t *d_scr = scr > 0 ? (uint8_t *)iamx_workspace(12, (size_t)scr) : NULL;
+    uint8_t *d_out = (uint8_t *)iamx_workspace(14, (size_t)gbytes + 256);
+    int64_t *d_sizes = (int64_t *)iamx_workspace(13, (size_t)sizes_bytes + 256);
+    if (scr <= 0 || !d_scr || !d_out || !d_sizes) rc = iax_dev_error(IAMX_ENOMEM, "batch encode buffers");
+    if (INA_SUCCEED(rc) && ((e = iamx_side_wait()) != 0 || (e = iamx_use_stream(1)) != 0))
+      rc = iax_dev_error(e, "encode");
+    int64_t so = 0, zo = 0;
+    for (int g = 0; g < np - nhp && INA_SUCCEED(rc); g++) {
+      const int i0 = gprod[g], m = gprod[g + 1] - gprod[g];
+      e = iamx_encode_batch(m, p->tilings + i0, split + i0, p->d_packed, p->out_off + i0, d_scr + so, d_out,
+                            gout + i0, d_sizes + zo);
+      const int64_t b0 = gout[i0], b1 = g + 1 < np - nhp ? gout[gprod[g + 1]] : gbytes;
+      if (!e) e = iamx_d2h_async(h + gbase + b0, d_out + b0, (size_t)(b1 - b0), nhp + g);
+      if (e) rc = iax_dev_error(e, "encode");
+      so += (iamx_encode_batch_scratch_bytes(m, p->tilings + i0, split + i0) + 255) / 256 * 256;
+      zo += first[gprod[g + 1]] - first[i0];
+    }
+    iamx_use_stream(0);
+  }
+  if (INA_FAILED(rc)) {
+    for (int k = 0; k < np; k++) (void)iamx_event_sync(k);
+  } else {
+    wctx_t x;
+    x.c = c;
+    x.first = first;
+    x.n = n;
+    x.hsplit = hsplit;
+    x.h = h;
+    x.out_off = p->out_off;
+    x.gout_off = gout;
+    x.cap = cap;
+    x.gbase = gbase;
+    x.pfirst = pfirst;
+    x.npieces = np;
+    x.filtered = c[0]->cfg.compression_level > 0;
+    for (int k = 0; k < IAMX_COPY_EVENTS; k++) atomic_init(&x.landed[k], 0);
+    atomic_init(&x.err, 0);
+    iax_parallel_for(iax_nthreads(&ctx->cfg), first[n], write_task, &x);
+    if (atomic_load(&x.err)) {
+      for (int k = 0; k < np; k++) (void)iamx_event_sync(k);
+      rc = iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "chunk encode failed");
+    }
+    for (int i = hsplit; i < n && INA_SUCCEED(rc); i++) {
+      iax_last_gpu_enc_in += c[i]->nchunks * c[i]->chunk_nbytes;
+      for (int64_t k = 0; k < c[i]->nchunks; k++) iax_last_gpu_enc_out += c[i]->csizes[k];
+    }
+  }
+  free(first); free(gout); free(cap); free(split); free(pfirst); free(gprod);
+  return rc;
 }
 
 INA_API(ina_rc_t) iarray_linalg_matmul_batch(iarray_context_t *ctx, int n, iarray_container_t **a,
@@ -346,25 +550,9 @@ INA_API(ina_rc_t) iarray_linalg_matmul_batch(iarray_context_t *ctx, int n, iarra
   if (st)
     return st < 0 ? iax_dev_error(st, "batch status")
                   : iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "malformed LZ4 stream (0x%x)", st);
-  int64_t *first = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n + 1));
-  if (!first) return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
-  first[0] = 0;
-  for (int i = 0; i < n; i++) first[i + 1] = first[i] + c[i]->nchunks;
-  int64_t *ends = (int64_t *)malloc(sizeof(int64_t) * (size_t)(first[n] + 1));
-  if (!ends) {
-    free(first);
-    return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
-  }
-  for (int i = 0; i < n; i++)
-    for (int64_t k = 0; k < c[i]->nchunks; k++) ends[first[i] + k] = g_plan.out_off[i] + (k + 1) * c[i]->chunk_nbytes;
-  wctx_t x = {c, first, n, NULL, g_plan.out_off, 0};
   const double t0 = iax_now();
-  rc = iax_download_pipelined(g_plan.d_packed, (size_t)g_plan.packed_bytes, ends, first[n],
-                                       iax_nthreads(&ctx->cfg), write_task, &x, &x.packed);
+  iax_last_gpu_enc_in = iax_last_gpu_enc_out = 0;
+  const ina_rc_t rc2 = batch_write(ctx, n, c);
   iax_last_encode_s = iax_now() - t0;
-  free(ends);
-  free(first);
-  if (INA_FAILED(rc)) return rc;
-  if (x.err) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "chunk encode failed");
-  return INA_SUCCESS;
+  return rc2;
 }

@@ -867,6 +867,11 @@ static ina_rc_t write_hybrid(iarray_container_t *c, const iamx_tiling_t *t, cons
     for (int k = 0; k < npanels; k++) (void)iamx_event_sync(k);
     return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "chunk write failed");
   }
+  for (int k = 0; k < npanels; k++)
+    for (int64_t nc = pf[k]; nc < pf[k + 1] && on_gpu[k]; nc++) {
+      iax_last_gpu_enc_in += c->chunk_nbytes;
+      iax_last_gpu_enc_out += c->csizes[nc];
+    }
   return INA_SUCCESS;
 }
 
@@ -930,6 +935,7 @@ ina_rc_t iax_write_from_dense_rows(iarray_container_t *c, const void *d_dense, i
     pf[k] = r * c->nchunks_ax[1];
   }
   const double share = gpu_encode_share(c, npanels);
+  iax_last_gpu_enc_in = iax_last_gpu_enc_out = 0;
   if (share > 0.0) {
     const double t0 = iax_now();
     ina_rc_t rc = write_hybrid(c, &t, d_dense, d_packed, pf, npanels, rows, rctx, share);

@@ -17,6 +17,7 @@ static int g_initialized = 0;
 static __thread char g_msg[512];
 double iax_last_encode_s = 0.0;
 double iax_last_device_ms = 0.0;
+int64_t iax_last_gpu_enc_in = 0, iax_last_gpu_enc_out = 0;
 int iax_last_slices = 0;
 double iax_last_slice_err = -1.0;
 
@@ -104,6 +105,11 @@ INA_API(int) iarray_last_ozaki_slices(double *sampled_err) {
 INA_API(void) iarray_last_matmul_times(double *host_encode_s, double *device_ms) {
   if (host_encode_s) *host_encode_s = iax_last_encode_s;
   if (device_ms) *device_ms = iax_last_device_ms;
+}
+
+INA_API(void) iarray_last_gpu_encode_bytes(int64_t *stream_bytes, int64_t *chunk_bytes) {
+  if (stream_bytes) *stream_bytes = iax_last_gpu_enc_in;
+  if (chunk_bytes) *chunk_bytes = iax_last_gpu_enc_out;
 }
 
 INA_API(ina_rc_t) iarray_device_info(char *name, int namelen, int *cus, char *arch, int archlen) {

@@ -154,6 +154,7 @@ ina_rc_t iax_dev_error(int code, const char *what);
 
 extern double iax_last_encode_s;
 extern double iax_last_device_ms;
+extern int64_t iax_last_gpu_enc_in, iax_last_gpu_enc_out; /* bytes the GPU encoder read and wrote in the last write */
 extern int iax_last_slices;
 extern double iax_last_slice_err;
 

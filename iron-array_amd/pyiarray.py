@@ -124,6 +124,7 @@ def lib():
         "iarray_prof_enable": ([ctypes.c_bool], None), "iarray_prof_reset": ([], None),
         "iarray_prof_entry": ([ctypes.c_int, P(ctypes.c_char_p), P(i64), P(ctypes.c_double)], ctypes.c_bool),
         "iarray_last_matmul_times": ([P(ctypes.c_double), P(ctypes.c_double)], None),
+        "iarray_last_gpu_encode_bytes": ([P(i64), P(i64)], None),
         "iarray_last_ozaki_slices": ([P(ctypes.c_double)], ctypes.c_int),
         "iarray_device_info": ([ctypes.c_char_p, ctypes.c_int, P(ctypes.c_int), ctypes.c_char_p, ctypes.c_int], rc_t),
     }
@@ -140,6 +141,13 @@ def _libc_free(ptr):
     libc = ctypes.CDLL(None)
     libc.free.argtypes = [ctypes.c_void_p]
     libc.free(ctypes.cast(ptr, ctypes.c_void_p))
+
+
+def last_gpu_encode_bytes():
+    """(filtered stream bytes read, chunk bytes written) by the GPU encoder in the last matmul."""
+    a, b = ctypes.c_int64(), ctypes.c_int64()
+    lib().iarray_last_gpu_encode_bytes(ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
 
 
 def remove(path):

@@ -185,3 +185,40 @@ def test_mixed_host_and_gpu_panels(monkeypatch, orc, share):
         _check_chunk(ch, nbytes)
     dense = orc.Array(chunks, z.shape, (128, 150), (64, 50), 4).to_dense()
     assert dense.tobytes() == got.tobytes()
+
+
+@pytest.mark.parametrize("share", ["0.5", "1"])
+def test_batch_outputs_encoded_on_the_gpu(monkeypatch, orc, share):
+    """A batch whose later products are encoded by one GPU encoder pass per
+    group (ragged f32 products, several chunks each, edge chunks): every
+    chunk valid, every output equal to the all-host batch and decoding to the
+    same values through the oracle."""
+    print(f"case: batch gpu encode share {share}", flush=True)
+    ctx = ia.Context(clevel=5, filters=ia.COMP_SHUFFLE)
+    rng = np.random.default_rng(11)
+    pairs, cs_l, bs_l = [], [], []
+    for _ in range(24):
+        M, N, K = (int(v) for v in rng.integers(40, 400, 3))
+        a = rng.standard_normal((M, K)).astype(np.float32)
+        b = rng.standard_normal((K, N)).astype(np.float32)
+        ca = ia.Container.from_numpy(ctx, a, (min(M, 128), min(K, 128)), (min(M, 64), min(K, 64)))
+        cb = ia.Container.from_numpy(ctx, b, (min(K, 128), min(N, 128)), (min(K, 64), min(N, 64)))
+        pairs.append((ca, cb))
+        cs_l.append((min(M, 150), min(N, 120)))
+        bs_l.append((min(M, 50), min(N, 40)))
+    monkeypatch.setenv("IARRAY_GPU_ENCODE", "0")
+    ref = [z.to_numpy() for z in ia.matmul_batch(pairs, cs_l, bs_l)]
+    monkeypatch.setenv("IARRAY_GPU_ENCODE", share)
+    for rep in range(2):  # the second call replays the graph
+        zs = ia.matmul_batch(pairs, cs_l, bs_l)
+        for z, r, cs, bs in zip(zs, ref, cs_l, bs_l):
+            got = z.to_numpy()
+            assert got.tobytes() == r.tobytes()
+            chunks = z.chunks()
+            ext = [(-(-cs[d] // bs[d])) * bs[d] for d in range(2)]
+            for ch in chunks:
+                _check_chunk(ch, ext[0] * ext[1] * 4)
+            if rep == 1:
+                dense = orc.Array(chunks, z.shape, cs, bs, 4).to_dense()
+                assert dense.tobytes() == got.tobytes()
+
