@@ -123,7 +123,7 @@ def main():
         M = K = 0
         dtype = "f32"
         config = {"workload": "512 independent f32 matmuls, M,N,K ~ U{64..1536} (seed 1001), chunks min(dim,512), "
-                              "blocks 128x128 with ragged edges, LZ4 clevel5 + shuffle, one graph replay per step",
+                              "blocks 128x128 with ragged edges, LZ4 clevel5 + shuffle, one graph replay per phase of products",
                   "products": 512, "total_flop": flops}
         batch_spec = (pairs, cs_l, bs_l)
     else:
@@ -182,7 +182,7 @@ def main():
 
     # roofline of the dominant kernel.  The library's launch timings cover its
     # last call, i.e. one step (row-panel GEMMs launch once a panel, Ozaki
-    # once a level and panel).  The batch runs as one graph replay, so its
+    # once a level and panel).  The batch runs as graph replays, so its
     # kernels are timed on one more, eager replay of the same launches.
     if args.workload == "batch512":
         os.environ["IARRAY_BATCH_EAGER"] = "1"

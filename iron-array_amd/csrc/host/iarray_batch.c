@@ -14,10 +14,13 @@
  *     work-queue GEMM;
  *   - every C block goes into ONE pack table;
  * and the launch sequence (decode, queue reset, GEMM, pack) is captured into
- * a hipGraph.  A call with the same signature replays the graph; then a
- * suffix of the products is LZ4-encoded and assembled into chunks on the GPU
- * (one encoder pass per group of products) while the host threads encode the
- * others from their downloaded packed blocks.
+ * hipGraphs, one per phase of consecutive products (about equal output
+ * bytes).  A call with the same signature replays the phases one after the
+ * other, and behind each queues its C write: the phase's last products are
+ * LZ4-encoded and assembled into chunks on the GPU (one encoder pass per group
+ * of products) while the host threads encode the others from their
+ * downloaded packed blocks; the downloads and GPU encoding of a phase overlap
+ * the products of the next.
  * Plain precision only; the mixed forms run product by product.
  */
 #include <stdatomic.h>
@@ -27,10 +30,23 @@
 #include "iarray_private.h"
 
 /* Default share of a batch's output bytes encoded on the GPU (outputs of 128
- * MiB or more).  Measured on 512 ragged float32 products (1.3 GB of C, 16
- * host threads), ms per call at shares 0 / 0.3 / 0.5 / 0.6 / 0.7 / 0.8 / 1:
- * 99 / 73 / 63-72 / 62 / 60-61 / 65 / 70. */
-#define IAX_BATCH_GPU_SHARE 0.65
+ * MiB or more).  Measured on 512 ragged float32 products (2.1 GB of C, 16
+ * host threads) in 2 phases, ms per call at shares 0.65 / 0.8 / 0.9 / 1:
+ * 60 / 59 / 59 / 59 (one phase, share 0: 93-99). */
+#define IAX_BATCH_GPU_SHARE 0.8
+
+/* A batch runs in up to this many phases of consecutive products (about equal
+ * output bytes each), one graph per phase: the C write of a phase (downloads
+ * and GPU encoding) overlaps the products of the next.  Default 2: on 512
+ * ragged products 1 / 2 / 4 phases took 62 / 59-60 / 59-64 ms. */
+#define IAX_BATCH_MAX_PHASES 4
+#define IAX_BATCH_PHASES 2
+typedef struct {
+  int p0, p1;         /* products [p0, p1) */
+  int l0, nl, b0, nb; /* its LZ4 and block decode jobs */
+  int t0, nt, q0, nq; /* its GEMM tiles and C blocks to pack */
+  void *graph;        /* decode, GEMM and pack of the phase */
+} batch_phase_t;
 
 typedef struct {
   int n;
@@ -38,7 +54,8 @@ typedef struct {
   uint64_t *gen;
   int64_t *store; /* chunk/block shapes of each output, 4 per product */
   int dt;
-  void *graph;
+  int nphase;
+  batch_phase_t ph[IAX_BATCH_MAX_PHASES];
   /* device buffers owned by the plan (their addresses are baked into the graph) */
   void *d_dense, *d_dense_c, *d_scratch, *d_packed;
   iamx_lz4_job_t *d_lz4;
@@ -49,15 +66,13 @@ typedef struct {
   int64_t packed_bytes;
   int64_t *out_off; /* per product: byte offset of its packed blocks */
   iamx_tiling_t *tilings; /* per product: C's tiling (for the GPU encoder) */
-  /* the captured launches' arguments, for an eager replay */
-  int n_lz4, n_blk, ntiles, npb;
   int64_t b_base; /* byte offset of the B images in d_dense */
 } batch_plan_t;
 
 static batch_plan_t g_plan;
 
 static void free_batch_plan(batch_plan_t *p) {
-  iamx_graph_destroy(p->graph);
+  for (int k = 0; k < p->nphase; k++) iamx_graph_destroy(p->ph[k].graph);
   void *bufs[] = {p->d_dense, p->d_dense_c, p->d_scratch, p->d_packed, p->d_lz4,
                   p->d_blocks, p->d_prods, p->d_tiles, p->d_pblocks, p->d_packs};
   for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++) iamx_free(bufs[i]);
@@ -70,9 +85,18 @@ static void free_batch_plan(batch_plan_t *p) {
   memset(p, 0, sizeof(*p));
 }
 
+/* IARRAY_BATCH_PHASES (1..4) sets the number of phases */
+static int batch_phases(int n) {
+  const char *e = getenv("IARRAY_BATCH_PHASES");
+  int k = e ? atoi(e) : IAX_BATCH_PHASES;
+  if (k < 1) k = 1;
+  if (k > IAX_BATCH_MAX_PHASES) k = IAX_BATCH_MAX_PHASES;
+  return k < n ? k : n;
+}
+
 static int same_signature(const batch_plan_t *p, int n, iarray_container_t **a, iarray_container_t **b,
                           const iarray_storage_t *st, int dt) {
-  if (!p->graph || p->n != n || p->dt != dt) return 0;
+  if (p->nphase < 1 || p->n != n || p->dt != dt || p->nphase != batch_phases(n)) return 0;
   for (int i = 0; i < n; i++) {
     if (p->a[i] != a[i] || p->b[i] != b[i]) return 0;
     if (!a[i]->device_valid || !b[i]->device_valid) return 0;
@@ -123,6 +147,17 @@ static ina_rc_t upload(void **d, const void *h, size_t bytes) {
   if (!*d) return iax_dev_error(IAMX_ENOMEM, "batch plan");
   int e = iamx_h2d(*d, h, bytes);
   return e ? iax_dev_error(e, "batch plan upload") : INA_SUCCESS;
+}
+
+/* enqueue phase k's decode, GEMM and pack */
+static int run_phase(const batch_plan_t *p, int k) {
+  const batch_phase_t *q = &p->ph[k];
+  int e = iamx_decode(p->d_lz4 + q->l0, q->nl, p->d_blocks + q->b0, q->nb, p->d_scratch, p->d_dense);
+  if (!e)
+    e = iamx_gemm_batch(p->dt, p->d_prods, p->d_tiles + 3 * (int64_t)q->t0, q->nt, p->d_dense,
+                        (uint8_t *)p->d_dense + p->b_base, p->d_dense_c);
+  if (!e) e = iamx_pack_batch(p->d_packs, p->d_pblocks + 2 * (int64_t)q->q0, q->nq, p->d_dense_c, p->d_packed);
+  return e;
 }
 
 static ina_rc_t build_plan(batch_plan_t *p, int n, iarray_container_t **a, iarray_container_t **b,
@@ -186,6 +221,22 @@ static ina_rc_t build_plan(batch_plan_t *p, int n, iarray_container_t **a, iarra
     free(packs);
     return iax_set_error(INA_ERROR(INA_ERR_INVALID_ARGUMENT), "batch too large for one pass");
   }
+  /* phases of about equal output bytes */
+  const int nph = batch_phases(n);
+  p->nphase = nph;
+  {
+    int k = 0;
+    p->ph[0].p0 = 0;
+    for (int i = 0; i < n; i++) {
+      const int64_t end = p->out_off[i] + c[i]->nchunks * c[i]->chunk_nbytes;
+      if (k < nph - 1 && end * nph >= packed * (k + 1) && i < n - 1) {
+        p->ph[k].p1 = i + 1;
+        p->ph[++k].p0 = i + 1;
+      }
+    }
+    p->ph[k].p1 = n;
+    p->nphase = k + 1;
+  }
   ina_rc_t rc = INA_SUCCESS;
   iamx_lz4_job_t *lz4 = (iamx_lz4_job_t *)malloc(sizeof(iamx_lz4_job_t) * (size_t)(nlz + 1));
   iamx_block_job_t *blk = (iamx_block_job_t *)malloc(sizeof(iamx_block_job_t) * (size_t)(nblk + 1));
@@ -194,7 +245,15 @@ static ina_rc_t build_plan(batch_plan_t *p, int n, iarray_container_t **a, iarra
   int32_t *pblk = (int32_t *)malloc(sizeof(int32_t) * 2 * (size_t)(npb + 1));
   if (!lz4 || !blk || !tl || !tiles || !pblk) rc = INA_ERROR(INA_ERR_OUT_OF_MEMORY);
   int64_t il = 0, ib = 0, it = 0, ip = 0, so = 0;
+  int ph = 0;
   for (int i = 0; i < n && INA_SUCCEED(rc); i++) {
+    if (i == p->ph[ph].p1) ph++;
+    if (i == p->ph[ph].p0) {
+      p->ph[ph].l0 = (int)il;
+      p->ph[ph].b0 = (int)ib;
+      p->ph[ph].t0 = (int)it;
+      p->ph[ph].q0 = (int)ip;
+    }
     append_jobs(a[i], prods[i].a_off, so, lz4, &il, blk, &ib);
     so += a[i]->plan.scratch_bytes;
     append_jobs(b[i], na + prods[i].b_off, so, lz4, &il, blk, &ib);
@@ -208,16 +267,25 @@ static ina_rc_t build_plan(batch_plan_t *p, int n, iarray_container_t **a, iarra
       pblk[2 * ip + 1] = (int32_t)k;
       ip++;
     }
+    if (i == p->ph[ph].p1 - 1) {
+      p->ph[ph].nl = (int)il - p->ph[ph].l0;
+      p->ph[ph].nb = (int)ib - p->ph[ph].b0;
+      p->ph[ph].nt = (int)it - p->ph[ph].t0;
+      p->ph[ph].nq = (int)ip - p->ph[ph].q0;
+    }
   }
   if (INA_SUCCEED(rc)) {
-    qsort(tl, (size_t)ntiles, sizeof(tile_t), by_k_desc);
+    /* each phase's tiles longest K first, its LZ4 jobs largest first */
+    for (int k = 0; k < p->nphase; k++) {
+      qsort(tl + p->ph[k].t0, (size_t)p->ph[k].nt, sizeof(tile_t), by_k_desc);
+      iax_sort_lz4_jobs(lz4 + p->ph[k].l0, p->ph[k].nl);
+    }
     for (int64_t k = 0; k < ntiles; k++) {
       tiles[3 * k] = tl[k].p;
       tiles[3 * k + 1] = tl[k].tm;
       tiles[3 * k + 2] = tl[k].tn;
     }
   }
-  iax_sort_lz4_jobs(lz4, il);
   if (INA_SUCCEED(rc)) rc = upload((void **)&p->d_lz4, lz4, sizeof(iamx_lz4_job_t) * (size_t)il);
   if (INA_SUCCEED(rc)) rc = upload((void **)&p->d_blocks, blk, sizeof(iamx_block_job_t) * (size_t)ib);
   if (INA_SUCCEED(rc)) rc = upload((void **)&p->d_prods, prods, sizeof(iamx_batch_prod_t) * (size_t)n);
@@ -234,18 +302,13 @@ static ina_rc_t build_plan(batch_plan_t *p, int n, iarray_container_t **a, iarra
       rc = iax_dev_error(IAMX_ENOMEM, "batch buffers");
   }
   if (INA_SUCCEED(rc)) {
-    uint8_t *dense_b = (uint8_t *)p->d_dense + na * is;
-    p->n_lz4 = (int)il;
-    p->n_blk = (int)ib;
-    p->ntiles = (int)ntiles;
-    p->npb = (int)npb;
     p->b_base = na * is;
-    int e = iamx_capture_begin();
-    if (!e) e = iamx_decode(p->d_lz4, (int)il, p->d_blocks, (int)ib, p->d_scratch, p->d_dense);
-    if (!e) e = iamx_gemm_batch(dt, p->d_prods, p->d_tiles, (int)ntiles, p->d_dense, dense_b, p->d_dense_c);
-    if (!e) e = iamx_pack_batch(p->d_packs, p->d_pblocks, (int)npb, p->d_dense_c, p->d_packed);
-    const int e2 = iamx_capture_end(&p->graph);
-    if (e || e2) rc = iax_dev_error(e ? e : e2, "batch graph capture");
+    for (int k = 0; k < p->nphase && INA_SUCCEED(rc); k++) {
+      int e = iamx_capture_begin();
+      if (!e) e = run_phase(p, k);
+      const int e2 = iamx_capture_end(&p->ph[k].graph);
+      if (e || e2) rc = iax_dev_error(e ? e : e2, "batch graph capture");
+    }
   }
   free(lz4);
   free(blk);
@@ -257,21 +320,23 @@ static ina_rc_t build_plan(batch_plan_t *p, int n, iarray_container_t **a, iarra
   return rc;
 }
 
-/* The C write of a batch.  Products [0, hsplit) are LZ4-encoded on the host
- * threads from their packed blocks; products [hsplit, n) are encoded and
- * assembled into chunks on the GPU (on the side stream, one encoder pass per
- * group of products) and only copied into place on the host.  Every
- * download is queued up front on the copy stream, host pieces first; the
- * host threads take each chunk as soon as its piece or group has landed.
- * Staging: the host products' packed bytes at their packed offsets, then,
- * from gbase, the GPU products' chunks at gout_off[i] + chunk * cap[i]. */
+/* The C write of a batch, phase by phase.  Each phase's graph is launched,
+ * then its downloads and GPU encoding are queued behind it, so they overlap
+ * the next phases' products.  In each phase the products holding about
+ * `share` of its output bytes (the last ones) are LZ4-encoded and assembled
+ * into chunks on the GPU (side stream, one encoder pass per group) and only
+ * copied into place on the host; the others are encoded on the host threads
+ * from their downloaded packed blocks.  Every chunk lands in the pinned
+ * staging buffer at stage[i] + chunk * stride[i] (stride = chunk bytes for a
+ * host product, the chunk capacity for a GPU one); the host threads take each
+ * chunk as soon as its piece has landed. */
 typedef struct {
   iarray_container_t **c;
   const int64_t *first; /* prefix sums of chunk counts, n + 1 entries */
-  int n, hsplit;
+  int n;
+  const uint8_t *on_gpu;
   const uint8_t *h;
-  const int64_t *out_off, *gout_off, *cap;
-  int64_t gbase;
+  const int64_t *stage, *cap;
   const int64_t *pfirst; /* piece k covers global chunks [pfirst[k], pfirst[k + 1]) */
   int npieces;
   int filtered;
@@ -305,8 +370,8 @@ static void write_task(void *vctx, int64_t g, int worker) {
   const int64_t nc = g - x->first[i];
   free(c->chunks[nc]);
   c->chunks[nc] = NULL;
-  if (i >= x->hsplit) {
-    const uint8_t *src = x->h + x->gbase + x->gout_off[i] + nc * x->cap[i];
+  if (x->on_gpu[i]) {
+    const uint8_t *src = x->h + x->stage[i] + nc * x->cap[i];
     int32_t sz;
     memcpy(&sz, src + 12, 4); /* cbytes of the assembled chunk */
     uint8_t *q = sz > 32 && sz <= x->cap[i] ? (uint8_t *)malloc((size_t)sz) : NULL;
@@ -317,7 +382,7 @@ static void write_task(void *vctx, int64_t g, int worker) {
     memcpy(q, src, (size_t)sz);
     c->chunks[nc] = q;
     c->csizes[nc] = sz;
-  } else if (iax_encode_chunk(x->h + x->out_off[i] + nc * c->chunk_nbytes, (int32_t)c->chunk_nbytes, c->blocksize,
+  } else if (iax_encode_chunk(x->h + x->stage[i] + nc * c->chunk_nbytes, (int32_t)c->chunk_nbytes, c->blocksize,
                               c->itemsize, &c->cfg, x->filtered, &c->chunks[nc], &c->csizes[nc]) != 0) {
     atomic_store(&x->err, 1);
   }
@@ -341,138 +406,177 @@ static double batch_gpu_share(iarray_container_t **c, int n, int64_t total) {
   return v > 1.0 ? 1.0 : v;
 }
 
-static ina_rc_t batch_write(iarray_context_t *ctx, int n, iarray_container_t **c) {
+#define IAX_PIECES_PER_PHASE 10 /* host downloads per phase */
+#define IAX_GROUPS_PER_PHASE 4  /* GPU encoder passes per phase */
+
+static ina_rc_t batch_write(iarray_context_t *ctx, int n, iarray_container_t **c, int eager) {
   const batch_plan_t *p = &g_plan;
-  const int64_t total = p->packed_bytes;
-  const double share = batch_gpu_share(c, n, total);
-  /* the GPU takes a suffix of the products holding about `share` of the bytes */
-  int hsplit = n;
-  if (share > 0.0) {
-    int64_t acc = 0;
-    while (hsplit > 0 && acc < share * (double)total) {
-      hsplit--;
-      acc += c[hsplit]->nchunks * c[hsplit]->chunk_nbytes;
-    }
-  }
-  const int ngpu = n - hsplit;
+  const double share = batch_gpu_share(c, n, p->packed_bytes);
   int64_t *first = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n + 1));
+  int64_t *stage = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
   int64_t *gout = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
   int64_t *cap = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
   int *split = (int *)calloc((size_t)n + 1, sizeof(int));
+  uint8_t *on_gpu = (uint8_t *)calloc((size_t)n + 1, 1);
   int64_t *pfirst = (int64_t *)malloc(sizeof(int64_t) * (IAMX_COPY_EVENTS + 1));
-  int *gprod = (int *)malloc(sizeof(int) * (IAMX_COPY_EVENTS + 1)); /* first product of each GPU group */
-  if (!first || !gout || !cap || !split || !pfirst || !gprod) {
-    free(first); free(gout); free(cap); free(split); free(pfirst); free(gprod);
+  int *hsplit = (int *)malloc(sizeof(int) * (IAX_BATCH_MAX_PHASES + 1)); /* first GPU product of each phase */
+  if (!first || !stage || !gout || !cap || !split || !on_gpu || !pfirst || !hsplit) {
+    free(first); free(stage); free(gout); free(cap); free(split); free(on_gpu); free(pfirst); free(hsplit);
     return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
   }
+  ina_rc_t rc = INA_SUCCESS;
   first[0] = 0;
   for (int i = 0; i < n; i++) first[i + 1] = first[i] + c[i]->nchunks;
-  ina_rc_t rc = INA_SUCCESS;
-  int64_t gbytes = 0, sizes_bytes = 0;
-  for (int i = hsplit; i < n && INA_SUCCEED(rc); i++) {
-    split[i] = iax_split(&c[i]->cfg, c[i]->itemsize, c[i]->blocksize);
-    cap[i] = iamx_chunk_capacity(&p->tilings[i], split[i]);
-    if (cap[i] < c[i]->chunk_nbytes) rc = iax_dev_error(IAMX_EINVAL, "chunk capacity");
-    gout[i] = gbytes;
-    gbytes += c[i]->nchunks * cap[i];
-    sizes_bytes += c[i]->nchunks * 8;
-  }
-  /* pieces: host products in downloads of about 4 MB or more (at most 40),
-   * then the GPU products in groups of about equal bytes (at most 16) */
-  const int64_t hbytes = hsplit < n ? p->out_off[hsplit] : total; /* packed bytes of the host products */
-  int np = 0;
-  pfirst[0] = 0;
-  if (hsplit > 0) {
-    int64_t piece = hbytes / 40 > ((int64_t)4 << 20) ? hbytes / 40 : ((int64_t)4 << 20);
-    int64_t at = 0;
-    for (int i = 0; i < hsplit; i++)
-      for (int64_t k = 0; k < c[i]->nchunks; k++) {
-        at += c[i]->chunk_nbytes;
-        const int last = i == hsplit - 1 && k == c[i]->nchunks - 1;
-        if ((at >= piece * (np + 1) && np < 39) || last) pfirst[++np] = first[i] + k + 1;
+  /* plan: the GPU products of each phase, staging offsets (host products
+   * first, then the GPU products' chunks), pieces and groups */
+  int64_t hbytes = 0, gbytes = 0, sizes_bytes = 0;
+  for (int k = 0; k < p->nphase; k++) {
+    const int p0 = p->ph[k].p0, p1 = p->ph[k].p1;
+    const int64_t pb = p->out_off[p1 - 1] + c[p1 - 1]->nchunks * c[p1 - 1]->chunk_nbytes - p->out_off[p0];
+    int hs = p1;
+    if (share > 0.0) {
+      int64_t acc = 0;
+      while (hs > p0 && acc < share * (double)pb) {
+        hs--;
+        acc += c[hs]->nchunks * c[hs]->chunk_nbytes;
       }
-  }
-  const int nhp = np;
-  if (ngpu > 0) {
-    const int ng = ngpu < 16 ? ngpu : 16;
-    int64_t acc = 0;
-    int g = 0;
-    gprod[0] = hsplit;
-    for (int i = hsplit; i < n; i++) {
-      acc += c[i]->nchunks * cap[i];
-      if ((acc * ng >= gbytes * (g + 1) && g < ng - 1) || i == n - 1) {
-        pfirst[++np] = first[i + 1];
-        gprod[++g] = i + 1;
-      }
+    }
+    hsplit[k] = hs;
+    for (int i = p0; i < hs; i++) {
+      stage[i] = hbytes;
+      hbytes += c[i]->nchunks * c[i]->chunk_nbytes;
+    }
+    for (int i = hs; i < p1 && INA_SUCCEED(rc); i++) {
+      on_gpu[i] = 1;
+      split[i] = iax_split(&c[i]->cfg, c[i]->itemsize, c[i]->blocksize);
+      cap[i] = iamx_chunk_capacity(&p->tilings[i], split[i]);
+      if (cap[i] < c[i]->chunk_nbytes) rc = iax_dev_error(IAMX_EINVAL, "chunk capacity");
+      gout[i] = gbytes;
+      gbytes += c[i]->nchunks * cap[i];
+      sizes_bytes += c[i]->nchunks * 8;
     }
   }
   const int64_t gbase = (hbytes + 255) / 256 * 256;
+  for (int i = 0; i < n; i++)
+    if (on_gpu[i]) stage[i] = gbase + gout[i];
+  int np = 0, ng = 0;
+  pfirst[0] = 0;
+  int *gphase = (int *)malloc(sizeof(int) * (IAMX_COPY_EVENTS + 1)); /* phase of each piece */
+  uint8_t *is_group = (uint8_t *)calloc(IAMX_COPY_EVENTS + 1, 1);
+  if (!gphase || !is_group) rc = INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+  for (int k = 0; k < p->nphase && INA_SUCCEED(rc); k++) {
+    const int p0 = p->ph[k].p0, p1 = p->ph[k].p1, hs = hsplit[k];
+    if (hs > p0) { /* host pieces of about equal bytes */
+      const int64_t hb = stage[hs - 1] + c[hs - 1]->nchunks * c[hs - 1]->chunk_nbytes - stage[p0];
+      const int64_t piece = hb / IAX_PIECES_PER_PHASE > ((int64_t)4 << 20) ? hb / IAX_PIECES_PER_PHASE : ((int64_t)4 << 20);
+      int64_t at = 0;
+      int made = 0;
+      for (int i = p0; i < hs; i++)
+        for (int64_t q = 0; q < c[i]->nchunks; q++) {
+          at += c[i]->chunk_nbytes;
+          const int last = i == hs - 1 && q == c[i]->nchunks - 1;
+          if ((at >= piece * (made + 1) && made < IAX_PIECES_PER_PHASE - 1) || last) {
+            gphase[np] = k;
+            pfirst[++np] = first[i] + q + 1;
+            made++;
+          }
+        }
+    }
+    if (p1 > hs) { /* GPU groups of about equal bytes */
+      const int m = p1 - hs < IAX_GROUPS_PER_PHASE ? p1 - hs : IAX_GROUPS_PER_PHASE;
+      const int64_t gb = gout[p1 - 1] + c[p1 - 1]->nchunks * cap[p1 - 1] - gout[hs];
+      int64_t acc = 0;
+      int g = 0;
+      for (int i = hs; i < p1; i++) {
+        acc += c[i]->nchunks * cap[i];
+        if ((acc * m >= gb * (g + 1) && g < m - 1) || i == p1 - 1) {
+          g++;
+          gphase[np] = k;
+          is_group[np] = 1;
+          pfirst[++np] = first[i + 1];
+          ng++;
+        }
+      }
+    }
+  }
   uint8_t *h = INA_SUCCEED(rc) ? (uint8_t *)iamx_host_buffer((size_t)(gbase + gbytes + 256)) : NULL;
   if (INA_SUCCEED(rc) && !h) rc = iax_dev_error(IAMX_ENOMEM, "pinned staging buffer");
-  int e = 0;
-  /* host pieces first on the copy stream */
-  for (int k = 0; k < nhp && INA_SUCCEED(rc) && !e; k++) {
-    const int i0 = find_le(first, n, pfirst[k]), i1 = find_le(first, n, pfirst[k + 1] - 1);
-    const int64_t b0 = p->out_off[i0] + (pfirst[k] - first[i0]) * c[i0]->chunk_nbytes;
-    const int64_t b1 = p->out_off[i1] + (pfirst[k + 1] - first[i1]) * c[i1]->chunk_nbytes;
-    e = iamx_d2h_async(h + b0, (const uint8_t *)p->d_packed + b0, (size_t)(b1 - b0), k);
-  }
-  if (e) rc = iax_dev_error(e, "download");
-  /* then the GPU groups: encode on the side stream, download behind each */
-  if (INA_SUCCEED(rc) && ngpu > 0) {
+  /* GPU encoder buffers: one scratch region per group */
+  int64_t *gscr = (int64_t *)calloc((size_t)ng + 1, sizeof(int64_t));
+  uint8_t *d_scr = NULL, *d_out = NULL;
+  int64_t *d_sizes = NULL;
+  if (!gscr) rc = INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+  if (INA_SUCCEED(rc) && ng > 0) {
     int64_t scr = 0;
-    for (int g = 0; g < np - nhp && scr >= 0; g++) {
-      const int64_t b = iamx_encode_batch_scratch_bytes(gprod[g + 1] - gprod[g], p->tilings + gprod[g], split + gprod[g]);
+    for (int g = 0, q = 0; g < ng && scr >= 0; q++) {
+      if (!is_group[q]) continue;
+      const int i0 = find_le(first, n, pfirst[q]), i1 = find_le(first, n, pfirst[q + 1] - 1) + 1;
+      const int64_t b = iamx_encode_batch_scratch_bytes(i1 - i0, p->tilings + i0, split + i0);
+      gscr[g++] = scr;
       scr = b < 0 ? -1 : scr + (b + 255) / 256 * 256;
     }
-    uint8_t *d_scr = scr > 0 ? (uint8_t *)iamx_workspace(12, (size_t)scr) : NULL;
-    uint8_t *d_out = (uint8_t *)iamx_workspace(14, (size_t)gbytes + 256);
-    int64_t *d_sizes = (int64_t *)iamx_workspace(13, (size_t)sizes_bytes + 256);
+    d_scr = scr > 0 ? (uint8_t *)iamx_workspace(12, (size_t)scr) : NULL;
+    d_out = (uint8_t *)iamx_workspace(14, (size_t)gbytes + 256);
+    d_sizes = (int64_t *)iamx_workspace(13, (size_t)sizes_bytes + 256);
     if (scr <= 0 || !d_scr || !d_out || !d_sizes) rc = iax_dev_error(IAMX_ENOMEM, "batch encode buffers");
-    if (INA_SUCCEED(rc) && ((e = iamx_side_wait()) != 0 || (e = iamx_use_stream(1)) != 0))
-      rc = iax_dev_error(e, "encode");
-    int64_t so = 0, zo = 0;
-    for (int g = 0; g < np - nhp && INA_SUCCEED(rc); g++) {
-      const int i0 = gprod[g], m = gprod[g + 1] - gprod[g];
-      e = iamx_encode_batch(m, p->tilings + i0, split + i0, p->d_packed, p->out_off + i0, d_scr + so, d_out,
-                            gout + i0, d_sizes + zo);
-      const int64_t b0 = gout[i0], b1 = g + 1 < np - nhp ? gout[gprod[g + 1]] : gbytes;
-      if (!e) e = iamx_d2h_async(h + gbase + b0, d_out + b0, (size_t)(b1 - b0), nhp + g);
-      if (e) rc = iax_dev_error(e, "encode");
-      so += (iamx_encode_batch_scratch_bytes(m, p->tilings + i0, split + i0) + 255) / 256 * 256;
-      zo += first[gprod[g + 1]] - first[i0];
+  }
+  /* run: each phase's products, then its downloads and GPU groups */
+  int queued = 0, g = 0;
+  int64_t zo = 0;
+  for (int k = 0; k < p->nphase && INA_SUCCEED(rc); k++) {
+    int e = eager ? run_phase(p, k) : iamx_graph_launch(p->ph[k].graph);
+    for (; queued < np && gphase[queued] == k && !e; queued++) {
+      const int q = queued;
+      const int i0 = find_le(first, n, pfirst[q]), i1 = find_le(first, n, pfirst[q + 1] - 1);
+      if (!is_group[q]) {
+        const int64_t b0 = p->out_off[i0] + (pfirst[q] - first[i0]) * c[i0]->chunk_nbytes;
+        const int64_t b1 = p->out_off[i1] + (pfirst[q + 1] - first[i1]) * c[i1]->chunk_nbytes;
+        const int64_t s0 = stage[i0] + (pfirst[q] - first[i0]) * c[i0]->chunk_nbytes;
+        e = iamx_d2h_async(h + s0, (const uint8_t *)p->d_packed + b0, (size_t)(b1 - b0), q);
+      } else {
+        const int m = i1 + 1 - i0;
+        if ((e = iamx_side_wait()) == 0 && (e = iamx_use_stream(1)) == 0) {
+          e = iamx_encode_batch(m, p->tilings + i0, split + i0, p->d_packed, p->out_off + i0, d_scr + gscr[g], d_out,
+                                gout + i0, d_sizes + zo);
+          const int64_t b0 = gout[i0], b1 = gout[i1] + c[i1]->nchunks * cap[i1];
+          if (!e) e = iamx_d2h_async(h + gbase + b0, d_out + b0, (size_t)(b1 - b0), q);
+        }
+        iamx_use_stream(0);
+        zo += first[i1 + 1] - first[i0];
+        g++;
+      }
     }
-    iamx_use_stream(0);
+    if (e) rc = iax_dev_error(e, "batch phase");
   }
   if (INA_FAILED(rc)) {
-    for (int k = 0; k < np; k++) (void)iamx_event_sync(k);
+    for (int q = 0; q < queued; q++) (void)iamx_event_sync(q);
   } else {
     wctx_t x;
     x.c = c;
     x.first = first;
     x.n = n;
-    x.hsplit = hsplit;
+    x.on_gpu = on_gpu;
     x.h = h;
-    x.out_off = p->out_off;
-    x.gout_off = gout;
+    x.stage = stage;
     x.cap = cap;
-    x.gbase = gbase;
     x.pfirst = pfirst;
     x.npieces = np;
     x.filtered = c[0]->cfg.compression_level > 0;
-    for (int k = 0; k < IAMX_COPY_EVENTS; k++) atomic_init(&x.landed[k], 0);
+    for (int q = 0; q < IAMX_COPY_EVENTS; q++) atomic_init(&x.landed[q], 0);
     atomic_init(&x.err, 0);
     iax_parallel_for(iax_nthreads(&ctx->cfg), first[n], write_task, &x);
     if (atomic_load(&x.err)) {
-      for (int k = 0; k < np; k++) (void)iamx_event_sync(k);
+      for (int q = 0; q < np; q++) (void)iamx_event_sync(q);
       rc = iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "chunk encode failed");
     }
-    for (int i = hsplit; i < n && INA_SUCCEED(rc); i++) {
+    for (int i = 0; i < n && INA_SUCCEED(rc); i++) {
+      if (!on_gpu[i]) continue;
       iax_last_gpu_enc_in += c[i]->nchunks * c[i]->chunk_nbytes;
-      for (int64_t k = 0; k < c[i]->nchunks; k++) iax_last_gpu_enc_out += c[i]->csizes[k];
+      for (int64_t q = 0; q < c[i]->nchunks; q++) iax_last_gpu_enc_out += c[i]->csizes[q];
     }
   }
-  free(first); free(gout); free(cap); free(split); free(pfirst); free(gprod);
+  free(first); free(stage); free(gout); free(cap); free(split); free(on_gpu); free(pfirst);
+  free(hsplit); free(gphase); free(is_group); free(gscr);
   return rc;
 }
 
@@ -534,25 +638,16 @@ INA_API(ina_rc_t) iarray_linalg_matmul_batch(iarray_context_t *ctx, int n, iarra
   /* IARRAY_BATCH_EAGER=1 launches the captured stages one by one (same
    * kernels and arguments), so each can be timed on its own */
   const char *eager = getenv("IARRAY_BATCH_EAGER");
-  int e;
-  if (eager && atoi(eager) == 1) {
-    e = iamx_decode(g_plan.d_lz4, g_plan.n_lz4, g_plan.d_blocks, g_plan.n_blk, g_plan.d_scratch, g_plan.d_dense);
-    if (!e)
-      e = iamx_gemm_batch(dt, g_plan.d_prods, g_plan.d_tiles, g_plan.ntiles, g_plan.d_dense,
-                          (uint8_t *)g_plan.d_dense + g_plan.b_base, g_plan.d_dense_c);
-    if (!e) e = iamx_pack_batch(g_plan.d_packs, g_plan.d_pblocks, g_plan.npb, g_plan.d_dense_c, g_plan.d_packed);
-  } else {
-    e = iamx_graph_launch(g_plan.graph);
-  }
-  if (e) return iax_dev_error(e, "batch replay");
-  int st = iamx_decode_status();
+  const double t0 = iax_now();
+  iax_last_gpu_enc_in = iax_last_gpu_enc_out = 0;
+  rc = batch_write(ctx, n, c, eager && atoi(eager) == 1);
+  iax_last_encode_s = iax_now() - t0;
+  /* the decode status of every phase (a malformed stream leaves its flag) */
+  const int st = iamx_decode_status();
   iax_last_device_ms = iamx_prof_span_ms();
+  if (INA_FAILED(rc)) return rc;
   if (st)
     return st < 0 ? iax_dev_error(st, "batch status")
                   : iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "malformed LZ4 stream (0x%x)", st);
-  const double t0 = iax_now();
-  iax_last_gpu_enc_in = iax_last_gpu_enc_out = 0;
-  const ina_rc_t rc2 = batch_write(ctx, n, c);
-  iax_last_encode_s = iax_now() - t0;
-  return rc2;
+  return INA_SUCCESS;
 }
