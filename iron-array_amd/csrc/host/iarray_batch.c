@@ -54,7 +54,7 @@ typedef struct {
   uint64_t *gen;
   int64_t *store; /* chunk/block shapes of each output, 4 per product */
   int dt;
-  int nphase;
+  int nphase, nphase_asked; /* phases made, and the count asked for when the plan was built */
   batch_phase_t ph[IAX_BATCH_MAX_PHASES];
   /* device buffers owned by the plan (their addresses are baked into the graph) */
   void *d_dense, *d_dense_c, *d_scratch, *d_packed;
@@ -96,7 +96,7 @@ static int batch_phases(int n) {
 
 static int same_signature(const batch_plan_t *p, int n, iarray_container_t **a, iarray_container_t **b,
                           const iarray_storage_t *st, int dt) {
-  if (p->nphase < 1 || p->n != n || p->dt != dt || p->nphase != batch_phases(n)) return 0;
+  if (p->nphase < 1 || p->n != n || p->dt != dt || p->nphase_asked != batch_phases(n)) return 0;
   for (int i = 0; i < n; i++) {
     if (p->a[i] != a[i] || p->b[i] != b[i]) return 0;
     if (!a[i]->device_valid || !b[i]->device_valid) return 0;
@@ -224,6 +224,7 @@ static ina_rc_t build_plan(batch_plan_t *p, int n, iarray_container_t **a, iarra
   /* phases of about equal output bytes */
   const int nph = batch_phases(n);
   p->nphase = nph;
+  p->nphase_asked = nph;
   {
     int k = 0;
     p->ph[0].p0 = 0;

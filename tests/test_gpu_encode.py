@@ -187,18 +187,23 @@ def test_mixed_host_and_gpu_panels(monkeypatch, orc, share):
     assert dense.tobytes() == got.tobytes()
 
 
-@pytest.mark.parametrize("share", ["0.5", "1"])
-def test_batch_outputs_encoded_on_the_gpu(monkeypatch, orc, share):
-    """A batch whose later products are encoded by one GPU encoder pass per
-    group (ragged f32 products, several chunks each, edge chunks): every
-    chunk valid, every output equal to the all-host batch and decoding to the
-    same values through the oracle."""
-    print(f"case: batch gpu encode share {share}", flush=True)
+@pytest.mark.parametrize("share,phases,skew", [("0.5", "2", False), ("1", "1", False), ("1", "4", False),
+                                               ("0.7", "4", True)])
+def test_batch_outputs_encoded_on_the_gpu(monkeypatch, orc, share, phases, skew):
+    """A batch run in phases whose later products are encoded by one GPU
+    encoder pass per group (ragged f32 products, several chunks each, edge
+    chunks; `skew`: one product holds most of the output, so fewer phases are
+    made than asked): every chunk valid, every output equal to the all-host
+    one-phase batch and decoding to the same values through the oracle, also
+    when the plan is replayed."""
+    print(f"case: batch gpu encode share {share} phases {phases} skew {skew}", flush=True)
     ctx = ia.Context(clevel=5, filters=ia.COMP_SHUFFLE)
     rng = np.random.default_rng(11)
     pairs, cs_l, bs_l = [], [], []
-    for _ in range(24):
+    for j in range(24):
         M, N, K = (int(v) for v in rng.integers(40, 400, 3))
+        if skew:
+            M, N = (600, 500) if j == 0 else (int(M) // 8 + 1, int(N) // 8 + 1)
         a = rng.standard_normal((M, K)).astype(np.float32)
         b = rng.standard_normal((K, N)).astype(np.float32)
         ca = ia.Container.from_numpy(ctx, a, (min(M, 128), min(K, 128)), (min(M, 64), min(K, 64)))
@@ -207,9 +212,11 @@ def test_batch_outputs_encoded_on_the_gpu(monkeypatch, orc, share):
         cs_l.append((min(M, 150), min(N, 120)))
         bs_l.append((min(M, 50), min(N, 40)))
     monkeypatch.setenv("IARRAY_GPU_ENCODE", "0")
+    monkeypatch.setenv("IARRAY_BATCH_PHASES", "1")
     ref = [z.to_numpy() for z in ia.matmul_batch(pairs, cs_l, bs_l)]
     monkeypatch.setenv("IARRAY_GPU_ENCODE", share)
-    for rep in range(2):  # the second call replays the graph
+    monkeypatch.setenv("IARRAY_BATCH_PHASES", phases)
+    for rep in range(2):  # the second call replays the graphs
         zs = ia.matmul_batch(pairs, cs_l, bs_l)
         for z, r, cs, bs in zip(zs, ref, cs_l, bs_l):
             got = z.to_numpy()
