@@ -39,6 +39,10 @@ IAMX_API int iamx_h2d(void *dst, const void *src, size_t bytes);
 IAMX_API int iamx_d2h(void *dst, const void *src, size_t bytes);
 IAMX_API int iamx_d2d(void *dst, const void *src, size_t bytes);
 IAMX_API int iamx_memset(void *dst, int value, size_t bytes);
+/* height rows of width bytes, device to device, on the current stream (the
+ * rectangle of a sliced view out of its container's dense image,
+ * src/iarray_container.c:628-736) */
+IAMX_API int iamx_copy2d(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, size_t height);
 IAMX_API int iamx_sync(void);
 /* Pinned host staging buffer (grows, kept until iamx_shutdown) and
  * downloads into it: iamx_d2h_async queues the copy on a copy stream, after

@@ -326,6 +326,15 @@ IAMX_API int iamx_d2d(void *dst, const void *src, size_t bytes) {
   return IAMX_OK;
 }
 
+IAMX_API int iamx_copy2d(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, size_t height) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (!width || !height) return IAMX_OK;
+  if (width > dpitch || width > spitch) return set_error(IAMX_EINVAL, "iamx_copy2d: width %zu exceeds a pitch", width);
+  IAMX_HIP_CHECK(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, height, hipMemcpyDeviceToDevice, g_cur));
+  return IAMX_OK;
+}
+
 IAMX_API int iamx_memset(void *dst, int value, size_t bytes) {
   int rc = ensure_init();
   if (rc) return rc;

@@ -567,9 +567,25 @@ ina_rc_t iax_operand(iarray_container_t *c, int slot, int tmp_slot, void **d_out
   const size_t bytes = (size_t)(base->shape[0] * base->shape[1] * base->itemsize);
   void *dst = iamx_workspace(slot, bytes + 16);
   if (!dst) return iax_dev_error(iamx_init() ? iamx_init() : IAMX_ENOMEM, "operand buffer");
-  if (!c->transposed) {
+  if (!c->viewed) {
     ina_rc_t rc = iax_decode_dense(base, dst, 0);
     if (INA_FAILED(rc)) return rc;
+    *d_out = dst;
+    return INA_SUCCESS;
+  }
+  if (!c->transposed) { /* a slice: its rectangle out of the base's image */
+    void *img = iamx_workspace(tmp_slot, bytes + 16);
+    if (!img) return iax_dev_error(IAMX_ENOMEM, "slice buffer");
+    ina_rc_t rc = iax_decode_dense(base, img, 0);
+    if (INA_FAILED(rc)) return rc;
+    const int64_t rows = c->dtshape.ndim == 2 ? c->dtshape.shape[0] : 1;
+    const int64_t cols = c->dtshape.ndim == 2 ? c->dtshape.shape[1] : c->dtshape.shape[0];
+    const size_t is = (size_t)base->itemsize;
+    if (c->voff[0] < 0 || c->voff[1] < 0 || c->voff[0] + rows > base->shape[0] || c->voff[1] + cols > base->shape[1])
+      return iax_set_error(INA_ERROR(IARRAY_ERR_ASSERTION_FAILED), "slice outside its container");
+    const uint8_t *src = (const uint8_t *)img + ((size_t)c->voff[0] * (size_t)base->shape[1] + (size_t)c->voff[1]) * is;
+    int e = iamx_copy2d(dst, (size_t)cols * is, src, (size_t)base->shape[1] * is, (size_t)cols * is, (size_t)rows);
+    if (e) return iax_dev_error(e, "slice copy");
     *d_out = dst;
     return INA_SUCCESS;
   }
@@ -1133,6 +1149,63 @@ INA_API(ina_rc_t) iarray_linalg_transpose(iarray_context_t *ctx, iarray_containe
   v->plan_valid = 0;
   memset(&v->plan, 0, sizeof(v->plan));
   *b = v;
+  return INA_SUCCESS;
+}
+
+/* src/iarray_container.c:364-455.  Negative start / stop count from the end;
+ * start > stop is INA_ERR_INVALID_ARGUMENT, a copy whose chunkshape exceeds the
+ * slice IARRAY_ERR_INVALID_CHUNKSHAPE.  A view shares its container's chunks
+ * (a slice of a slice starts at the sum of the offsets); a copy is decoded on
+ * the GPU and written with `storage`.  Slices of transposed views are not
+ * supported. */
+INA_API(ina_rc_t) iarray_get_slice(iarray_context_t *ctx, iarray_container_t *src, const int64_t *start,
+                                   const int64_t *stop, bool view, iarray_storage_t *storage,
+                                   iarray_container_t **container) {
+  FAIL_NULL(ctx);
+  FAIL_NULL(src);
+  FAIL_NULL(start);
+  FAIL_NULL(stop);
+  FAIL_NULL(container);
+  if (!view) FAIL_NULL(storage);
+  const int nd = src->dtshape.ndim;
+  iarray_dtshape_t dts = src->dtshape;
+  int64_t s[2] = {0, 0};
+  for (int i = 0; i < nd; i++) {
+    const int64_t dim = src->dtshape.shape[i];
+    const int64_t a = start[i] < 0 ? start[i] + dim : start[i];
+    const int64_t b = stop[i] < 0 ? stop[i] + dim : stop[i];
+    if (a > b) return iax_set_error(INA_ERROR(INA_ERR_INVALID_ARGUMENT), "Start is bigger than stop");
+    if (a < 0 || b > dim) return iax_set_error(INA_ERROR(INA_ERR_INVALID_ARGUMENT), "slice outside the array (axis %d)", i);
+    if (!view && storage->chunkshape[i] > b - a)
+      return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_CHUNKSHAPE), "The chunkshape is bigger than shape");
+    s[nd == 2 ? i : 1] = a;
+    dts.shape[i] = b - a;
+  }
+  if (!view) {
+    const int64_t n = (nd == 2 ? dts.shape[0] * dts.shape[1] : dts.shape[0]) * src->itemsize;
+    void *buf = malloc((size_t)(n > 0 ? n : 1));
+    if (!buf) return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+    ina_rc_t rc = iarray_get_slice_buffer(ctx, src, start, stop, buf, n);
+    if (INA_SUCCEED(rc)) rc = iarray_from_buffer(ctx, &dts, buf, n, storage, container);
+    free(buf);
+    return rc;
+  }
+  if (src->transposed) return iax_set_error(INA_ERROR(INA_ERR_NOT_SUPPORTED), "slices of transposed views are not supported");
+  iarray_container_t *base = src->viewed ? src->viewed : src;
+  iarray_container_t *v = (iarray_container_t *)calloc(1, sizeof(*v));
+  if (!v) return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+  *v = *base;
+  v->dtshape = dts;
+  v->viewed = base;
+  v->transposed = false;
+  v->voff[0] = src->voff[0] + s[0];
+  v->voff[1] = src->voff[1] + s[1];
+  v->plan_valid = 0;
+  memset(&v->plan, 0, sizeof(v->plan));
+  v->frame.valid = 0;
+  v->on_disk = false;
+  v->tuned_b = NULL;
+  *container = v;
   return INA_SUCCESS;
 }
 

@@ -70,9 +70,11 @@ struct iarray_container_s {
   uint64_t generation; /* bumped whenever the device arena is (re)built */
   iax_plan_t plan;
   int plan_valid;
-  /* views */
+  /* views: a transposed view, or a slice of `viewed` starting at voff (2-D
+   * normalised coordinates of the viewed container) */
   iarray_container_t *viewed;
   bool transposed;
+  int64_t voff[2];
   /* frames: header fields of the frame it was read from; opened (rather than
    * loaded) containers count as stored on disk */
   iax_frame_params_t frame;

@@ -75,6 +75,8 @@ ERR_BLOSC_FAILED = err_code("FAILED", "BLOSC")
 ERR_CATERVA_FAILED = err_code("FAILED", "CATERVA")
 ERR_NO_DEVICE = err_code("NOT_SUPPORTED", "DEVICE")
 ERR_FILE_OPEN = (_K["FAILED"] << 32) | 0x05  # libinac's file subject, not a user-defined one
+ERR_NOT_SUPPORTED = 0x06 << 32  # INA_ERR_NOT_SUPPORTED (no subject)
+ERR_INVALID_ARGUMENT = 0x08 << 32  # INA_ERR_INVALID_ARGUMENT (no subject)
 
 _lib = None
 
@@ -107,6 +109,7 @@ def lib():
         "iarray_linalg_matmul_params": ([vp, vp, vp, P(Storage), P(MatmulParams), P(vp)], rc_t),
         "iarray_linalg_matmul_batch": ([vp, ctypes.c_int, P(vp), P(vp), P(Storage), P(MatmulParams), P(vp)], rc_t),
         "iarray_linalg_transpose": ([vp, vp, P(vp)], rc_t),
+        "iarray_get_slice": ([vp, vp, P(i64), P(i64), ctypes.c_bool, P(Storage), P(vp)], rc_t),
         "iarray_opt_gemv": ([vp, vp, vp, P(Storage), P(vp)], rc_t),
         "iarray_opt_gemm": ([vp, vp, vp, P(Storage), P(vp)], rc_t),
         "iarray_opt_gemm_a": ([vp, vp, vp, P(Storage), P(vp)], rc_t),
@@ -351,6 +354,21 @@ class Container:
         p = ctypes.c_void_p()
         _check(lib().iarray_linalg_transpose(self.ctx.ptr, self.ptr, ctypes.byref(p)))
         return Container(p, self.ctx, self.dtype, self.shape[::-1], base=self)
+
+    def get_slice(self, start, stop, view=True, chunks=None, blocks=None):
+        """iarray_get_slice: a view of [start, stop), or a copy written with chunks / blocks."""
+        st = (ctypes.c_int64 * len(start))(*start)
+        sp = (ctypes.c_int64 * len(stop))(*stop)
+        s = _storage(chunks or [], blocks or [])
+        p = ctypes.c_void_p()
+        _check(lib().iarray_get_slice(self.ctx.ptr, self.ptr, st, sp, view, ctypes.byref(s), ctypes.byref(p)))
+        d = DTShape()
+        _check(lib().iarray_get_dtshape(self.ctx.ptr, p, ctypes.byref(d)))
+        shape = tuple(d.shape[i] for i in range(d.ndim))
+        out = Container(p, self.ctx, self.dtype, shape, base=self if view else None)
+        if not view:
+            out._cs, out._bs = tuple(chunks), tuple(blocks)
+        return out
 
 
 def _out_shape(a, b):

@@ -313,6 +313,40 @@ def test_matmul_through_frames(ctx, tmp_path, monkeypatch, share):
     assert not (tmp_path / "zarr.iarray").exists()
 
 
+def test_slice_views_decode_and_multiply(ctx):
+    """iarray_get_slice (src/iarray_container.c:364-455) on the GPU: slice
+    views (negative bounds, a slice of a slice, a 1-D slice) decode to the
+    numpy slices and multiply as operands of the gemm and gemv branches; a
+    copy is a new container with the storage asked for.  Integer-valued
+    float64 operands make every product exact."""
+    rng = np.random.default_rng(21)
+    a = rng.integers(-8, 9, (300, 260)).astype(np.float64)
+    b = rng.integers(-8, 9, (260, 170)).astype(np.float64)
+    x = rng.integers(-8, 9, 260).astype(np.float64)
+    ca = ia.Container.from_numpy(ctx, a, (100, 90), (25, 30))
+    cb = ia.Container.from_numpy(ctx, b, (90, 80), (30, 20))
+    cx = ia.Container.from_numpy(ctx, x, (90,), (30,))
+    print("case: slice views", flush=True)
+    sa, sb, sx = a[13:-21, 7:207], b[7:207, 3:-5], x[7:207]
+    va = ca.get_slice((13, 7), (-21, 207))
+    vb = cb.get_slice((7, 3), (207, -5))
+    vx = cx.get_slice((7,), (207,))
+    assert va.shape == sa.shape and vb.shape == sb.shape and vx.shape == sx.shape
+    assert va.to_numpy().tobytes() == np.ascontiguousarray(sa).tobytes()
+    assert vx.to_numpy().tobytes() == np.ascontiguousarray(sx).tobytes()
+    assert np.array_equal(vb.slice((5, 9), (50, 100)), sb[5:50, 9:100])
+    w = va.get_slice((10, 20), (60, 90))
+    assert w.to_numpy().tobytes() == np.ascontiguousarray(sa[10:60, 20:90]).tobytes()
+    z = ia.matmul(va, vb, (100, 60), (50, 30)).to_numpy()
+    assert np.array_equal(z, sa @ sb)
+    y = ia.matmul(va, vx, (100,), (50,)).to_numpy()
+    assert np.array_equal(y, sa @ sx)
+    cp = ca.get_slice((13, 7), (-21, 207), view=False, chunks=(64, 64), blocks=(32, 32))
+    assert cp.to_numpy().tobytes() == np.ascontiguousarray(sa).tobytes()
+    z2 = ia.matmul(cp, vb, (100, 60), (50, 30)).to_numpy()
+    assert z2.tobytes() == z.tobytes()
+
+
 def test_matmul_deterministic(ctx):
     rng = np.random.default_rng(2)
     a = rng.standard_normal((400, 300))

@@ -374,6 +374,29 @@ def test_frame_save_errors_and_remove(tmp_path):
     assert _err(lambda: src.save(str(tmp_path / "no" / "such" / "dir.iarray"))) == ia.ERR_BLOSC_FAILED
 
 
+def test_get_slice_views_and_errors():
+    """iarray_get_slice (src/iarray_container.c:364-455): a view's shape,
+    negative bounds, slices of slices, and the argument errors, all before any
+    decode (no GPU needed)."""
+    ctx = _ctx()
+    src = ia.Container.load(ctx, FRAMES[0])
+    c = ia.Container.from_chunks(ctx, src.dtype, (10, 100000), (1, 100000), (1, 10000), src.chunks())
+    v = c.get_slice((2, 100), (-1, 5000))
+    assert v.shape == (7, 4900)
+    is_view = ctypes.c_bool()
+    ia._check(ia.lib().iarray_is_view(ctx.ptr, v.ptr, ctypes.byref(is_view)))
+    assert is_view.value
+    w = v.get_slice((1, 10), (3, 20))
+    assert w.shape == (2, 10)
+    assert _err(lambda: c.get_slice((5, 0), (2, 10))) == ia.ERR_INVALID_ARGUMENT
+    assert _err(lambda: c.get_slice((0, 0), (11, 10))) == ia.ERR_INVALID_ARGUMENT
+    assert _err(lambda: c.get_slice((0, 0), (4, 100), view=False, chunks=(5, 10), blocks=(5, 10))) == \
+        ia.ERR_INVALID_CHUNKSHAPE
+    assert _err(lambda: c.transpose().get_slice((0, 0), (2, 2))) == ia.ERR_NOT_SUPPORTED
+    one = src.get_slice((999000,), (-1,))
+    assert one.shape == (999,)
+
+
 def test_constructor_errors():
     ctx = _ctx()
     small = np.zeros(10)
