@@ -241,8 +241,8 @@ INA_API(ina_rc_t) iarray_container_almost_equal(iarray_container_t *a, iarray_co
 INA_API(ina_rc_t) iarray_linalg_matmul(iarray_context_t *ctx, iarray_container_t *a, iarray_container_t *b,
                                        iarray_storage_t *storage, iarray_container_t **c);
 /* iarray.h:560-566, src/iarray_container.c:364-455: a slice as a view of
- * src (sharing its chunks) or as a new container written with `storage`.
- * Slices of transposed views return INA_ERR_NOT_SUPPORTED. */
+ * src (sharing its chunks) or as a new container written with `storage`; a
+ * slice of a transposed view is the transposed slice of its container. */
 INA_API(ina_rc_t) iarray_get_slice(iarray_context_t *ctx, iarray_container_t *src, const int64_t *start,
                                    const int64_t *stop, bool view, iarray_storage_t *storage,
                                    iarray_container_t **container);

@@ -593,9 +593,23 @@ ina_rc_t iax_operand(iarray_container_t *c, int slot, int tmp_slot, void **d_out
   if (!tmp) return iax_dev_error(IAMX_ENOMEM, "transpose buffer");
   ina_rc_t rc = iax_decode_dense(base, tmp, 0);
   if (INA_FAILED(rc)) return rc;
-  int e = iamx_transpose(base->itemsize, base->shape[0], base->shape[1], tmp, dst);
-  if (e) return iax_dev_error(e, "transpose");
-  *d_out = dst;
+  /* the rectangle of base rows x cols the view transposes */
+  const int64_t rows = c->dtshape.shape[1], cols = c->dtshape.shape[0];
+  if (rows == base->shape[0] && cols == base->shape[1]) {
+    int e = iamx_transpose(base->itemsize, base->shape[0], base->shape[1], tmp, dst);
+    if (e) return iax_dev_error(e, "transpose");
+    *d_out = dst;
+    return INA_SUCCESS;
+  }
+  /* a slice of a transposed view: cut the rectangle into dst, transpose it back into tmp */
+  const size_t is = (size_t)base->itemsize;
+  if (c->voff[0] < 0 || c->voff[1] < 0 || c->voff[0] + rows > base->shape[0] || c->voff[1] + cols > base->shape[1])
+    return iax_set_error(INA_ERROR(IARRAY_ERR_ASSERTION_FAILED), "slice outside its container");
+  const uint8_t *src = (const uint8_t *)tmp + ((size_t)c->voff[0] * (size_t)base->shape[1] + (size_t)c->voff[1]) * is;
+  int e = iamx_copy2d(dst, (size_t)cols * is, src, (size_t)base->shape[1] * is, (size_t)cols * is, (size_t)rows);
+  if (!e) e = iamx_transpose(base->itemsize, rows, cols, dst, tmp);
+  if (e) return iax_dev_error(e, "transposed slice");
+  *d_out = tmp;
   return INA_SUCCESS;
 }
 
@@ -1156,8 +1170,8 @@ INA_API(ina_rc_t) iarray_linalg_transpose(iarray_context_t *ctx, iarray_containe
  * start > stop is INA_ERR_INVALID_ARGUMENT, a copy whose chunkshape exceeds the
  * slice IARRAY_ERR_INVALID_CHUNKSHAPE.  A view shares its container's chunks
  * (a slice of a slice starts at the sum of the offsets); a copy is decoded on
- * the GPU and written with `storage`.  Slices of transposed views are not
- * supported. */
+ * the GPU and written with `storage`.  A slice of a transposed view is the
+ * transposed slice of its container. */
 INA_API(ina_rc_t) iarray_get_slice(iarray_context_t *ctx, iarray_container_t *src, const int64_t *start,
                                    const int64_t *stop, bool view, iarray_storage_t *storage,
                                    iarray_container_t **container) {
@@ -1190,16 +1204,17 @@ INA_API(ina_rc_t) iarray_get_slice(iarray_context_t *ctx, iarray_container_t *sr
     free(buf);
     return rc;
   }
-  if (src->transposed) return iax_set_error(INA_ERROR(INA_ERR_NOT_SUPPORTED), "slices of transposed views are not supported");
   iarray_container_t *base = src->viewed ? src->viewed : src;
   iarray_container_t *v = (iarray_container_t *)calloc(1, sizeof(*v));
   if (!v) return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
   *v = *base;
   v->dtshape = dts;
   v->viewed = base;
-  v->transposed = false;
-  v->voff[0] = src->voff[0] + s[0];
-  v->voff[1] = src->voff[1] + s[1];
+  /* offsets are kept in the base's coordinates: a transposed view's rows are
+   * the base's columns (a slice of a transposed view stays transposed) */
+  v->transposed = src->transposed;
+  v->voff[0] = src->voff[0] + (src->transposed ? s[1] : s[0]);
+  v->voff[1] = src->voff[1] + (src->transposed ? s[0] : s[1]);
   v->plan_valid = 0;
   memset(&v->plan, 0, sizeof(v->plan));
   v->frame.valid = 0;

@@ -341,6 +341,13 @@ def test_slice_views_decode_and_multiply(ctx):
     assert np.array_equal(z, sa @ sb)
     y = ia.matmul(va, vx, (100,), (50,)).to_numpy()
     assert np.array_equal(y, sa @ sx)
+    st = cb.transpose().get_slice((3, 7), (-5, 207))  # (b.T)[3:-5, 7:207] = (b[7:207, 3:-5]).T
+    assert st.shape == sb.T.shape
+    assert st.to_numpy().tobytes() == np.ascontiguousarray(sb.T).tobytes()
+    st2 = st.get_slice((4, 10), (60, 150))
+    assert st2.to_numpy().tobytes() == np.ascontiguousarray(sb.T[4:60, 10:150]).tobytes()
+    zt = ia.matmul(st, va.get_slice((0, 0), (200, 100)), (60, 50), (30, 25)).to_numpy()
+    assert np.array_equal(zt, sb.T @ sa[:200, :100])
     cp = ca.get_slice((13, 7), (-21, 207), view=False, chunks=(64, 64), blocks=(32, 32))
     assert cp.to_numpy().tobytes() == np.ascontiguousarray(sa).tobytes()
     z2 = ia.matmul(cp, vb, (100, 60), (50, 30)).to_numpy()
