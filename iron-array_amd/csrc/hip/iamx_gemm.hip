@@ -42,14 +42,13 @@ constexpr int BM = 128, BN = 128, PAD = 4;
 
 // Map the launch index onto output tiles so that the 8 XCDs (workgroups are
 // dealt round-robin to XCDs) each get a contiguous band of tile rows: tiles
-// that share A rows then share one L2.
+// that share A rows then share one L2.  XCD x receives nt / 8 workgroups, one
+// more when x < nt % 8, and its band starts after the bands of XCDs 0..x-1, so
+// the map is a bijection for every tile count.
 __device__ __forceinline__ void tile_coords(int &tm, int &tn, int tiles_m, int tiles_n) {
   const int nt = tiles_m * tiles_n;
-  int id = blockIdx.x;
-  if (nt % 8 == 0) {
-    const int per = nt / 8;
-    id = (id % 8) * per + id / 8;
-  }
+  const int q = nt / 8, r = nt % 8, x = (int)blockIdx.x % 8;
+  const int id = x * q + min(x, r) + (int)blockIdx.x / 8;
   tm = id / tiles_n;
   tn = id % tiles_n;
 }
