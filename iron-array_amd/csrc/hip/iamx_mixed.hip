@@ -371,9 +371,10 @@ __global__ __launch_bounds__(256) void k_seg_gemm_big(const SegArgs P, int level
   constexpr int BUF = (BM + BN) * ROW2;
   __shared__ __attribute__((aligned(16))) uint8_t lds[2 * BUF];  // [2][A 256 rows | B 256 rows] x ROW2
   typedef typename Acc32<MODE>::t acc_t;
-  int id = blockIdx.x;
+  // XCD x gets nt / 8 tiles (one more when x < nt % 8) in a contiguous band
   const int nt = tiles_m * P.tiles_n;
-  if (nt % 8 == 0) id = (id % 8) * (nt / 8) + id / 8;
+  const int xq = nt / 8, xr = nt % 8, xcd = (int)blockIdx.x % 8;
+  const int id = xcd * xq + min(xcd, xr) + (int)blockIdx.x / 8;
   // grouped order: consecutive ids walk 4 tile rows, then the next column, so
   // the tiles running together share few A and B panels
   constexpr int GM = 4;
