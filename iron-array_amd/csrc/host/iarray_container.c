@@ -562,17 +562,48 @@ ina_rc_t iax_decode_dense(iarray_container_t *c, void *d_dst, int scratch_slot) 
   return INA_SUCCESS;
 }
 
+/* Which container's dense image each workspace slot holds (its device
+ * generation, unique across containers, and the buffer address): a plain
+ * container asked for again in the same slot, unchanged, is not decoded again
+ * (the GEMV's x every call).  Every other writer of a slot forgets it. */
+#define IAX_IMG_SLOTS 16
+static struct {
+  uint64_t gen;
+  const void *ptr;
+} g_img[IAX_IMG_SLOTS];
+
+void iax_operand_forget(int slot) {
+  if (slot < 0) {
+    memset(g_img, 0, sizeof(g_img));
+  } else if (slot < IAX_IMG_SLOTS) {
+    g_img[slot].gen = 0;
+    g_img[slot].ptr = NULL;
+  }
+}
+
 ina_rc_t iax_operand(iarray_container_t *c, int slot, int tmp_slot, void **d_out) {
   iarray_container_t *base = c->viewed ? c->viewed : c;
   const size_t bytes = (size_t)(base->shape[0] * base->shape[1] * base->itemsize);
   void *dst = iamx_workspace(slot, bytes + 16);
   if (!dst) return iax_dev_error(iamx_init() ? iamx_init() : IAMX_ENOMEM, "operand buffer");
   if (!c->viewed) {
+    if (slot >= 0 && slot < IAX_IMG_SLOTS && c->device_valid && g_img[slot].gen == c->generation &&
+        g_img[slot].ptr == dst) {
+      *d_out = dst;
+      return INA_SUCCESS;
+    }
+    iax_operand_forget(slot);
     ina_rc_t rc = iax_decode_dense(base, dst, 0);
     if (INA_FAILED(rc)) return rc;
+    if (slot >= 0 && slot < IAX_IMG_SLOTS && c->device_valid) {
+      g_img[slot].gen = c->generation;
+      g_img[slot].ptr = dst;
+    }
     *d_out = dst;
     return INA_SUCCESS;
   }
+  iax_operand_forget(slot);
+  iax_operand_forget(tmp_slot);
   if (!c->transposed) { /* a slice: its rectangle out of the base's image */
     void *img = iamx_workspace(tmp_slot, bytes + 16);
     if (!img) return iax_dev_error(IAMX_ENOMEM, "slice buffer");

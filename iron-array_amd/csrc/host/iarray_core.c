@@ -62,6 +62,7 @@ INA_API(ina_rc_t) iarray_init(void) {
 }
 
 INA_API(void) iarray_destroy(void) {
+  iax_operand_forget(-1);
   if (g_initialized) iamx_shutdown();
   g_initialized = 0;
 }

@@ -116,6 +116,7 @@ static ina_rc_t decode_operands(iarray_container_t *a, iarray_container_t *b, vo
     if (INA_FAILED(rc)) return rc;
   }
   const size_t bytes = (size_t)((g_pair.b_off + b->shape[0] * b->shape[1]) * a->itemsize);
+  iax_operand_forget(2);
   uint8_t *dense = (uint8_t *)iamx_workspace(2, bytes + 16);
   void *scratch = iamx_workspace(0, (size_t)g_pair.scratch_bytes + 16);
   if (!dense || !scratch) return iax_dev_error(IAMX_ENOMEM, "operand buffers");
@@ -218,6 +219,7 @@ static ina_rc_t run_gemv(iarray_container_t *a, iarray_container_t *x, iarray_co
     const iax_plan_t *p = &a->plan;
     void *scratch = iamx_workspace(0, (size_t)p->scratch_bytes + 16);
     const size_t pbytes = (size_t)(p->nbcols * M * a->itemsize);
+    iax_operand_forget(6);
     void *partial = iamx_workspace(6, pbytes + 16);
     if (!scratch || !partial) return iax_dev_error(IAMX_ENOMEM, "gemv scratch");
     e = iamx_memset(partial, 0, pbytes);

@@ -140,6 +140,9 @@ ina_rc_t iax_decode_dense(iarray_container_t *c, void *d_dst, int scratch_slot);
 /* Decode a container or transposed view as an operand: dense logical-shape
  * row-major device buffer in workspace slot `slot` (uses `tmp_slot` for views). */
 ina_rc_t iax_operand(iarray_container_t *c, int slot, int tmp_slot, void **d_out);
+/* The dense image in workspace `slot` is about to be overwritten by other
+ * code (slot < 0: all slots, e.g. when the device runtime shuts down). */
+void iax_operand_forget(int slot);
 /* Write C from a dense device buffer (logical shape, ld = shape1): GPU pack +
  * host LZ4 encode; fills c->chunks. */
 ina_rc_t iax_write_from_dense(iarray_container_t *c, const void *d_dense);

@@ -354,6 +354,32 @@ def test_slice_views_decode_and_multiply(ctx):
     assert z2.tobytes() == z.tobytes()
 
 
+def test_operand_image_cache_follows_containers(ctx):
+    """A plain operand decoded into a workspace is reused only while the
+    same, unchanged container is asked for again in the same slot: other
+    vectors, a freed and reallocated container, a view through the same slot
+    and a gemm in between all give exact results."""
+    rng = np.random.default_rng(31)
+    a = rng.integers(-8, 9, (300, 260)).astype(np.float64)
+    b = rng.integers(-8, 9, (260, 120)).astype(np.float64)
+    ca = ia.Container.from_numpy(ctx, a, (100, 90), (25, 30))
+    cb = ia.Container.from_numpy(ctx, b, (90, 60), (30, 20))
+    print("case: operand cache", flush=True)
+    xs = [rng.integers(-8, 9, 260).astype(np.float64) for _ in range(3)]
+    cx = [ia.Container.from_numpy(ctx, x, (90,), (30,)) for x in xs]
+    for k in (0, 1, 0, 0):
+        assert np.array_equal(ia.matmul(ca, cx[k], (100,), (50,)).to_numpy(), a @ xs[k])
+    del cx[1]
+    x3 = rng.integers(-8, 9, 260).astype(np.float64)
+    c3 = ia.Container.from_numpy(ctx, x3, (90,), (30,))
+    assert np.array_equal(ia.matmul(ca, c3, (100,), (50,)).to_numpy(), a @ x3)
+    assert np.array_equal(ia.matmul(ca, cb, (100, 60), (50, 30)).to_numpy(), a @ b)
+    v = cx[0].get_slice((0,), (260,))
+    assert np.array_equal(ia.matmul(ca, v, (100,), (50,)).to_numpy(), a @ xs[0])
+    assert np.array_equal(ia.matmul(ca, cx[0], (100,), (50,)).to_numpy(), a @ xs[0])
+    assert np.array_equal(ia.matmul(ca, c3, (100,), (50,)).to_numpy(), a @ x3)
+
+
 def test_matmul_deterministic(ctx):
     rng = np.random.default_rng(2)
     a = rng.standard_normal((400, 300))
