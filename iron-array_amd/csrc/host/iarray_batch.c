@@ -598,7 +598,10 @@ INA_API(ina_rc_t) iarray_linalg_matmul_batch(iarray_context_t *ctx, int n, iarra
     /* views, GEMVs, mixed dtypes or mixed precision: product by product */
     for (int i = 0; i < n; i++) {
       ina_rc_t rc = iarray_linalg_matmul_params(ctx, a[i], b[i], &storage[i], params, &c[i]);
-      if (INA_FAILED(rc)) return rc;
+      if (INA_FAILED(rc)) {
+        for (int j = 0; j < i; j++) iarray_container_free(ctx, &c[j]);
+        return rc;
+      }
     }
     return INA_SUCCESS;
   }
@@ -644,11 +647,11 @@ INA_API(ina_rc_t) iarray_linalg_matmul_batch(iarray_context_t *ctx, int n, iarra
   rc = batch_write(ctx, n, c, eager && atoi(eager) == 1);
   iax_last_encode_s = iax_now() - t0;
   /* the decode status of every phase (a malformed stream leaves its flag) */
-  const int st = iamx_decode_status();
+  const ina_rc_t st = iax_check_decode_status();
   iax_last_device_ms = iamx_prof_span_ms();
-  if (INA_FAILED(rc)) return rc;
-  if (st)
-    return st < 0 ? iax_dev_error(st, "batch status")
-                  : iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "malformed LZ4 stream (0x%x)", st);
-  return INA_SUCCESS;
+  if (INA_SUCCEED(rc)) rc = st;
+  /* a failed batch leaves no partly written outputs behind */
+  if (INA_FAILED(rc))
+    for (int i = 0; i < n; i++) iarray_container_free(ctx, &c[i]);
+  return rc;
 }

@@ -162,16 +162,21 @@ INA_API(ina_rc_t) iarray_empty(iarray_context_t *ctx, iarray_dtshape_t *dtshape,
   return iax_container_new(ctx, dtshape, storage, container);
 }
 
-/* a special-value chunk: 32-byte header, value kind in bits 4..6 of byte 31 */
-static uint8_t *zero_chunk(const iarray_container_t *c, int32_t *csize) {
+/* A special-value chunk: a 32-byte header only, the value kind (1 zeros,
+ * 2 NaNs, 4 uninitialized) in bits 4..6 of byte 31. */
+uint8_t *iax_special_chunk(int kind, int32_t nbytes, int32_t blocksize, int typesize) {
   uint8_t *h = (uint8_t *)calloc(1, 32);
   if (!h) return NULL;
-  const int32_t nb = (int32_t)c->chunk_nbytes, bs = c->blocksize, cb = 32;
-  h[0] = 5; h[1] = 1; h[2] = 0x05 | 0x10 | (1 << 5); h[3] = (uint8_t)c->itemsize;
-  memcpy(h + 4, &nb, 4); memcpy(h + 8, &bs, 4); memcpy(h + 12, &cb, 4);
-  h[31] = 1 << 4;
-  *csize = 32;
+  const int32_t cb = 32;
+  h[0] = 5; h[1] = 1; h[2] = 0x05 | 0x10 | (1 << 5); h[3] = (uint8_t)typesize;
+  memcpy(h + 4, &nbytes, 4); memcpy(h + 8, &blocksize, 4); memcpy(h + 12, &cb, 4);
+  h[31] = (uint8_t)((kind & 7) << 4);
   return h;
+}
+
+static uint8_t *zero_chunk(const iarray_container_t *c, int32_t *csize) {
+  *csize = 32;
+  return iax_special_chunk(1, (int32_t)c->chunk_nbytes, c->blocksize, c->itemsize);
 }
 
 INA_API(ina_rc_t) iarray_zeros(iarray_context_t *ctx, iarray_dtshape_t *dtshape, iarray_storage_t *storage,
@@ -562,6 +567,21 @@ ina_rc_t iax_decode_dense(iarray_container_t *c, void *d_dst, int scratch_slot) 
   return INA_SUCCESS;
 }
 
+iax_operand_key_t iax_operand_key(const iarray_container_t *c) {
+  const iarray_container_t *base = c->viewed ? c->viewed : c;
+  iax_operand_key_t k;
+  memset(&k, 0, sizeof(k));
+  k.gen = base->generation;
+  k.transposed = c->transposed ? 1 : 0;
+  if (c->viewed) {
+    k.voff[0] = c->voff[0];
+    k.voff[1] = c->voff[1];
+  }
+  k.shape[0] = c->dtshape.shape[0];
+  k.shape[1] = c->dtshape.ndim == 2 ? c->dtshape.shape[1] : 1;
+  return k;
+}
+
 /* Which container's dense image each workspace slot holds (its device
  * generation, unique across containers, and the buffer address): a plain
  * container asked for again in the same slot, unchanged, is not decoded again
@@ -644,12 +664,19 @@ ina_rc_t iax_operand(iarray_container_t *c, int slot, int tmp_slot, void **d_out
   return INA_SUCCESS;
 }
 
-static ina_rc_t check_decode_status(void) {
+/* The device status word of the decodes queued since the last check.  A
+ * nonzero word means some image written since then may hold garbage, so every
+ * remembered operand image is forgotten: the next call decodes again and
+ * reports the same error. */
+ina_rc_t iax_check_decode_status(void) {
   int st = iamx_decode_status();
+  if (st) iax_operand_forget(-1);
   if (st < 0) return iax_dev_error(st, "decode status");
   if (st) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "malformed LZ4 stream (device flags 0x%x)", st);
   return INA_SUCCESS;
 }
+
+static ina_rc_t check_decode_status(void) { return iax_check_decode_status(); }
 
 /* ---- write stage ------------------------------------------------------------------------------- */
 typedef struct {
@@ -1193,6 +1220,9 @@ INA_API(ina_rc_t) iarray_linalg_transpose(iarray_context_t *ctx, iarray_containe
   v->transposed = true;
   v->plan_valid = 0;
   memset(&v->plan, 0, sizeof(v->plan));
+  v->frame.valid = 0;
+  v->on_disk = false;
+  v->tuned_b = NULL;
   *b = v;
   return INA_SUCCESS;
 }

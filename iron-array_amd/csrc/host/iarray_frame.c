@@ -258,8 +258,9 @@ static ina_rc_t container_from_frame(iarray_context_t *ctx, const uint8_t *d, in
   if (ext != chunksize || ncells != nchunks) return frame_error("caterva metalayer disagrees with the chunks");
 
   /* chunk offsets: the chunk after the data, stored (memcpyed) */
+  /* bounded before adding: hsize and cbytes come from the file */
+  if (cbytes > size - hsize - 32) return frame_error("offsets chunk position");
   const int64_t opos = hsize + cbytes;
-  if (opos < hsize || opos + 32 > size) return frame_error("offsets chunk position");
   const uint8_t *oc = d + opos;
   int32_t onb, ocb;
   memcpy(&onb, oc + 4, 4);
@@ -269,9 +270,11 @@ static ina_rc_t container_from_frame(iarray_context_t *ctx, const uint8_t *d, in
     return iax_set_error(INA_ERROR(INA_ERR_NOT_SUPPORTED), "frames with a compressed offsets chunk are not read");
   const uint8_t **ptrs = (const uint8_t **)malloc(sizeof(*ptrs) * (size_t)(nchunks > 0 ? nchunks : 1));
   int32_t *csz = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nchunks > 0 ? nchunks : 1));
-  if (!ptrs || !csz) {
+  uint8_t **special = (uint8_t **)calloc((size_t)(nchunks > 0 ? nchunks : 1), sizeof(uint8_t *));
+  if (!ptrs || !csz || !special) {
     free(ptrs);
     free(csz);
+    free(special);
     return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
   }
   rc = INA_SUCCESS;
@@ -279,14 +282,30 @@ static ina_rc_t container_from_frame(iarray_context_t *ctx, const uint8_t *d, in
     int64_t off;
     memcpy(&off, oc + 32 + 8 * i, 8);
     if (off < 0) {
-      rc = iax_set_error(INA_ERROR(INA_ERR_NOT_SUPPORTED), "special-value chunks in frames are not read");
-      break;
+      /* a special chunk has no bytes in the frame: bit 63 of its offset is
+       * set and the top byte's low 7 bits name the value (1 zeros, 2 NaNs,
+       * 4 uninitialized) */
+      const int kind = (int)(((uint64_t)off >> 56) & 0x7f);
+      if (kind != 1 && kind != 2 && kind != 4) {
+        rc = frame_error("special chunk kind");
+        break;
+      }
+      const int64_t cn = (i == nchunks - 1 && nbytes % chunksize) ? nbytes % chunksize : chunksize;
+      int64_t bsz = typesize;
+      for (int64_t k = 0; k < ndim; k++) bsz *= st.blockshape[k];
+      if (!(special[i] = iax_special_chunk(kind, (int32_t)cn, (int32_t)bsz, (int)typesize))) {
+        rc = INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+        break;
+      }
+      ptrs[i] = special[i];
+      csz[i] = 32;
+      continue;
     }
-    const int64_t pos = hsize + off;
-    if (pos + 32 > opos) {
+    if (off > opos - hsize - 32) {
       rc = frame_error("chunk offset");
       break;
     }
+    const int64_t pos = hsize + off;
     int32_t cs;
     memcpy(&cs, d + pos + 12, 4);
     if (cs < 32 || pos + cs > opos) {
@@ -298,6 +317,8 @@ static ina_rc_t container_from_frame(iarray_context_t *ctx, const uint8_t *d, in
   }
   if (INA_SUCCEED(rc)) rc = iarray_from_chunks(ctx, &dts, &st, nchunks, ptrs, csz, container);
   if (INA_SUCCEED(rc)) (*container)->frame = fp;
+  for (int64_t i = 0; i < nchunks; i++) free(special[i]);
+  free(special);
   free(ptrs);
   free(csz);
   return rc;

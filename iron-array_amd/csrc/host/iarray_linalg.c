@@ -25,12 +25,7 @@ static int dtype_code(iarray_data_type_t t) {
   return t == IARRAY_DATA_TYPE_DOUBLE ? 0 : t == IARRAY_DATA_TYPE_FLOAT ? 1 : -1;
 }
 
-static ina_rc_t check_status(void) {
-  int st = iamx_decode_status();
-  if (st < 0) return iax_dev_error(st, "device status");
-  if (st) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "malformed LZ4 stream (device flags 0x%x)", st);
-  return INA_SUCCESS;
-}
+static ina_rc_t check_status(void) { return iax_check_decode_status(); }
 
 /* A and B decoded by one launch pair into one dense buffer (A, then B at
  * element offset b_off), so the LZ4 kernel sees the streams of both operands
@@ -177,7 +172,8 @@ static ina_rc_t run_gemm(iarray_container_t *a, iarray_container_t *b, iarray_co
   if (prec == IARRAY_MATMUL_OZAKI_INT8) {
     iax_last_slice_err = -1.0;
     if (slices == 0) {
-      if (a->tuned_b == b && a->tuned_gen_a == a->generation && a->tuned_gen_b == b->generation &&
+      const iax_operand_key_t ka = iax_operand_key(a), kb = iax_operand_key(b);
+      if (a->tuned_b == b && !memcmp(&a->tuned_key_a, &ka, sizeof(ka)) && !memcmp(&a->tuned_key_b, &kb, sizeof(kb)) &&
           a->tuned_slices > 0) {
         slices = a->tuned_slices;
       } else {
@@ -187,8 +183,8 @@ static ina_rc_t run_gemm(iarray_container_t *a, iarray_container_t *b, iarray_co
         int e = iamx_ozaki_pick_slices(M, N, K, (const double *)dA, K, (const double *)dB, N, target, 4, &slices, &err);
         if (e) return iax_dev_error(e, "ozaki slice tuning");
         a->tuned_b = b;
-        a->tuned_gen_a = a->generation;
-        a->tuned_gen_b = b->generation;
+        a->tuned_key_a = ka;
+        a->tuned_key_b = kb;
         a->tuned_slices = slices;
         iax_last_slice_err = err;
       }
@@ -270,6 +266,8 @@ static ina_rc_t matmul_impl(iarray_context_t *ctx, iarray_container_t *a, iarray
   iamx_prof_reset();
   rc = b->dtshape.ndim == 2 ? run_gemm(a, b, *c, params) : run_gemv(a, b, *c);
   iax_last_device_ms = iamx_prof_span_ms();
+  /* a failed product leaves no partly written output behind */
+  if (INA_FAILED(rc)) iarray_container_free(ctx, c);
   return rc;
 }
 

@@ -48,6 +48,18 @@ typedef struct {
   uint8_t coding[16];
 } iax_frame_params_t;
 
+/* What an operand reads: its base container's device generation (unique
+ * across containers, renewed when the arena is rebuilt) and the view's
+ * rectangle of that base.  Two operands with equal keys hold the same
+ * elements. */
+typedef struct {
+  uint64_t gen;
+  int64_t voff[2], shape[2];
+  int transposed;
+} iax_operand_key_t;
+
+iax_operand_key_t iax_operand_key(const iarray_container_t *c);
+
 struct iarray_container_s {
   iarray_dtshape_t dtshape; /* logical shape (a transposed view swaps it) */
   iarray_storage_t storage;
@@ -80,9 +92,9 @@ struct iarray_container_s {
   iax_frame_params_t frame;
   bool on_disk;
   /* Ozaki slice count picked for this container as A against one B: valid
-   * while both generations are unchanged */
+   * while both operands read the same elements (iax_operand_key) */
   const iarray_container_t *tuned_b;
-  uint64_t tuned_gen_a, tuned_gen_b;
+  iax_operand_key_t tuned_key_a, tuned_key_b;
   int tuned_slices;
 };
 
@@ -143,6 +155,8 @@ ina_rc_t iax_operand(iarray_container_t *c, int slot, int tmp_slot, void **d_out
 /* The dense image in workspace `slot` is about to be overwritten by other
  * code (slot < 0: all slots, e.g. when the device runtime shuts down). */
 void iax_operand_forget(int slot);
+ina_rc_t iax_check_decode_status(void);
+uint8_t *iax_special_chunk(int kind, int32_t nbytes, int32_t blocksize, int typesize);
 /* Write C from a dense device buffer (logical shape, ld = shape1): GPU pack +
  * host LZ4 encode; fills c->chunks. */
 ina_rc_t iax_write_from_dense(iarray_container_t *c, const void *d_dense);

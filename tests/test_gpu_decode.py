@@ -206,6 +206,26 @@ def test_corrupted_stream_reports_error(ctx):
     assert ia.decode_chunks_flat(ctx, c.chunks(), x.nbytes) == x.tobytes()
 
 
+def test_corrupted_container_errors_on_every_call(ctx):
+    """A container holding a malformed LZ4 stream reports the error on every
+    read, not only the first: a failed decode leaves no remembered image."""
+    x = (np.arange(4096) % 5).astype(np.float32)
+    c = ia.Container.from_numpy(ctx, x, (4096,), (1024,))
+    bad = ia.Container.from_chunks(ctx, np.float32, (4096,), (4096,), (1024,), [_corrupt_offset(c.chunks()[0])])
+    for _ in range(2):
+        with pytest.raises(ia.IArrayError) as e:
+            bad.to_numpy()
+        assert e.value.rc == ia.ERR_BLOSC_FAILED
+    # as the x of a GEMV, twice
+    A = ia.Container.from_numpy(ctx, np.ones((64, 4096), np.float32), (64, 4096), (64, 1024))
+    for _ in range(2):
+        with pytest.raises(ia.IArrayError) as e:
+            ia.matmul(A, bad, (64,), (64,))
+        assert e.value.rc == ia.ERR_BLOSC_FAILED
+    assert np.array_equal(c.to_numpy(), x)
+    assert np.array_equal(ia.matmul(A, c, (64,), (64,)).to_numpy(), np.full(64, x.sum(), np.float32))
+
+
 def _lenbytes(v):
     """LZ4 length continuation bytes for a 4-bit field that holds 15."""
     v -= 15

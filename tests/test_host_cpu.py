@@ -306,6 +306,68 @@ def test_frame_load_survives_damage(tmp_path):
     assert "error" in outcomes
 
 
+def _offsets_pos(frame: bytes):
+    """Byte position of the int64 chunk offsets in a contiguous frame: header
+    size (int32 BE at byte 11) + data cbytes (the sixth msgpack field, an
+    int64 BE after its 0xd3 marker), then the offsets chunk's 32-byte header."""
+    hsize = int.from_bytes(frame[11:15], "big")
+    # fields: magic (str8), hsize (int32), fsize (uint64), flags, nbytes, cbytes
+    p = frame.index(b"\xd3", 16 + 8 + 5)      # nbytes marker after flags
+    cbytes = int.from_bytes(frame[p + 10:p + 18], "big", signed=True)
+    assert frame[p + 9] == 0xd3
+    return hsize + cbytes + 32, hsize, cbytes
+
+
+def test_frame_load_rejects_overflowing_offsets(tmp_path):
+    """A frame whose data cbytes or chunk offset is near INT64_MAX is refused
+    with an error: the reader bounds each field before adding it to a
+    position."""
+    ctx = _ctx()
+    data = bytearray(open(FRAMES[0], "rb").read())
+    opos, hsize, cbytes = _offsets_pos(bytes(data))
+    f = tmp_path / "o.iarray"
+    for off in (2 ** 63 - 1 - hsize, 2 ** 63 - 40, 2 ** 62):
+        d = bytearray(data)
+        d[opos:opos + 8] = int(off).to_bytes(8, "little")
+        f.write_bytes(bytes(d))
+        assert _err(lambda: ia.Container.load(ctx, str(f))) == ia.ERR_BLOSC_FAILED
+    p = data.index(b"\xd3", 16 + 8 + 5)
+    for cb in (2 ** 63 - 1, 2 ** 63 - 1 - hsize + 8, 2 ** 63 - 20):
+        d = bytearray(data)
+        d[p + 10:p + 18] = int(cb).to_bytes(8, "big")
+        f.write_bytes(bytes(d))
+        assert _err(lambda: ia.Container.load(ctx, str(f))) == ia.ERR_BLOSC_FAILED
+
+
+@pytest.mark.parametrize("kind,fill", [(1, 0), (2, None), (4, 0)])
+def test_frame_load_special_value_chunks(orc, tmp_path, kind, fill):
+    """A chunk whose frame offset is negative holds no bytes: bit 63 is set
+    and the top byte's low bits name the value (1 zeros, 2 NaNs, 4
+    uninitialized, read as zeros).  The container gets a 32-byte special
+    chunk that the oracle decodes to that value, the other chunks as stored."""
+    ctx = _ctx()
+    x = np.arange(3 * 40 * 50, dtype=np.float64).reshape(120, 50) + 1.0
+    c = ia.Container.from_numpy(ctx, x, (40, 50), (20, 25))
+    f = tmp_path / "s.iarray"
+    c.save(str(f))
+    data = bytearray(f.read_bytes())
+    opos, _, _ = _offsets_pos(bytes(data))
+    data[opos + 8:opos + 16] = ((1 << 63) | (kind << 56)).to_bytes(8, "little")
+    f.write_bytes(bytes(data))
+    d = ia.Container.load(ctx, str(f))
+    ch = d.chunks()
+    assert len(ch[1]) == 32 and (ch[1][31] >> 4) & 7 == kind
+    assert ch[0] == c.chunks()[0] and ch[2] == c.chunks()[2]
+    got = orc.Array(ch, x.shape, (40, 50), (20, 25), 8).to_dense()
+    want = x.copy()
+    want[40:80] = np.nan if fill is None else fill
+    assert np.array_equal(got, want, equal_nan=True)
+    for bad in (3, 5, 0x7f):
+        data[opos + 8:opos + 16] = ((1 << 63) | (bad << 56)).to_bytes(8, "little")
+        f.write_bytes(bytes(data))
+        assert _err(lambda: ia.Container.load(ctx, str(f))) == ia.ERR_BLOSC_FAILED
+
+
 @pytest.mark.parametrize("path", FRAMES, ids=os.path.basename)
 def test_frame_save_rewrites_fixture(path, tmp_path):
     """A frame the reference wrote, loaded and saved again, is the same file
