@@ -28,10 +28,15 @@
  *                              src/iarray_zproxy.c:46-85 and src/linalg/matmul/gemm.c:25-69),
  *                              brute-force checked on tiny shapes in tests.
  *   - orc_gemm_blosc / orc_gemv_blosc / orc_opt_gemm / orc_opt_gemv / orc_opt_gemm_a /
- *     orc_opt_gemm_b ......... integer-valued
- *                              inputs (exact in any summation order) against a dense
- *                              product, the reference tests' arange/linspace fixtures at
- *                              their tolerances (tests/test_linalg_gemm.c:128-147).
+ *     orc_opt_gemm_b ......... integer-valued inputs (exact in any summation order)
+ *                              against numpy's dense product: ragged tilings with
+ *                              all-padding blocks, transposed operands, planted special
+ *                              zero chunks and zero blocks; the opt paths' zero skipping
+ *                              by an Inf that meets only those zeros (tests/
+ *                              test_oracle_linalg.py, tests/test_host_cpu.py).
+ *   - orc_dd_gemm_sample ..... exact rational dot products (fractions.Fraction) on
+ *                              wide-range and cancelling data: within half an ulp of
+ *                              the exact sum (tests/test_oracle_linalg.py).
  */
 #ifndef IAMX_ORACLE_H
 #define IAMX_ORACLE_H
