@@ -123,7 +123,9 @@ IAMX_API int iamx_decode(const iamx_lz4_job_t *d_lz4, int n_lz4, const iamx_bloc
  * (src/linalg/matmul/gemm.c:96-102) and of iarray_opt_gemm
  * (src/linalg/iarray_gemm.c:276-294): one launch computes the whole dense
  * C(MxN) = A(MxK) B(KxN), row-major.  dtype 0 = float64, 1 = float32.
- * precision: 0 plain, 1 split-bf16 (float32), 2 ozaki-int8 (float64, `slices`). */
+ * precision: 0 plain, 1 split-bf16 (float32), 2 ozaki-int8 (float64, `slices`
+ * INT8 slices), 3 ozaki-crt (float64, `slices` = number of INT8 moduli,
+ * 2..15). */
 /* Smallest Ozaki slice count S in [smin, 8] whose product reaches a
  * componentwise error max |C_S - C| / (|A||B|) <= target on a sample of up
  * to 256 evenly spaced rows of A, measured against the plain FP64 MFMA product
@@ -131,6 +133,11 @@ IAMX_API int iamx_decode(const iamx_lz4_job_t *d_lz4, int n_lz4, const iamx_bloc
  * the chosen S.  Replaces nothing in the reference (its FP64 path is MKL). */
 IAMX_API int iamx_ozaki_pick_slices(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B,
                                     int64_t ldb, double target, int smin, int *slices, double *err);
+/* The same pick for the CRT form: the smallest number of moduli in
+ * [nmin, 15] reaching `target` on the sampled rows (15 if none does). */
+IAMX_API int iamx_ozaki_crt_pick_moduli(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda,
+                                        const double *B, int64_t ldb, double target, int nmin, int *moduli,
+                                        double *err);
 IAMX_API int iamx_gemm(int dtype, int precision, int slices, int64_t M, int64_t N, int64_t K, const void *A,
                        int64_t lda, const void *B, int64_t ldb, void *C, int64_t ldc);
 

@@ -22,16 +22,24 @@ extern "C" {
  *               on BF16 MFMA with FP32 accumulation;
  *   OZAKI_INT8  (float64 only): per-row/column power-of-two scaling, operands
  *               cut into signed 7-bit INT8 slices, exact INT32 MFMA slice
- *               products, FP64 recombination and unscaling. */
+ *               products, FP64 recombination and unscaling;
+ *   OZAKI_CRT   (float64 only): per-row/column power-of-two scaling to
+ *               integers, one exact INT8 MFMA product per modulus of their
+ *               residues modulo n pairwise coprime moduli <= 256, exact
+ *               Chinese-remainder recombination and FP64 unscaling (n INT8
+ *               products where OZAKI_INT8 needs S(S+1)/2). */
 typedef enum iarray_matmul_precision_e {
   IARRAY_MATMUL_PLAIN = 0,
   IARRAY_MATMUL_SPLIT_BF16 = 1,
   IARRAY_MATMUL_OZAKI_INT8 = 2,
+  IARRAY_MATMUL_OZAKI_CRT = 3,
 } iarray_matmul_precision_t;
 
 typedef struct iarray_matmul_params_s {
   iarray_matmul_precision_t precision;
-  int ozaki_slices; /* 0 = pick the count that reaches ~1e-13 componentwise error */
+  /* slices (OZAKI_INT8, 2..8) or moduli (OZAKI_CRT, 2..15); 0 = pick the
+   * smallest count that reaches ~1e-13 componentwise error */
+  int ozaki_slices;
 } iarray_matmul_params_t;
 
 /* iarray_linalg_matmul with an explicit arithmetic form. */
@@ -81,10 +89,11 @@ INA_API(void) iarray_last_matmul_times(double *host_encode_s, double *device_ms)
  * encoded on the host). */
 INA_API(void) iarray_last_gpu_encode_bytes(int64_t *stream_bytes, int64_t *chunk_bytes);
 
-/* Slice count used by the last OZAKI_INT8 product (0 if none ran) and, when
+/* Slice (OZAKI_INT8) or modulus (OZAKI_CRT) count used by the last Ozaki
+ * product (0 if none ran) and, when
  * it was picked automatically (ozaki_slices = 0), the componentwise error
  * max |C_S - C| / (|A||B|) measured on the sampled rows (else -1).  The
- * automatic pick is the smallest S >= 4 whose sampled error is at most
+ * automatic pick is the smallest S >= 4 (moduli: >= 8) whose sampled error is at most
  * IARRAY_OZAKI_TARGET (environment, default 1e-13), cached per (A, B) pair
  * while neither container changes. */
 INA_API(int) iarray_last_ozaki_slices(double *sampled_err);

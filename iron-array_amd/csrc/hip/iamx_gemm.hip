@@ -185,6 +185,8 @@ int iamx_gemm_split_bf16(int64_t M, int64_t N, int64_t K, const float *A, int64_
                          float *C, int64_t ldc);
 int iamx_gemm_ozaki_i8(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B, int64_t ldb,
                        double *C, int64_t ldc, int S);
+int iamx_gemm_ozaki_crt(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B, int64_t ldb,
+                        double *C, int64_t ldc, int n);
 int iamx_mixed_prepare_b(int form, int64_t N, int64_t K, const void *B, int64_t ldb, int S);
 int iamx_mixed_rows(int form, int64_t M, const void *A, int64_t lda, void *C, int64_t ldc);
 
@@ -203,6 +205,11 @@ IAMX_API int iamx_gemm(int dtype, int precision, int slices, int64_t M, int64_t 
     if (dtype != 0) return iamx::set_error(IAMX_EINVAL, "iamx_gemm: ozaki-int8 needs float64");
     return iamx_gemm_ozaki_i8(M, N, K, (const double *)A, lda, (const double *)B, ldb, (double *)C, ldc,
                               slices > 0 ? slices : 7);
+  }
+  if (precision == 3) {
+    if (dtype != 0) return iamx::set_error(IAMX_EINVAL, "iamx_gemm: ozaki-crt needs float64");
+    return iamx_gemm_ozaki_crt(M, N, K, (const double *)A, lda, (const double *)B, ldb, (double *)C, ldc,
+                               slices > 0 ? slices : 14);
   }
   if (precision != 0) return iamx::set_error(IAMX_EINVAL, "iamx_gemm: precision %d not available", precision);
   const int64_t tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
@@ -237,10 +244,12 @@ IAMX_API int iamx_gemm_prepare_b(int dtype, int precision, int slices, int64_t N
   if (dtype != 0 && dtype != 1) return iamx::set_error(IAMX_EINVAL, "iamx_gemm_prepare_b: dtype %d", dtype);
   if (precision == 1 && dtype != 1) return iamx::set_error(IAMX_EINVAL, "iamx_gemm: split-bf16 needs float32");
   if (precision == 2 && dtype != 0) return iamx::set_error(IAMX_EINVAL, "iamx_gemm: ozaki-int8 needs float64");
-  if (precision < 0 || precision > 2)
+  if (precision == 3 && dtype != 0) return iamx::set_error(IAMX_EINVAL, "iamx_gemm: ozaki-crt needs float64");
+  if (precision < 0 || precision > 3)
     return iamx::set_error(IAMX_EINVAL, "iamx_gemm: precision %d not available", precision);
   if (precision != 0) {
-    rc = iamx_mixed_prepare_b(precision, N, K, B, ldb, precision == 2 ? (slices > 0 ? slices : 7) : 0);
+    const int count = precision == 2 ? (slices > 0 ? slices : 7) : precision == 3 ? (slices > 0 ? slices : 14) : 0;
+    rc = iamx_mixed_prepare_b(precision, N, K, B, ldb, count);
     if (rc) return rc;
   }
   g_rows.dtype = dtype;

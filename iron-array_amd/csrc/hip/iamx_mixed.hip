@@ -22,6 +22,17 @@
 //   the INT32 tile is added into an FP64 accumulator with weight 2^-7L, and
 //   the epilogue applies 2^(e_i+f_j).  FP64 arithmetic appears only in that
 //   recombination and in the exponent scaling.
+// OZAKI_CRT (float64 operands, the CRT scheme of Ozaki, Uchino and Imamura,
+//   "Ozaki Scheme II", 2024): row i of A is scaled to integers A'_ik =
+//   rint(A_ik 2^(p+1-e_i)) with |A'_ik| <= 2^p, column j of B likewise with q,
+//   p + q chosen so that K 2^(p+q) < M/2 for M = m_1 ... m_n, n pairwise
+//   coprime moduli <= 256.  Each modulus gets one INT8 product of the
+//   symmetric residues (A' mod m_l)(B' mod m_l) on v_mfma_i32_32x32x32_i8 with
+//   exact INT32 sums, reduced mod m_l in the epilogue to one byte.  A combine
+//   kernel rebuilds C' = sum_l r_l W_l mod M exactly (W_l = (M/m_l)
+//   ((M/m_l)^-1 mod m_l); 20-bit limbs), takes the representative in
+//   [-M/2, M/2) and converts it to FP64 with the unscaling 2^(e_i+f_j-p-q-2):
+//   n INT8 products instead of the S(S+1)/2 of the slicing form.
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -57,6 +68,12 @@ struct SegArgs {
   int nseg;
   int tiles_n;
   Seg seg[MAX_SEGS];
+  // OZAKI_CRT: one residue product per modulus, plane l of A at l * a_plane
+  // bytes, of B at l * b_plane, residue bytes of C at l * c_plane (row
+  // stride ldc bytes)
+  int nmod;
+  int64_t a_plane, b_plane, c_plane;
+  int32_t mods[16];
 };
 
 // ---- bf16 split ------------------------------------------------------------------
@@ -185,6 +202,140 @@ __global__ __launch_bounds__(256) void k_slice_i8(const double *__restrict__ X, 
       planes[(int64_t)s * rows_p * cols_p + r * cols_p + c] = (int8_t)d;
     }
   }
+}
+
+// ---- Ozaki CRT prep and combine ------------------------------------------------------------
+constexpr int kMaxMods = 15;   // W_l < M < 2^118 fits 6 limbs of 20 bits; p, q <= 52
+constexpr int kLimbs = 7;      // S = sum r_l W_l < 2^130 needs 7
+constexpr uint32_t kLimbMask = (1u << 20) - 1;
+
+struct CrtMods {
+  int32_t n;
+  int32_t m[kMaxMods];
+  double inv[kMaxMods];  // 1 / m_l, rounded
+};
+
+// n residue planes, each rows_p x cols_p int8: the integer rint(X 2^(scale -
+// e[r])) reduced to the symmetric range of m_l ([-128, 127] for 256,
+// [-(m-1)/2, (m-1)/2] for odd m).  trans as in k_slice_i8 (B's planes are
+// stored transposed); e[r] is the exponent of plane row r (ilogb(max) + 2, or
+// 0 for an all-zero line).  The scaled integer has at most 52 bits, so
+// a - m rint(a/m) is exact (the quotient may be off by one; one correction).
+__global__ __launch_bounds__(256) void k_residues_i8(const double *__restrict__ X, int64_t xr, int64_t xc,
+                                                     int64_t ldx, int trans, const int32_t *__restrict__ e, int scale,
+                                                     const CrtMods mods, int8_t *__restrict__ planes, int64_t rows_p,
+                                                     int64_t cols_p) {
+  __shared__ double t[32][33];
+  const int64_t r0 = (int64_t)blockIdx.y * 32, c0 = (int64_t)blockIdx.x * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  if (trans) {
+    for (int i = ty; i < 32; i += 8) {
+      const int64_t xrow = c0 + i, xcol = r0 + tx;
+      t[i][tx] = (xrow < xr && xcol < xc) ? X[xrow * ldx + xcol] : 0.0;
+    }
+    __syncthreads();
+  }
+  for (int i = ty; i < 32; i += 8) {
+    const int64_t r = r0 + i, c = c0 + tx;
+    if (r >= rows_p || c >= cols_p) continue;
+    double v;
+    if (trans) v = t[tx][i];
+    else v = (r < xr && c < xc) ? X[r * ldx + c] : 0.0;
+    const int er = r < (trans ? xc : xr) ? e[r] : 0;
+    const double a = rint(ldexp(v, scale - er));
+    const int64_t plane = rows_p * cols_p;
+    for (int l = 0; l < mods.n; l++) {
+      const double m = (double)mods.m[l];
+      double res = fma(-m, rint(a * mods.inv[l]), a);
+      const double hi = (double)((mods.m[l] - 1) / 2), lo = -(double)(mods.m[l] / 2);
+      res -= res > hi ? m : 0.0;
+      res += res < lo ? m : 0.0;
+      planes[(int64_t)l * plane + r * cols_p + c] = (int8_t)(int)res;
+    }
+  }
+}
+
+struct CrtCombine {
+  int32_t n;
+  uint32_t w[kMaxMods][kLimbs - 1];  // W_l in 20-bit limbs, least significant first
+  uint32_t mlimb[kLimbs];            // M
+  uint32_t half[kLimbs];             // ceil(M / 2)
+  double inv_m;                      // 1 / M, rounded
+  int32_t p1, q1;                    // p + 1, q + 1
+};
+
+// r >= s over kLimbs limbs (most significant last)
+__device__ __forceinline__ bool limbs_ge(const uint32_t *r, const uint32_t *s) {
+#pragma unroll
+  for (int k = kLimbs - 1; k >= 0; k--)
+    if (r[k] != s[k]) return r[k] > s[k];
+  return true;
+}
+
+// C[i][j] = 2^(e_i + f_j - p - q - 2) * C', C' the representative in
+// [-M/2, M/2) of sum_l r_l W_l (mod M), r_l the residue byte of plane l.
+// Everything up to the conversion to FP64 is exact integer arithmetic.
+__global__ __launch_bounds__(256) void k_crt_combine(const uint8_t *__restrict__ R, int64_t plane, int64_t ldr,
+                                                     int64_t rows, int64_t cols, const int32_t *__restrict__ ea,
+                                                     const int32_t *__restrict__ fb, const CrtCombine P,
+                                                     double *__restrict__ C, int64_t ldc) {
+  const int64_t i = blockIdx.y, j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= rows || j >= cols) return;
+  uint32_t acc[kLimbs] = {0, 0, 0, 0, 0, 0, 0};
+  const uint8_t *rp = R + i * ldr + j;
+  for (int l = 0; l < P.n; l++) {
+    const uint32_t r = rp[(int64_t)l * plane];
+#pragma unroll
+    for (int k = 0; k < kLimbs - 1; k++) acc[k] += __umul24(r, P.w[l][k]);  // < 15 * 255 * 2^20 < 2^32
+  }
+#pragma unroll
+  for (int k = 0; k < kLimbs - 1; k++) {
+    acc[k + 1] += acc[k] >> 20;
+    acc[k] &= kLimbMask;
+  }
+  // q = floor(S / M) from the top limbs (may be one off), then S - q M exactly
+  const double sd = ldexp((double)acc[6], 120) + ldexp((double)acc[5], 100) + ldexp((double)acc[4], 80) +
+                    ldexp((double)acc[3], 60);
+  const int64_t q = (int64_t)floor(sd * P.inv_m);
+  int64_t carry = 0;
+#pragma unroll
+  for (int k = 0; k < kLimbs; k++) {
+    const int64_t t = (int64_t)acc[k] - q * (int64_t)P.mlimb[k] + carry;
+    acc[k] = (uint32_t)(t & kLimbMask);
+    carry = t >> 20;  // arithmetic shift: floor division
+  }
+  if (carry < 0) {  // q one too large: add M back
+    carry = 0;
+#pragma unroll
+    for (int k = 0; k < kLimbs; k++) {
+      const int64_t t = (int64_t)acc[k] + P.mlimb[k] + carry;
+      acc[k] = (uint32_t)(t & kLimbMask);
+      carry = t >> 20;
+    }
+  } else if (limbs_ge(acc, P.mlimb)) {  // q one too small
+    int64_t b = 0;
+#pragma unroll
+    for (int k = 0; k < kLimbs; k++) {
+      const int64_t t = (int64_t)acc[k] - P.mlimb[k] + b;
+      acc[k] = (uint32_t)(t & kLimbMask);
+      b = t >> 20;
+    }
+  }
+  // representative in [-M/2, M/2)
+  const bool neg = limbs_ge(acc, P.half);
+  if (neg) {
+    int64_t b = 0;
+#pragma unroll
+    for (int k = 0; k < kLimbs; k++) {
+      const int64_t t = (int64_t)P.mlimb[k] - acc[k] + b;
+      acc[k] = (uint32_t)(t & kLimbMask);
+      b = t >> 20;
+    }
+  }
+  double v = (double)acc[kLimbs - 1];
+#pragma unroll
+  for (int k = kLimbs - 2; k >= 0; k--) v = fma(v, 1048576.0, (double)acc[k]);
+  C[i * ldc + j] = ldexp(neg ? -v : v, ea[i] + fb[j] - P.p1 - P.q1);
 }
 
 // ---- the segmented GEMM -----------------------------------------------------------------
@@ -365,6 +516,14 @@ template <>
 __device__ __forceinline__ i32x16 mma32<1>(const i32x4 &a, const i32x4 &b, i32x16 c) {
   return __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c, 0, 0, 0);
 }
+template <>
+struct Acc32<2> {
+  typedef i32x16 t;
+};
+template <>
+__device__ __forceinline__ i32x16 mma32<2>(const i32x4 &a, const i32x4 &b, i32x16 c) {
+  return __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c, 0, 0, 0);
+}
 
 template <int MODE>
 __global__ __launch_bounds__(256) void k_seg_gemm_big(const SegArgs P, int level, int accumulate, int tiles_m) {
@@ -372,9 +531,17 @@ __global__ __launch_bounds__(256) void k_seg_gemm_big(const SegArgs P, int level
   __shared__ __attribute__((aligned(16))) uint8_t lds[2 * BUF];  // [2][A 256 rows | B 256 rows] x ROW2
   typedef typename Acc32<MODE>::t acc_t;
   // XCD x gets nt / 8 tiles (one more when x < nt % 8) in a contiguous band
+  // (OZAKI_CRT: nt tiles per modulus, the modulus-major list banded the same
+  // way, so an XCD works on one or two moduli and shares their planes in L2)
   const int nt = tiles_m * P.tiles_n;
-  const int xq = nt / 8, xr = nt % 8, xcd = (int)blockIdx.x % 8;
-  const int id = xcd * xq + min(xcd, xr) + (int)blockIdx.x / 8;
+  const int ntot = MODE == 2 ? nt * P.nmod : nt;
+  const int xq = ntot / 8, xr = ntot % 8, xcd = (int)blockIdx.x % 8;
+  int id = xcd * xq + min(xcd, xr) + (int)blockIdx.x / 8;
+  int lmod = 0;
+  if (MODE == 2) {
+    lmod = id / nt;
+    id -= lmod * nt;
+  }
   // grouped order: consecutive ids walk 4 tile rows, then the next column, so
   // the tiles running together share few A and B panels
   constexpr int GM = 4;
@@ -398,9 +565,10 @@ __global__ __launch_bounds__(256) void k_seg_gemm_big(const SegArgs P, int level
   // stored to LDS, so global latency has two stages of MFMA work to hide behind
   i32x4 r0[16], r1[16];
   auto gload = [&](i32x4 *r) {
-    const Seg &sg = P.seg[lseg];
-    const uint8_t *pa = P.A + sg.a_off + (m0 + lrow) * P.lda + lk + lcol;
-    const uint8_t *pb = P.B + sg.b_off + (n0 + lrow) * P.ldb + lk + lcol;
+    const int64_t a_off = MODE == 2 ? lmod * P.a_plane : P.seg[lseg].a_off;
+    const int64_t b_off = MODE == 2 ? lmod * P.b_plane : P.seg[lseg].b_off;
+    const uint8_t *pa = P.A + a_off + (m0 + lrow) * P.lda + lk + lcol;
+    const uint8_t *pb = P.B + b_off + (n0 + lrow) * P.ldb + lk + lcol;
 #pragma unroll
     for (int i = 0; i < 8; i++) {
       r[i] = *reinterpret_cast<const i32x4 *>(pa + (int64_t)(32 * i) * P.lda);
@@ -487,6 +655,14 @@ __global__ __launch_bounds__(256) void k_seg_gemm_big(const SegArgs P, int level
         if (row < P.M && col < P.N) {
           if (MODE == 0) {
             reinterpret_cast<float *>(P.C)[row * P.ldc + col] = (float)acc[i][j][v];
+          } else if (MODE == 2) {
+            // exact residue of the INT32 sum in [0, m)
+            const int m = P.mods[lmod];
+            const int a = (int)acc[i][j][v];
+            int r = a - m * (int)floor((double)a / (double)m);
+            r += r < 0 ? m : 0;
+            r -= r >= m ? m : 0;
+            reinterpret_cast<uint8_t *>(P.C)[lmod * P.c_plane + row * P.ldc + col] = (uint8_t)r;
           } else {
             double *c = reinterpret_cast<double *>(P.C) + row * P.ldc + col;
             const double t = ldexp((double)acc[i][j][v], P.ea[row] + P.fb[col] - 7 * level);
@@ -624,13 +800,74 @@ static int tuned(int form, int64_t M, int64_t N, int64_t K, F run, int *rc) {
 // One-shot products keep their B in slot 1 and the row-panel interface in
 // slot 10, so a one-shot product between two panels leaves the panels' B intact.
 struct PrepB {
-  int form;  // 1 split-bf16, 2 ozaki
+  int form;  // 1 split-bf16, 2 ozaki slices, 3 ozaki crt
   int64_t N, K, Np, Kp;
-  int S;
+  int S;                  // slices (form 2) or moduli (form 3)
   const uint8_t *planes;  // B planes (transposed), then the column exponents (ozaki)
   const int32_t *fb;
   bool valid;
+  int p, q;               // form 3: integer bits of the scaled A rows and B columns
 };
+
+// ---- CRT constants (host) ------------------------------------------------------------------
+// The moduli, largest first: pairwise coprime (256 = 2^8, 255 = 3 5 17,
+// 253 = 11 23, 247 = 13 19, 217 = 7 31, the rest prime).
+static const int kMods[kMaxMods] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197};
+
+// p and q for n moduli and depth K: the largest p + q with K 2^(p+q) < M/2
+// (so sum_k |A'_ik B'_kj| < M/2 and the representative in [-M/2, M/2) is
+// the product), split evenly, each at most 52 so the scaled integers and their
+// residue arithmetic stay exact in FP64.
+static int crt_bits(int n, int64_t K, int *p, int *q) {
+  if (n < 2 || n > kMaxMods) return iamx::set_error(IAMX_EINVAL, "ozaki-crt: moduli %d outside 2..%d", n, kMaxMods);
+  double log2m = 0.0;
+  for (int l = 0; l < n; l++) log2m += log2((double)kMods[l]);
+  const int budget = (int)floor(log2m - 1.0 - log2((double)K) - 1e-9);
+  *p = budget / 2 < 52 ? budget / 2 : 52;
+  *q = budget - *p < 52 ? budget - *p : 52;
+  if (*p < 1 || *q < 1) return iamx::set_error(IAMX_EINVAL, "ozaki-crt: %d moduli too few for K=%lld", n, (long long)K);
+  return IAMX_OK;
+}
+
+static CrtMods crt_mods(int n) {
+  CrtMods m;
+  memset(&m, 0, sizeof(m));
+  m.n = n;
+  for (int l = 0; l < n; l++) {
+    m.m[l] = kMods[l];
+    m.inv[l] = 1.0 / (double)kMods[l];
+  }
+  return m;
+}
+
+static void to_limbs(unsigned __int128 v, uint32_t *limb, int nl) {
+  for (int k = 0; k < nl; k++) {
+    limb[k] = (uint32_t)(v & kLimbMask);
+    v >>= 20;
+  }
+}
+
+static CrtCombine crt_combine_consts(int n, int p, int q) {
+  CrtCombine c;
+  memset(&c, 0, sizeof(c));
+  c.n = n;
+  unsigned __int128 M = 1;
+  for (int l = 0; l < n; l++) M *= (unsigned)kMods[l];
+  for (int l = 0; l < n; l++) {
+    const unsigned m = (unsigned)kMods[l];
+    const unsigned __int128 Ml = M / m;
+    const unsigned r = (unsigned)(Ml % m);
+    unsigned inv = 1;
+    while ((r * inv) % m != 1) inv++;  // m <= 256: a direct search
+    to_limbs((Ml * inv) % M, c.w[l], kLimbs - 1);
+  }
+  to_limbs(M, c.mlimb, kLimbs);
+  to_limbs(M / 2, c.half, kLimbs);
+  c.inv_m = 1.0 / (double)M;
+  c.p1 = p + 1;
+  c.q1 = q + 1;
+  return c;
+}
 static PrepB g_pb;  // the row-panel interface's B
 
 static int mixed_prepare_b(PrepB &pb, int slot, int form, int64_t N, int64_t K, const void *B, int64_t ldb, int S) {
@@ -643,7 +880,28 @@ static int mixed_prepare_b(PrepB &pb, int slot, int form, int64_t N, int64_t K, 
     if (!ws) return iamx::set_error(IAMX_ENOMEM, "split-bf16 workspace");
     IAMX_LAUNCH("split_bf16_B", k_split_bf16, dim3((unsigned)(Kp / 32), (unsigned)(Np / 32)), dim3(256), 0,
                 (const float *)B, K, N, ldb, 1, (uint16_t *)ws, (uint16_t *)(ws + b_bytes), Np, Kp);
-    pb = PrepB{1, N, K, Np, Kp, 0, ws, nullptr, true};
+    pb = PrepB{1, N, K, Np, Kp, 0, ws, nullptr, true, 0, 0};
+    return IAMX_OK;
+  }
+  if (form == 3) {
+    int p, q;
+    const int rc = crt_bits(S, K, &p, &q);
+    if (rc) return rc;
+    if (K > 131071) return iamx::set_error(IAMX_EINVAL, "ozaki-crt: K=%lld too large for exact INT32 sums", (long long)K);
+    const int64_t Kp = rup(K, KB);
+    const size_t b_bytes = (size_t)(S * Np * Kp);
+    uint8_t *ws = (uint8_t *)iamx_workspace(slot, b_bytes + (size_t)Np * 4 + 256);
+    if (!ws) return iamx::set_error(IAMX_ENOMEM, "ozaki-crt workspace");
+    int32_t *fb = (int32_t *)(ws + b_bytes);
+    const double *Bd = (const double *)B;
+    IAMX_LAUNCH("ozaki_col_exp", k_fill_i32, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, fb, N, INT32_MIN);
+    IAMX_LAUNCH("ozaki_col_exp", k_col_exp,
+                dim3((unsigned)((N + 63) / 64), (unsigned)((K + kColSlab - 1) / kColSlab)), dim3(256), 0, Bd, K, N,
+                ldb, fb);
+    IAMX_LAUNCH("ozaki_col_exp", k_col_exp_zero, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, fb, N);
+    IAMX_LAUNCH("ozaki_crt_residues_B", k_residues_i8, dim3((unsigned)(Kp / 32), (unsigned)(Np / 32)), dim3(256), 0,
+                Bd, K, N, ldb, 1, fb, q + 1, crt_mods(S), (int8_t *)ws, Np, Kp);
+    pb = PrepB{3, N, K, Np, Kp, S, ws, fb, true, p, q};
     return IAMX_OK;
   }
   if (S < 2 || S > 8) return iamx::set_error(IAMX_EINVAL, "ozaki: slices %d outside 2..8", S);
@@ -661,7 +919,7 @@ static int mixed_prepare_b(PrepB &pb, int slot, int form, int64_t N, int64_t K, 
   IAMX_LAUNCH("ozaki_col_exp", k_col_exp_zero, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, fb, N);
   IAMX_LAUNCH("ozaki_slice_B", k_slice_i8, dim3((unsigned)(Kp / 32), (unsigned)(Np / 32)), dim3(256), 0, Bd, K, N,
               ldb, 1, fb, S, (int8_t *)ws, Np, Kp);
-  pb = PrepB{2, N, K, Np, Kp, S, ws, fb, true};
+  pb = PrepB{2, N, K, Np, Kp, S, ws, fb, true, 0, 0};
   return IAMX_OK;
 }
 
@@ -707,6 +965,37 @@ static int mixed_rows(const PrepB &pb, int form, int64_t M, const void *A, int64
           },
           &rc);
     return rc;
+  }
+  if (form == 3) {
+    const int n = pb.S;
+    const size_t a_bytes = (size_t)(n * Mp * Kp), r_bytes = (size_t)(n * Mp * Np);
+    uint8_t *ws = (uint8_t *)iamx_workspace(11, a_bytes + (size_t)Mp * 4 + r_bytes + 512);
+    if (!ws) return iamx::set_error(IAMX_ENOMEM, "ozaki-crt workspace");
+    int32_t *ea = (int32_t *)(ws + a_bytes);
+    uint8_t *R = ws + rup((int64_t)(a_bytes + (size_t)Mp * 4), 256);
+    const double *Ad = (const double *)A;
+    IAMX_LAUNCH("ozaki_row_exp", k_row_exp, dim3((unsigned)((M + 3) / 4)), dim3(256), 0, Ad, M, K, lda, ea);
+    IAMX_LAUNCH("ozaki_crt_residues_A", k_residues_i8, dim3((unsigned)(Kp / 32), (unsigned)(Mp / 32)), dim3(256), 0,
+                Ad, M, K, lda, 0, ea, pb.p + 1, crt_mods(n), (int8_t *)ws, Mp, Kp);
+    P.A = ws;
+    P.lda = Kp;
+    P.ldb = Kp;
+    P.Kb = Kp;
+    P.nseg = 1;
+    P.nmod = n;
+    P.a_plane = Mp * Kp;
+    P.b_plane = Np * Kp;
+    P.c_plane = Mp * Np;
+    for (int l = 0; l < n; l++) P.mods[l] = kMods[l];
+    P.C = R;
+    P.ldc = Np;
+    P.tiles_n = (int)(Np / BN);
+    IAMX_LAUNCH("gemm_ozaki_crt_i8_mfma", k_seg_gemm_big<2>, dim3((unsigned)((Mp / BM) * (Np / BN) * n)), dim3(256),
+                0, P, 0, 0, (int)(Mp / BM));
+    const CrtCombine comb = crt_combine_consts(n, pb.p, pb.q);
+    IAMX_LAUNCH("ozaki_crt_combine", k_crt_combine, dim3((unsigned)((N + 255) / 256), (unsigned)M), dim3(256), 0,
+                (const uint8_t *)R, Mp * Np, Np, M, N, (const int32_t *)ea, pb.fb, comb, (double *)C, ldc);
+    return IAMX_OK;
   }
   const int S = pb.S;
   const size_t a_bytes = (size_t)(S * Mp * Kp);
@@ -781,6 +1070,14 @@ int iamx_gemm_ozaki_i8(int64_t M, int64_t N, int64_t K, const double *A, int64_t
   return rc ? rc : mixed_rows(pb, 2, M, A, lda, C, ldc);
 }
 
+// float64: C = A B through the CRT form with n moduli.
+int iamx_gemm_ozaki_crt(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B, int64_t ldb,
+                        double *C, int64_t ldc, int n) {
+  PrepB pb;
+  int rc = mixed_prepare_b(pb, 1, 3, N, K, B, ldb, n);
+  return rc ? rc : mixed_rows(pb, 3, M, A, lda, C, ldc);
+}
+
 int iamx_mixed_prepare_b(int form, int64_t N, int64_t K, const void *B, int64_t ldb, int S) {
   return mixed_prepare_b(g_pb, 10, form, N, K, B, ldb, S);
 }
@@ -789,13 +1086,18 @@ int iamx_mixed_rows(int form, int64_t M, const void *A, int64_t lda, void *C, in
   return mixed_rows(g_pb, form, M, A, lda, C, ldc);
 }
 
-IAMX_API int iamx_ozaki_pick_slices(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B,
-                                    int64_t ldb, double target, int smin, int *slices, double *err) {
+}  // extern "C"
+
+// The smallest count in [smin, smax] (Ozaki slices, form 2, or CRT moduli,
+// form 3) whose product of up to 256 evenly spaced rows of A by B has a
+// componentwise error max |C_S - C| / (|A||B|) <= target against the plain
+// FP64 MFMA product of the same rows; smax if none does.
+static int pick_count(int form, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B,
+                      int64_t ldb, double target, int smin, int smax, int *slices, double *err) {
   int rc = iamx::ensure_init();
   if (rc) return rc;
   if (!slices || M <= 0 || N <= 0 || K <= 0) return iamx::set_error(IAMX_EINVAL, "ozaki tuner: bad arguments");
-  if (smin < 2) smin = 2;
-  if (smin > 8) smin = 8;
+  if (smin > smax) smin = smax;
   const int R = (int)(M < 256 ? M : 256);
   const size_t rk = (size_t)R * (size_t)K, rn = (size_t)R * (size_t)N;
   uint8_t *w = (uint8_t *)iamx_workspace(8, (2 * rk + 3 * rn) * sizeof(double) + 256);
@@ -809,10 +1111,11 @@ IAMX_API int iamx_ozaki_pick_slices(int64_t M, int64_t N, int64_t K, const doubl
   rc = iamx_gemm(0, 0, 0, R, N, K, As, K, B, ldb, Cref, N);
   if (!rc) rc = iamx_gemm(0, 0, 0, R, N, K, Aa, K, Ba, N, den, N);
   if (rc) return rc;
-  int chosen = 8;
+  int chosen = smax;
   double chosen_err = -1.0;
-  for (int S = smin; S <= 8; S++) {
-    rc = iamx_gemm_ozaki_i8(R, N, K, As, K, B, ldb, Cs, N, S);
+  for (int S = smin; S <= smax; S++) {
+    rc = form == 3 ? iamx_gemm_ozaki_crt(R, N, K, As, K, B, ldb, Cs, N, S)
+                   : iamx_gemm_ozaki_i8(R, N, K, As, K, B, ldb, Cs, N, S);
     if (rc) return rc;
     IAMX_LAUNCH("ozaki_tune_err", k_zero_u64, dim3(1), dim3(1), 0, d_err);
     IAMX_LAUNCH("ozaki_tune_err", k_err_max, dim3(256), dim3(256), 0, Cs, Cref, den, (int64_t)rn, d_err);
@@ -828,6 +1131,21 @@ IAMX_API int iamx_ozaki_pick_slices(int64_t M, int64_t N, int64_t K, const doubl
   *slices = chosen;
   if (err) *err = chosen_err;
   return IAMX_OK;
+}
+
+extern "C" {
+
+IAMX_API int iamx_ozaki_pick_slices(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B,
+                                    int64_t ldb, double target, int smin, int *slices, double *err) {
+  if (smin < 2) smin = 2;
+  return pick_count(2, M, N, K, A, lda, B, ldb, target, smin, 8, slices, err);
+}
+
+IAMX_API int iamx_ozaki_crt_pick_moduli(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda,
+                                        const double *B, int64_t ldb, double target, int nmin, int *moduli,
+                                        double *err) {
+  if (nmin < 8) nmin = 8;
+  return pick_count(3, M, N, K, A, lda, B, ldb, target, nmin, kMaxMods, moduli, err);
 }
 
 }  // extern "C"

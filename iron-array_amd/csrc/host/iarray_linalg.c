@@ -162,27 +162,34 @@ static ina_rc_t run_gemm(iarray_container_t *a, iarray_container_t *b, iarray_co
   int slices = params ? params->ozaki_slices : 0;
   if (prec == IARRAY_MATMUL_SPLIT_BF16 && dt != 1)
     return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_DTYPE), "split-bf16 needs float32 operands");
-  if (prec == IARRAY_MATMUL_OZAKI_INT8 && dt != 0)
-    return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_DTYPE), "ozaki-int8 needs float64 operands");
+  if ((prec == IARRAY_MATMUL_OZAKI_INT8 || prec == IARRAY_MATMUL_OZAKI_CRT) && dt != 0)
+    return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_DTYPE), "ozaki forms need float64 operands");
+  if (prec < IARRAY_MATMUL_PLAIN || prec > IARRAY_MATMUL_OZAKI_CRT)
+    return iax_set_error(INA_ERROR(INA_ERR_INVALID_ARGUMENT), "unknown matmul precision %d", prec);
   void *dA = NULL, *dB = NULL;
   ina_rc_t rc = decode_operands(a, b, &dA, &dB);
   if (INA_FAILED(rc)) return rc;
   void *dC = iamx_workspace(4, (size_t)(M * N * a->itemsize) + 16);
   if (!dC) return iax_dev_error(IAMX_ENOMEM, "C buffer");
-  if (prec == IARRAY_MATMUL_OZAKI_INT8) {
+  if (prec == IARRAY_MATMUL_OZAKI_INT8 || prec == IARRAY_MATMUL_OZAKI_CRT) {
     iax_last_slice_err = -1.0;
     if (slices == 0) {
       const iax_operand_key_t ka = iax_operand_key(a), kb = iax_operand_key(b);
-      if (a->tuned_b == b && !memcmp(&a->tuned_key_a, &ka, sizeof(ka)) && !memcmp(&a->tuned_key_b, &kb, sizeof(kb)) &&
-          a->tuned_slices > 0) {
+      if (a->tuned_b == b && a->tuned_form == prec && !memcmp(&a->tuned_key_a, &ka, sizeof(ka)) &&
+          !memcmp(&a->tuned_key_b, &kb, sizeof(kb)) && a->tuned_slices > 0) {
         slices = a->tuned_slices;
       } else {
         const char *env = getenv("IARRAY_OZAKI_TARGET");
         const double target = env && atof(env) > 0 ? atof(env) : 1e-13;
         double err = -1.0;
-        int e = iamx_ozaki_pick_slices(M, N, K, (const double *)dA, K, (const double *)dB, N, target, 4, &slices, &err);
-        if (e) return iax_dev_error(e, "ozaki slice tuning");
+        int e = prec == IARRAY_MATMUL_OZAKI_CRT
+                    ? iamx_ozaki_crt_pick_moduli(M, N, K, (const double *)dA, K, (const double *)dB, N, target, 8,
+                                                 &slices, &err)
+                    : iamx_ozaki_pick_slices(M, N, K, (const double *)dA, K, (const double *)dB, N, target, 4,
+                                             &slices, &err);
+        if (e) return iax_dev_error(e, "ozaki count tuning");
         a->tuned_b = b;
+        a->tuned_form = prec;
         a->tuned_key_a = ka;
         a->tuned_key_b = kb;
         a->tuned_slices = slices;

@@ -95,6 +95,7 @@ struct iarray_container_s {
    * while both operands read the same elements (iax_operand_key) */
   const iarray_container_t *tuned_b;
   iax_operand_key_t tuned_key_a, tuned_key_b;
+  int tuned_form; /* the precision the count was picked for */
   int tuned_slices;
 };
 

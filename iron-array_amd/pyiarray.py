@@ -18,7 +18,7 @@ DOUBLE, FLOAT = 0, 1
 COMP_SHUFFLE, COMP_BITSHUFFLE = 0x1, 0x2
 BLOSCLZ, LZ4, LZ4HC = 0, 1, 2
 ALWAYS_SPLIT, NEVER_SPLIT, AUTO_SPLIT = 1, 2, 3
-PLAIN, SPLIT_BF16, OZAKI_INT8 = 0, 1, 2
+PLAIN, SPLIT_BF16, OZAKI_INT8, OZAKI_CRT = 0, 1, 2, 3
 
 _DT = {np.dtype(np.float64): 0, np.dtype(np.float32): 1, np.dtype(np.int64): 10, np.dtype(np.int32): 11,
        np.dtype(np.int16): 12, np.dtype(np.int8): 13, np.dtype(np.uint64): 16, np.dtype(np.uint32): 17,

@@ -116,6 +116,96 @@ def test_ozaki_auto_slices_reach_target(orc):
     assert c2.tobytes() == c.tobytes()
 
 
+MODS = [256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197]
+
+
+def _crt_bits(n, K):
+    """p, q of the CRT form: the largest p + q with K 2^(p+q) < M/2, split
+    evenly, each at most 52."""
+    import math
+    log2m = sum(math.log2(m) for m in MODS[:n])
+    budget = math.floor(log2m - 1 - math.log2(K) - 1e-9)
+    p = min(budget // 2, 52)
+    return p, min(budget - p, 52)
+
+
+def _crt_bound(a, b, n):
+    """A-priori bound of the CRT form: A'_ik = rint(A_ik 2^(p+1-e_i)) is off
+    by at most 2^(e_i-p-2) in A's units (B alike with q), the modular
+    products and the recombination are exact, and the final FP64 conversion
+    and unscaling round at most 8 times."""
+    K = a.shape[1]
+    p, q = _crt_bits(n, K)
+    ea = np.frexp(np.abs(a).max(axis=1))[1] + 1
+    fb = np.frexp(np.abs(b).max(axis=0))[1] + 1
+    ra = np.ldexp(1.0, ea - p - 2)[:, None]
+    rb = np.ldexp(1.0, fb - q - 2)[None, :]
+    trunc = ra * np.abs(b).sum(axis=0)[None, :] + (np.abs(a).sum(axis=1)[:, None] + K * ra) * rb
+    return trunc + 8 * 2.0 ** -53 * (np.abs(a) @ np.abs(b))
+
+
+@pytest.mark.parametrize("moduli,bar", [(12, 1e-12), (14, 1e-14)])
+@pytest.mark.parametrize("shape", [(256, 500, 192), (129, 260, 300), (300, 2100, 260)])
+def test_ozaki_crt_accuracy(orc, shape, moduli, bar):
+    M, K, N = shape
+    print(f"case: ozaki-crt {shape} n={moduli}", flush=True)
+    ctx = ia.Context(clevel=5, filters=ia.COMP_SHUFFLE)
+    rng = np.random.default_rng(M * 5 + K)
+    a = _wide(rng, (M, K))
+    b = _wide(rng, (K, N))
+    ca = ia.Container.from_numpy(ctx, a, (min(M, 128), min(K, 250)), (64, 50))
+    cb = ia.Container.from_numpy(ctx, b, (min(K, 250), min(N, 128)), (50, 64))
+    c = ia.matmul(ca, cb, (min(M, 128), min(N, 128)), (64, 64), precision=ia.OZAKI_CRT, slices=moduli).to_numpy()
+    cm, nm = _metrics(orc, a, b, c)
+    print(f"ozaki-crt n={moduli} {shape}: componentwise {cm:.3e} normwise {nm:.3e}")
+    rows, cols = np.meshgrid(np.arange(M), np.arange(N), indexing="ij")
+    ref = orc.dd_gemm_sample(a, b, rows.ravel(), cols.ravel()).reshape(M, N)
+    assert np.all(np.abs(c - ref) <= _crt_bound(a, b, moduli))
+    assert cm <= bar
+    assert ia.last_ozaki_slices()[0] == moduli
+
+
+def test_ozaki_crt_auto_moduli_reach_target(orc):
+    M, K, N = 300, 700, 260
+    ctx = ia.Context(clevel=5, filters=ia.COMP_SHUFFLE)
+    rng = np.random.default_rng(78)
+    a = _wide(rng, (M, K))
+    b = _wide(rng, (K, N))
+    ca = ia.Container.from_numpy(ctx, a, (150, 350), (50, 70))
+    cb = ia.Container.from_numpy(ctx, b, (350, 130), (70, 65))
+    c = ia.matmul(ca, cb, (150, 130), (50, 65), precision=ia.OZAKI_CRT, slices=0).to_numpy()
+    n, err = ia.last_ozaki_slices()
+    cm, nm = _metrics(orc, a, b, c)
+    print(f"auto moduli: n={n} sampled {err:.3e} full componentwise {cm:.3e}")
+    assert 8 <= n <= 15
+    assert 0 <= err <= 1e-13 or n == 15
+    assert cm <= 1e-13
+    # the Ozaki-slice pick for the same pair is kept apart from this one
+    ia.matmul(ca, cb, (150, 130), (50, 65), precision=ia.OZAKI_INT8, slices=0)
+    S, serr = ia.last_ozaki_slices()
+    assert 4 <= S <= 8 and serr >= 0
+    c2 = ia.matmul(ca, cb, (150, 130), (50, 65), precision=ia.OZAKI_CRT, slices=0).to_numpy()
+    assert ia.last_ozaki_slices() == (n, -1.0)
+    assert c2.tobytes() == c.tobytes()
+
+
+def test_ozaki_crt_exact_on_integers_and_signs(orc):
+    """Integer operands are scaled by powers of two only, so every modular
+    product and the recombination are exact: the result equals a @ b, negative
+    entries (representatives above M/2) included."""
+    ctx = ia.Context(clevel=5, filters=ia.COMP_SHUFFLE)
+    rng = np.random.default_rng(10)
+    a = rng.integers(-1000, 1001, (200, 300)).astype(np.float64)
+    b = rng.integers(-1000, 1001, (300, 150)).astype(np.float64)
+    a[5] = 0.0
+    ca = ia.Container.from_numpy(ctx, a, (100, 100), (50, 50))
+    cb = ia.Container.from_numpy(ctx, b, (100, 75), (50, 25))
+    c = ia.matmul(ca, cb, (100, 75), (50, 25), precision=ia.OZAKI_CRT, slices=10).to_numpy()
+    want = a @ b
+    assert (want < 0).any() and (want > 0).any()
+    assert np.array_equal(c, want)
+
+
 def test_ozaki_exact_on_small_integers(orc):
     """Integers below 2^7 are one slice: every form must be exact."""
     ctx = ia.Context(clevel=5, filters=ia.COMP_SHUFFLE)
