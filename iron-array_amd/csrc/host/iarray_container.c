@@ -1222,7 +1222,7 @@ INA_API(ina_rc_t) iarray_linalg_transpose(iarray_context_t *ctx, iarray_containe
   memset(&v->plan, 0, sizeof(v->plan));
   v->frame.valid = 0;
   v->on_disk = false;
-  v->tuned_b = NULL;
+  memset(v->tuned, 0, sizeof(v->tuned));
   *b = v;
   return INA_SUCCESS;
 }
@@ -1280,7 +1280,7 @@ INA_API(ina_rc_t) iarray_get_slice(iarray_context_t *ctx, iarray_container_t *sr
   memset(&v->plan, 0, sizeof(v->plan));
   v->frame.valid = 0;
   v->on_disk = false;
-  v->tuned_b = NULL;
+  memset(v->tuned, 0, sizeof(v->tuned));
   *container = v;
   return INA_SUCCESS;
 }

@@ -175,9 +175,10 @@ static ina_rc_t run_gemm(iarray_container_t *a, iarray_container_t *b, iarray_co
     iax_last_slice_err = -1.0;
     if (slices == 0) {
       const iax_operand_key_t ka = iax_operand_key(a), kb = iax_operand_key(b);
-      if (a->tuned_b == b && a->tuned_form == prec && !memcmp(&a->tuned_key_a, &ka, sizeof(ka)) &&
-          !memcmp(&a->tuned_key_b, &kb, sizeof(kb)) && a->tuned_slices > 0) {
-        slices = a->tuned_slices;
+      const int t = prec == IARRAY_MATMUL_OZAKI_CRT;
+      if (a->tuned[t].b == b && !memcmp(&a->tuned[t].key_a, &ka, sizeof(ka)) &&
+          !memcmp(&a->tuned[t].key_b, &kb, sizeof(kb)) && a->tuned[t].count > 0) {
+        slices = a->tuned[t].count;
       } else {
         const char *env = getenv("IARRAY_OZAKI_TARGET");
         const double target = env && atof(env) > 0 ? atof(env) : 1e-13;
@@ -188,11 +189,10 @@ static ina_rc_t run_gemm(iarray_container_t *a, iarray_container_t *b, iarray_co
                     : iamx_ozaki_pick_slices(M, N, K, (const double *)dA, K, (const double *)dB, N, target, 4,
                                              &slices, &err);
         if (e) return iax_dev_error(e, "ozaki count tuning");
-        a->tuned_b = b;
-        a->tuned_form = prec;
-        a->tuned_key_a = ka;
-        a->tuned_key_b = kb;
-        a->tuned_slices = slices;
+        a->tuned[t].b = b;
+        a->tuned[t].key_a = ka;
+        a->tuned[t].key_b = kb;
+        a->tuned[t].count = slices;
         iax_last_slice_err = err;
       }
     }

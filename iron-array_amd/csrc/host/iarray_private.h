@@ -91,12 +91,14 @@ struct iarray_container_s {
    * loaded) containers count as stored on disk */
   iax_frame_params_t frame;
   bool on_disk;
-  /* Ozaki slice count picked for this container as A against one B: valid
-   * while both operands read the same elements (iax_operand_key) */
-  const iarray_container_t *tuned_b;
-  iax_operand_key_t tuned_key_a, tuned_key_b;
-  int tuned_form; /* the precision the count was picked for */
-  int tuned_slices;
+  /* Ozaki slice count [0] and CRT modulus count [1] picked for this
+   * container as A against one B: valid while both operands read the same
+   * elements (iax_operand_key) */
+  struct {
+    const iarray_container_t *b;
+    iax_operand_key_t key_a, key_b;
+    int count;
+  } tuned[2];
 };
 
 /* ---- codec (blosc_codec.c) ---------------------------------------------------- */
