@@ -170,6 +170,25 @@ IAMX_API int iamx_gemv_fused(int dtype, const iamx_lz4_job_t *d_lz4, int n_lz4, 
                              const int32_t *d_blk_rc, int n_blocks, int nbc, int bs0, int bs1, const void *x,
                              void *y, int64_t M, void *d_scratch, void *d_partial);
 
+/* iamx_gemv_fused without any allocation: the LZ4 scratch and the partial
+ * sums live in a fixed device array of iamx_gemv_fused_noalloc_limit() bytes
+ * (64 MiB; decoded stream bytes rounded up to 256, then nbc x M partial sums),
+ * resolved once per device at iamx_init, which must have run.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  `scratch_bytes` is the extent the job table's decoded
+ * streams need (max dst_off + nbytes, padded to 16); K is the length of x.
+ * Requests above the limit fail with IAMX_EINVAL before any launch; job
+ * entries outside these sizes are skipped on the device and flag
+ * IAMX_STATUS_RANGE.  Every call on one GPU shares that GPU's array and
+ * leaves its contents behind: calls that overlap on one GPU (other streams,
+ * other threads) race. */
+IAMX_API int64_t iamx_gemv_fused_noalloc_limit(void);
+IAMX_API int iamx_gemv_fused_noalloc(int dtype, const iamx_lz4_job_t *d_lz4, int n_lz4,
+                                     const iamx_block_job_t *d_blocks, const int32_t *d_blk_rc, int n_blocks, int nbc,
+                                     int bs0, int bs1, const void *x, int64_t K, void *y, int64_t M,
+                                     int64_t scratch_bytes);
+
 /* ---- C write stage --------------------------------------------------------------
  * Replaces the shuffle/bitshuffle that blosc2_compress_ctx applies to every C
  * block before LZ4 (src/linalg/matmul/gemm.c:216-230): gathers each C block
@@ -271,6 +290,9 @@ IAMX_API void iamx_graph_destroy(void *graph_exec);
  * a non-zero word. */
 IAMX_API int iamx_status_snapshot(void);
 IAMX_API int iamx_decode_status(void);
+/* Status bit of a job table entry that points outside the buffers it was
+ * given (the job is skipped); bits 1..7 are malformed LZ4 streams. */
+#define IAMX_STATUS_RANGE (1 << 8)
 
 #ifdef __cplusplus
 }

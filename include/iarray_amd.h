@@ -68,6 +68,24 @@ INA_API(ina_rc_t) iarray_from_chunks(iarray_context_t *ctx, iarray_dtshape_t *dt
                                      int64_t nchunks, const uint8_t *const *chunks, const int32_t *csizes,
                                      iarray_container_t **container);
 
+/* The device decode plan of a container (built and uploaded on first use,
+ * owned by the container, valid until it is freed or changed): the job
+ * tables that libiamx.so's iamx_decode / iamx_gemv_fused /
+ * iamx_gemv_fused_noalloc consume, for callers that drive those entry points
+ * themselves.  Views have no plan of their own (INA_ERR_NOT_SUPPORTED). */
+typedef struct iarray_device_plan_s {
+  const void *lz4;       /* iamx_lz4_job_t[n_lz4], device memory */
+  int n_lz4;
+  const void *blocks;    /* iamx_block_job_t[n_blocks], device memory */
+  const int32_t *blk_rc; /* 4 int32 per block job (first row, first column, block column, 0), device memory */
+  int n_blocks;
+  int nbcols;            /* block columns of the global block grid */
+  int64_t scratch_bytes; /* decoded-stream scratch the LZ4 jobs need */
+  int64_t blockshape[2];
+} iarray_device_plan_t;
+
+INA_API(ina_rc_t) iarray_container_device_plan(iarray_container_t *c, iarray_device_plan_t *plan);
+
 /* Decode chunk bytes back to their flat uncompressed content on the GPU
  * (no tiling applied): used to check the device decoder on arbitrary chunks. */
 INA_API(ina_rc_t) iarray_decode_chunks_flat(iarray_context_t *ctx, int64_t nchunks, const uint8_t *const *chunks,

@@ -426,11 +426,17 @@ __device__ __forceinline__ void lz4_stream(const iamx_lz4_job_t job, uint8_t *__
 
 template <int RING>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RING <= 4096 ? 8 : 1, 8))) void k_lz4_streams(
-    const iamx_lz4_job_t *__restrict__ jobs, int njobs, uint8_t *__restrict__ scratch, int *status) {
+    const iamx_lz4_job_t *__restrict__ jobs, int njobs, uint8_t *__restrict__ scratch, int64_t cap, int *status) {
   __shared__ __align__(16) uint8_t ring[RING];
   const int j = blockIdx.x;
   if (j >= njobs) return;
-  lz4_stream<RING>(jobs[j], scratch, status, ring);
+  const iamx_lz4_job_t job = jobs[j];
+  // the decoded stream, padded to 16 bytes, must lie inside the scratch
+  if (job.dst_off < 0 || job.nbytes < 0 || job.dst_off > cap - (((int64_t)job.nbytes + 15) & ~(int64_t)15)) {
+    if (threadIdx.x == 0) atomicOr(status, IAMX_STATUS_RANGE);
+    return;
+  }
+  lz4_stream<RING>(job, scratch, status, ring);
 }
 
 struct Planes {
@@ -641,18 +647,18 @@ static int lz4_ring_kib() {
   return kib;
 }
 
-int launch_lz4(const char *name, const iamx_lz4_job_t *d_lz4, int n_lz4, void *d_scratch) {
+int launch_lz4(const char *name, const iamx_lz4_job_t *d_lz4, int n_lz4, void *d_scratch, int64_t cap) {
   switch (lz4_ring_kib()) {
     case 8:
-      IAMX_LAUNCH(name, k_lz4_streams<8192>, dim3(n_lz4), dim3(64), 0, d_lz4, n_lz4, (uint8_t *)d_scratch,
+      IAMX_LAUNCH(name, k_lz4_streams<8192>, dim3(n_lz4), dim3(64), 0, d_lz4, n_lz4, (uint8_t *)d_scratch, cap,
                   status_word());
       break;
     case 16:
-      IAMX_LAUNCH(name, k_lz4_streams<16384>, dim3(n_lz4), dim3(64), 0, d_lz4, n_lz4, (uint8_t *)d_scratch,
+      IAMX_LAUNCH(name, k_lz4_streams<16384>, dim3(n_lz4), dim3(64), 0, d_lz4, n_lz4, (uint8_t *)d_scratch, cap,
                   status_word());
       break;
     default:
-      IAMX_LAUNCH(name, k_lz4_streams<4096>, dim3(n_lz4), dim3(64), 0, d_lz4, n_lz4, (uint8_t *)d_scratch,
+      IAMX_LAUNCH(name, k_lz4_streams<4096>, dim3(n_lz4), dim3(64), 0, d_lz4, n_lz4, (uint8_t *)d_scratch, cap,
                   status_word());
       break;
   }

@@ -231,18 +231,33 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_gemv_fused(const iamx_block_job_t *__restrict__ jobs,
                                                     const int32_t *__restrict__ blk_rc, int njobs,
                                                     const T *__restrict__ x, const uint8_t *__restrict__ scratch,
-                                                    T *__restrict__ partial, int64_t M) {
+                                                    T *__restrict__ partial, int64_t M, int64_t K, int nbc,
+                                                    int64_t cap, int *status) {
   __shared__ iamx_block_job_t J;
   __shared__ Src S;
   __shared__ T xs[kMaxCols + kMaxCols / 32];
   __shared__ T p0[kGroups], p1[kGroups];
+  __shared__ int job_ok;
   const int b = blockIdx.x;
   if (b >= njobs) return;
   const int tid = threadIdx.x;
-  if (tid == 0) J = jobs[b];
-  __syncthreads();
   const int64_t row0 = blk_rc[4 * b], col0 = blk_rc[4 * b + 1];
   const int bcol = blk_rc[4 * b + 2];
+  if (tid == 0) {
+    J = jobs[b];
+    // every range the job names must lie inside the buffers of this call:
+    // scratch streams below `cap`, its rows inside y, its columns inside x,
+    // its partial-sum row inside the nbc block columns
+    bool ok = J.nstreams >= 1 && J.nstreams <= IAMX_MAX_STREAMS && J.cols >= 1 && J.cols <= kMaxCols &&
+              J.vcols >= 0 && J.vcols <= J.cols && J.vrows >= 0 && J.neblock >= 0 && bcol >= 0 && bcol < nbc &&
+              row0 >= 0 && row0 <= M - J.vrows && col0 >= 0 && col0 <= K - J.vcols;
+    for (int s = 0; ok && s < J.nstreams; s++)
+      if (J.kind[s] == IAMX_SRC_SCRATCH) ok = (int64_t)J.src[s] >= 0 && (int64_t)J.src[s] <= cap - J.neblock;
+    job_ok = ok;
+    if (!ok) atomicOr(status, IAMX_STATUS_RANGE);
+  }
+  __syncthreads();
+  if (!job_ok) return;
   if (tid < IAMX_MAX_STREAMS) {
     const int s = tid;
     S.ptr[s] = s >= J.nstreams                ? nullptr
@@ -393,7 +408,75 @@ __global__ __launch_bounds__(256) void k_reduce_cols(const T *__restrict__ parti
   y[r] = s;
 }
 
+__global__ __launch_bounds__(256) void k_zero_words(uint32_t *__restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = 0u;
+}
+
+// Decode + fused GEMV + column reduction on the current stream.  `cap` bounds
+// the decoded-stream bytes of the scratch and K the length of x; the kernels
+// check every job against them (IAMX_STATUS_RANGE).
+int gemv_fused_launch(int dtype, const iamx_lz4_job_t *d_lz4, int n_lz4, const iamx_block_job_t *d_blocks,
+                      const int32_t *d_blk_rc, int n_blocks, int nbc, const void *x, void *y, int64_t M, int64_t K,
+                      int64_t cap, void *d_scratch, void *d_partial) {
+  if (n_lz4 > 0) {
+    const int rc = iamx::launch_lz4("lz4_streams_gemv", d_lz4, n_lz4, d_scratch, cap);
+    if (rc) return rc;
+  }
+  const dim3 rgrid((unsigned)((M + 255) / 256));
+  int *st = iamx::status_word();
+  if (dtype == 0) {
+    IAMX_LAUNCH("gemv_fused_f64", k_gemv_fused<double>, dim3(n_blocks), dim3(256), 0, d_blocks, d_blk_rc, n_blocks,
+                (const double *)x, (const uint8_t *)d_scratch, (double *)d_partial, M, K, nbc, cap, st);
+    IAMX_LAUNCH("gemv_reduce_f64", k_reduce_cols<double>, rgrid, dim3(256), 0, (const double *)d_partial, nbc, M,
+                (double *)y);
+  } else if (dtype == 1) {
+    IAMX_LAUNCH("gemv_fused_f32", k_gemv_fused<float>, dim3(n_blocks), dim3(256), 0, d_blocks, d_blk_rc, n_blocks,
+                (const float *)x, (const uint8_t *)d_scratch, (float *)d_partial, M, K, nbc, cap, st);
+    IAMX_LAUNCH("gemv_reduce_f32", k_reduce_cols<float>, rgrid, dim3(256), 0, (const float *)d_partial, nbc, M,
+                (float *)y);
+  } else {
+    return iamx::set_error(IAMX_EINVAL, "iamx_gemv_fused: dtype %d", dtype);
+  }
+  return IAMX_OK;
+}
+
+// ---- the no-allocation GEMV's scratch ------------------------------------------------------
+// Decoded LZ4 streams, then the partial sums, of iamx_gemv_fused_noalloc: a
+// fixed device array (every GPU the library runs on holds its own copy until
+// the process ends).  Host code never passes `noalloc_scratch` itself to a
+// kernel (in host code the name is a stand-in, not a device address): the
+// address of this device's copy is resolved once per device by
+// hipGetSymbolAddress, with that device current, and kept.
+constexpr size_t kNoAllocBytes = (size_t)64 << 20;
+__device__ __align__(256) uint8_t noalloc_scratch[kNoAllocBytes];
+constexpr int kMaxDevices = 64;
+uint8_t *g_noalloc[kMaxDevices];
+
+int noalloc_ptr(uint8_t **out) {
+  int dev = -1;
+  IAMX_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return iamx::set_error(IAMX_EINVAL, "no-allocation scratch: device %d", dev);
+  if (!g_noalloc[dev]) {
+    const int rc = iamx::noalloc_setup();
+    if (rc) return rc;
+  }
+  *out = g_noalloc[dev];
+  return IAMX_OK;
+}
+
 }  // namespace
+
+int iamx::noalloc_setup() {
+  int dev = -1;
+  IAMX_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return set_error(IAMX_EINVAL, "no-allocation scratch: device %d", dev);
+  if (g_noalloc[dev]) return IAMX_OK;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch));
+  if (e != hipSuccess) return check_hip(e, "hipGetSymbolAddress(noalloc_scratch)");
+  g_noalloc[dev] = (uint8_t *)p;
+  return IAMX_OK;
+}
 
 extern "C" {
 
@@ -405,25 +488,40 @@ IAMX_API int iamx_gemv_fused(int dtype, const iamx_lz4_job_t *d_lz4, int n_lz4, 
   if (bs1 > kMaxCols) return iamx::set_error(IAMX_EINVAL, "iamx_gemv_fused: block width %d > %d", bs1, kMaxCols);
   (void)bs0;
   if (n_blocks <= 0 || nbc <= 0) return iamx::set_error(IAMX_EINVAL, "iamx_gemv_fused: no blocks");
-  if (n_lz4 > 0) {
-    rc = iamx::launch_lz4("lz4_streams_gemv", d_lz4, n_lz4, d_scratch);
-    if (rc) return rc;
-  }
-  const dim3 rgrid((unsigned)((M + 255) / 256));
-  if (dtype == 0) {
-    IAMX_LAUNCH("gemv_fused_f64", k_gemv_fused<double>, dim3(n_blocks), dim3(256), 0, d_blocks, d_blk_rc, n_blocks,
-                (const double *)x, (const uint8_t *)d_scratch, (double *)d_partial, M);
-    IAMX_LAUNCH("gemv_reduce_f64", k_reduce_cols<double>, rgrid, dim3(256), 0, (const double *)d_partial, nbc, M,
-                (double *)y);
-  } else if (dtype == 1) {
-    IAMX_LAUNCH("gemv_fused_f32", k_gemv_fused<float>, dim3(n_blocks), dim3(256), 0, d_blocks, d_blk_rc, n_blocks,
-                (const float *)x, (const uint8_t *)d_scratch, (float *)d_partial, M);
-    IAMX_LAUNCH("gemv_reduce_f32", k_reduce_cols<float>, rgrid, dim3(256), 0, (const float *)d_partial, nbc, M,
-                (float *)y);
-  } else {
-    return iamx::set_error(IAMX_EINVAL, "iamx_gemv_fused: dtype %d", dtype);
-  }
-  return IAMX_OK;
+  return gemv_fused_launch(dtype, d_lz4, n_lz4, d_blocks, d_blk_rc, n_blocks, nbc, x, y, M, INT64_MAX, INT64_MAX,
+                           d_scratch, d_partial);
+}
+
+IAMX_API int64_t iamx_gemv_fused_noalloc_limit(void) { return (int64_t)kNoAllocBytes; }
+
+IAMX_API int iamx_gemv_fused_noalloc(int dtype, const iamx_lz4_job_t *d_lz4, int n_lz4,
+                                     const iamx_block_job_t *d_blocks, const int32_t *d_blk_rc, int n_blocks, int nbc,
+                                     int bs0, int bs1, const void *x, int64_t K, void *y, int64_t M,
+                                     int64_t scratch_bytes) {
+  (void)bs0;
+  if (!iamx::initialized()) return iamx::set_error(IAMX_EINVAL, "iamx_gemv_fused_noalloc: call iamx_init first");
+  if (dtype != 0 && dtype != 1) return iamx::set_error(IAMX_EINVAL, "iamx_gemv_fused_noalloc: dtype %d", dtype);
+  if (bs1 > kMaxCols || bs1 < 1)
+    return iamx::set_error(IAMX_EINVAL, "iamx_gemv_fused_noalloc: block width %d outside 1..%d", bs1, kMaxCols);
+  if (n_blocks <= 0 || nbc <= 0 || n_lz4 < 0 || M <= 0 || K <= 0 || scratch_bytes < 0)
+    return iamx::set_error(IAMX_EINVAL, "iamx_gemv_fused_noalloc: bad sizes");
+  const int64_t isz = dtype == 0 ? 8 : 4;
+  const int64_t poff = (scratch_bytes + 255) / 256 * 256;
+  if (poff > (int64_t)kNoAllocBytes || (int64_t)nbc > ((int64_t)kNoAllocBytes - poff) / isz / M)
+    return iamx::set_error(IAMX_EINVAL,
+                           "iamx_gemv_fused_noalloc: needs %lld bytes of scratch (decoded streams) + %lld (partial "
+                           "sums), above its fixed %lld",
+                           (long long)scratch_bytes, (long long)((int64_t)nbc * M * isz), (long long)kNoAllocBytes);
+  uint8_t *base = nullptr;
+  int rc = noalloc_ptr(&base);
+  if (rc) return rc;
+  // the partial sums start at zero (blocks that hold no job leave theirs);
+  // a fill kernel, not a memset node
+  const int64_t pn = (int64_t)nbc * M * isz / 4;
+  IAMX_LAUNCH("gemv_noalloc_clear", k_zero_words, dim3((unsigned)((pn + 1023) / 1024 < 4096 ? (pn + 1023) / 1024 : 4096)),
+              dim3(256), 0, (uint32_t *)(base + poff), pn);
+  return gemv_fused_launch(dtype, d_lz4, n_lz4, d_blocks, d_blk_rc, n_blocks, nbc, x, y, M, K, scratch_bytes, base,
+                           base + poff);
 }
 
 }  // extern "C"

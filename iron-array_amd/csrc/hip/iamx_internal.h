@@ -19,6 +19,9 @@ extern int g_num_cus;
 int set_error(int code, const char *fmt, ...);
 int check_hip(hipError_t e, const char *what);
 int ensure_init();
+// True once ensure_init has succeeded (the no-allocation entry point refuses
+// to run before: init allocates the stream and status word).
+bool initialized();
 
 // True between iamx_capture_begin and iamx_capture_end.
 bool capturing();
@@ -26,8 +29,15 @@ bool capturing();
 // Device word the decode kernels OR their error bits into.
 int *status_word();
 
-// Launch the LZ4 stream decoder over n_lz4 jobs into d_scratch, profiled as `name`.
-int launch_lz4(const char *name, const iamx_lz4_job_t *d_lz4, int n_lz4, void *d_scratch);
+// Launch the LZ4 stream decoder over n_lz4 jobs into d_scratch, profiled as
+// `name`; a job whose decoded bytes would not fit below byte `cap` of the
+// scratch is skipped and flags IAMX_STATUS_RANGE.
+int launch_lz4(const char *name, const iamx_lz4_job_t *d_lz4, int n_lz4, void *d_scratch,
+               int64_t cap = INT64_MAX);
+
+// Resolves this device's copy of the no-allocation scratch array (once per
+// device, with the device current); called by the library's init.
+int noalloc_setup();
 
 // Profiling: brackets one launch with HIP events when enabled.
 struct ProfScope {

@@ -66,9 +66,14 @@ int ensure_init() {
   if ((e = hipMalloc(&g_status, 256)) != hipSuccess) return check_hip(e, "hipMalloc(status)");
   if ((e = hipMemset(g_status, 0, 256)) != hipSuccess) return check_hip(e, "hipMemset(status)");
   if ((e = hipDeviceSynchronize()) != hipSuccess) return check_hip(e, "hipDeviceSynchronize");
+  // this device's copy of the no-allocation scratch
+  const int rc = noalloc_setup();
+  if (rc) return rc;
   g_init_state = 1;
   return IAMX_OK;
 }
+
+bool initialized() { return g_init_state == 1; }
 
 int *status_word() { return g_status; }
 

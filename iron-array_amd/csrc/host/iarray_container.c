@@ -1285,6 +1285,25 @@ INA_API(ina_rc_t) iarray_get_slice(iarray_context_t *ctx, iarray_container_t *sr
   return INA_SUCCESS;
 }
 
+INA_API(ina_rc_t) iarray_container_device_plan(iarray_container_t *c, iarray_device_plan_t *plan) {
+  FAIL_NULL(c);
+  FAIL_NULL(plan);
+  if (c->viewed) return iax_set_error(INA_ERROR(INA_ERR_NOT_SUPPORTED), "views have no decode plan of their own");
+  ina_rc_t rc = iax_ensure_plan(c);
+  if (INA_FAILED(rc)) return rc;
+  memset(plan, 0, sizeof(*plan));
+  plan->lz4 = c->plan.d_lz4;
+  plan->n_lz4 = c->plan.n_lz4;
+  plan->blocks = c->plan.d_blocks;
+  plan->blk_rc = c->plan.d_blk_rc;
+  plan->n_blocks = c->plan.n_blocks;
+  plan->nbcols = (int)c->plan.nbcols;
+  plan->scratch_bytes = c->plan.scratch_bytes;
+  plan->blockshape[0] = c->block[0];
+  plan->blockshape[1] = c->block[1];
+  return INA_SUCCESS;
+}
+
 /* ---- raw chunk access -------------------------------------------------------------------------------- */
 INA_API(ina_rc_t) iarray_container_nchunks(iarray_container_t *c, int64_t *nchunks) {
   FAIL_NULL(c);

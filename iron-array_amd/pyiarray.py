@@ -130,6 +130,7 @@ def lib():
         "iarray_last_gpu_encode_bytes": ([P(i64), P(i64)], None),
         "iarray_last_ozaki_slices": ([P(ctypes.c_double)], ctypes.c_int),
         "iarray_device_info": ([ctypes.c_char_p, ctypes.c_int, P(ctypes.c_int), ctypes.c_char_p, ctypes.c_int], rc_t),
+        "iarray_container_device_plan": ([vp, vp], rc_t),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
