@@ -624,7 +624,7 @@ INA_API(ina_rc_t) iarray_linalg_matmul_batch(iarray_context_t *ctx, int n, iarra
     for (int d = 0; d < 2 && INA_SUCCEED(rc); d++)
       if (dts.shape[d] < storage[i].chunkshape[d])
         rc = iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_CHUNKSHAPE), "product %d: chunkshape", i);
-    if (INA_SUCCEED(rc)) rc = iarray_empty(ctx, &dts, &storage[i], &c[i]);
+    if (INA_SUCCEED(rc)) rc = iax_container_new(ctx, &dts, &storage[i], &c[i]);
     if (INA_SUCCEED(rc)) rc = iax_ensure_plan(a[i]);
     if (INA_SUCCEED(rc)) rc = iax_ensure_plan(b[i]);
   }
@@ -650,6 +650,7 @@ INA_API(ina_rc_t) iarray_linalg_matmul_batch(iarray_context_t *ctx, int n, iarra
   const ina_rc_t st = iax_check_decode_status();
   iax_last_device_ms = iamx_prof_span_ms();
   if (INA_SUCCEED(rc)) rc = st;
+  for (int i = 0; i < n && INA_SUCCEED(rc); i++) rc = iax_persist(c[i]);
   /* a failed batch leaves no partly written outputs behind */
   if (INA_FAILED(rc))
     for (int i = 0; i < n; i++) iarray_container_free(ctx, &c[i]);

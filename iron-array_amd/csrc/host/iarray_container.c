@@ -75,8 +75,8 @@ ina_rc_t iax_container_new(iarray_context_t *ctx, iarray_dtshape_t *dtshape, iar
                          dtshape->ndim);
   const int64_t is = iax_dtype_size(dtshape->dtype);
   if (is < 0) return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_DTYPE), "dtype %d", dtshape->dtype);
-  if (storage->urlpath)
-    return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_STORAGE), "on-disk frames (urlpath) are not supported");
+  if (storage->urlpath && !storage->urlpath[0])
+    return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_STORAGE), "empty urlpath");
   ina_rc_t rc = check_config(&ctx->cfg);
   if (INA_FAILED(rc)) return rc;
   for (int i = 0; i < dtshape->ndim; i++) {
@@ -94,6 +94,14 @@ ina_rc_t iax_container_new(iarray_context_t *ctx, iarray_dtshape_t *dtshape, iar
   c->dtshape.dtype_size = (int32_t)is;
   c->storage = *storage;
   c->storage.urlpath = NULL;
+  if (storage->urlpath) {
+    /* stored on disk: the frame at urlpath follows the chunks (iax_persist) */
+    if (!(c->storage.urlpath = strdup(storage->urlpath))) {
+      free(c);
+      return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+    }
+    c->on_disk = true;
+  }
   c->cfg = ctx->cfg;
   c->itemsize = (int)is;
   c->ndim = dtshape->ndim;
@@ -118,7 +126,7 @@ ina_rc_t iax_container_new(iarray_context_t *ctx, iarray_dtshape_t *dtshape, iar
   c->csizes = (int32_t *)calloc((size_t)c->nchunks, sizeof(int32_t));
   c->arena_off = (int64_t *)calloc((size_t)c->nchunks, sizeof(int64_t));
   if (!c->chunks || !c->csizes || !c->arena_off) {
-    free(c->chunks); free(c->csizes); free(c->arena_off); free(c);
+    free(c->chunks); free(c->csizes); free(c->arena_off); free(c->storage.urlpath); free(c);
     return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
   }
   *cp = c;
@@ -152,6 +160,7 @@ INA_API(void) iarray_container_free(iarray_context_t *ctx, iarray_container_t **
     free(c->chunks);
     free(c->csizes);
     free(c->arena_off);
+    free(c->storage.urlpath); /* the frame on disk stays */
   }
   free(c);
   *cp = NULL;
@@ -159,7 +168,8 @@ INA_API(void) iarray_container_free(iarray_context_t *ctx, iarray_container_t **
 
 INA_API(ina_rc_t) iarray_empty(iarray_context_t *ctx, iarray_dtshape_t *dtshape, iarray_storage_t *storage,
                                iarray_container_t **container) {
-  return iax_container_new(ctx, dtshape, storage, container);
+  ina_rc_t rc = iax_container_new(ctx, dtshape, storage, container);
+  return INA_SUCCEED(rc) ? iax_persist(*container) : rc;
 }
 
 /* A special-value chunk: a 32-byte header only, the value kind (1 zeros,
@@ -186,7 +196,7 @@ INA_API(ina_rc_t) iarray_zeros(iarray_context_t *ctx, iarray_dtshape_t *dtshape,
   iarray_container_t *c = *container;
   for (int64_t i = 0; i < c->nchunks; i++)
     if (!(c->chunks[i] = zero_chunk(c, &c->csizes[i]))) return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
-  return INA_SUCCESS;
+  return iax_persist(c);
 }
 
 /* ---- host encode of a row-major buffer --------------------------------------------------- */
@@ -248,7 +258,8 @@ INA_API(ina_rc_t) iarray_from_buffer(iarray_context_t *ctx, iarray_dtshape_t *dt
   iarray_container_t *c = *container;
   if (c->shape[0] * c->shape[1] * c->itemsize > buflen)
     return iax_set_error(INA_ERROR(IARRAY_ERR_TOO_SMALL_BUFFER), "The size of the buffer is not enough");
-  return encode_buffer(c, buffer);
+  rc = encode_buffer(c, buffer);
+  return INA_SUCCEED(rc) ? iax_persist(c) : rc;
 }
 
 static ina_rc_t from_ramp(iarray_context_t *ctx, iarray_dtshape_t *dtshape, double start, double step,
@@ -1222,6 +1233,7 @@ INA_API(ina_rc_t) iarray_linalg_transpose(iarray_context_t *ctx, iarray_containe
   memset(&v->plan, 0, sizeof(v->plan));
   v->frame.valid = 0;
   v->on_disk = false;
+  v->storage.urlpath = NULL;
   memset(v->tuned, 0, sizeof(v->tuned));
   *b = v;
   return INA_SUCCESS;
@@ -1280,6 +1292,7 @@ INA_API(ina_rc_t) iarray_get_slice(iarray_context_t *ctx, iarray_container_t *sr
   memset(&v->plan, 0, sizeof(v->plan));
   v->frame.valid = 0;
   v->on_disk = false;
+  v->storage.urlpath = NULL;
   memset(v->tuned, 0, sizeof(v->tuned));
   *container = v;
   return INA_SUCCESS;
@@ -1346,7 +1359,7 @@ INA_API(ina_rc_t) iarray_from_chunks(iarray_context_t *ctx, iarray_dtshape_t *dt
     memcpy(c->chunks[i], chunks[i], (size_t)csizes[i]);
     c->csizes[i] = csizes[i];
   }
-  return INA_SUCCESS;
+  return iax_persist(c);
 }
 
 INA_API(ina_rc_t) iarray_decode_chunks_flat(iarray_context_t *ctx, int64_t nchunks, const uint8_t *const *chunks,

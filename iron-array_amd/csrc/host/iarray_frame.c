@@ -176,7 +176,37 @@ static void read_params(mp_t *m, iax_frame_params_t *fp) {
   fp->valid = form && !m->bad && m->p - p0 == 3 + 3 + 1 + 18;
 }
 
-static ina_rc_t container_from_frame(iarray_context_t *ctx, const uint8_t *d, int64_t size,
+/* Reads chunk file `id` of a sparse frame directory (%08X.chunk). */
+static uint8_t *read_chunk_file(const char *dir, int64_t id, int32_t *csize) {
+  char path[4096];
+  if (snprintf(path, sizeof(path), "%s/%08X.chunk", dir, (unsigned)id) >= (int)sizeof(path)) return NULL;
+  FILE *f = fopen(path, "rb");
+  if (!f) return NULL;
+  uint8_t *b = NULL;
+  long n = -1;
+  if (fseek(f, 0, SEEK_END) == 0) n = ftell(f);
+  if (n >= 32 && n <= INT32_MAX && fseek(f, 0, SEEK_SET) == 0 && (b = (uint8_t *)malloc((size_t)n)) != NULL &&
+      fread(b, 1, (size_t)n, f) != (size_t)n) {
+    free(b);
+    b = NULL;
+  }
+  fclose(f);
+  if (b) {
+    int32_t cs;
+    memcpy(&cs, b + 12, 4);
+    if (cs < 32 || cs > n) {  /* the chunk's own size must fit the file */
+      free(b);
+      return NULL;
+    }
+    *csize = cs;
+  }
+  return b;
+}
+
+/* `dir` non-NULL: d is the index frame (chunks.b2frame) of a sparse frame
+ * directory; its offsets chunk holds chunk-file numbers instead of byte
+ * offsets and follows the header directly (the index has no data section). */
+static ina_rc_t container_from_frame(iarray_context_t *ctx, const uint8_t *d, int64_t size, const char *dir,
                                      iarray_container_t **container) {
   if (size < 64 || memcmp(d, frame_magic, sizeof(frame_magic)) != 0) return frame_error("no b2frame header");
   mp_t m = {d + 10, d + size, 0};
@@ -259,8 +289,9 @@ static ina_rc_t container_from_frame(iarray_context_t *ctx, const uint8_t *d, in
 
   /* chunk offsets: the chunk after the data, stored (memcpyed) */
   /* bounded before adding: hsize and cbytes come from the file */
-  if (cbytes > size - hsize - 32) return frame_error("offsets chunk position");
-  const int64_t opos = hsize + cbytes;
+  const int64_t dbytes = dir ? 0 : cbytes;  /* bytes of the data section */
+  if (dbytes > size - hsize - 32) return frame_error("offsets chunk position");
+  const int64_t opos = hsize + dbytes;
   const uint8_t *oc = d + opos;
   int32_t onb, ocb;
   memcpy(&onb, oc + 4, 4);
@@ -301,6 +332,14 @@ static ina_rc_t container_from_frame(iarray_context_t *ctx, const uint8_t *d, in
       csz[i] = 32;
       continue;
     }
+    if (dir) {
+      if (!(special[i] = read_chunk_file(dir, off, &csz[i]))) {
+        rc = frame_error("chunk file");
+        break;
+      }
+      ptrs[i] = special[i];
+      continue;
+    }
     if (off > opos - hsize - 32) {
       rc = frame_error("chunk offset");
       break;
@@ -327,7 +366,12 @@ static ina_rc_t container_from_frame(iarray_context_t *ctx, const uint8_t *d, in
 static ina_rc_t load(iarray_context_t *ctx, const char *urlpath, bool on_disk, iarray_container_t **container) {
   if (!ctx || !urlpath || !container) return INA_ERROR(INA_ERR_NULL_POINTER);
   if (access(urlpath, F_OK) != 0) return iax_set_error(INA_ERROR(INA_ERR_FILE_OPEN), "%s does not exist", urlpath);
-  FILE *f = fopen(urlpath, "rb");
+  struct stat sb;
+  const bool sparse = stat(urlpath, &sb) == 0 && S_ISDIR(sb.st_mode);
+  char ipath[4096];
+  if (sparse && snprintf(ipath, sizeof(ipath), "%s/chunks.b2frame", urlpath) >= (int)sizeof(ipath))
+    return iax_set_error(INA_ERROR(INA_ERR_FILE_OPEN), "path too long");
+  FILE *f = fopen(sparse ? ipath : urlpath, "rb");
   if (!f) return iax_set_error(INA_ERROR(IARRAY_ERR_CATERVA_FAILED), "cannot open %s", urlpath);
   uint8_t *d = NULL;
   int64_t size = -1;
@@ -339,7 +383,7 @@ static ina_rc_t load(iarray_context_t *ctx, const char *urlpath, bool on_disk, i
   }
   fclose(f);
   if (!d) return iax_set_error(INA_ERROR(IARRAY_ERR_CATERVA_FAILED), "cannot read %s", urlpath);
-  const ina_rc_t rc = container_from_frame(ctx, d, size, container);
+  const ina_rc_t rc = container_from_frame(ctx, d, size, sparse ? urlpath : NULL, container);
   free(d);
   if (INA_SUCCEED(rc)) (*container)->on_disk = on_disk;
   return rc;
@@ -360,7 +404,7 @@ INA_API(ina_rc_t) iarray_from_cframe(iarray_context_t *ctx, uint8_t *frame, int6
                                      iarray_container_t **container) {
   (void)copy; /* the chunks are always copied into the container */
   if (!ctx || !frame || !container) return INA_ERROR(INA_ERR_NULL_POINTER);
-  return container_from_frame(ctx, frame, frame_len, container);
+  return container_from_frame(ctx, frame, frame_len, NULL, container);
 }
 
 /* ---- writing a frame ------------------------------------------------------------
@@ -399,7 +443,10 @@ static uint8_t *put_str(uint8_t *p, const char *s) {
   return p + n;
 }
 
-static ina_rc_t frame_write(const iarray_container_t *c, uint8_t **out, int64_t *out_len) {
+/* sparse: the index frame of a sparse frame directory (frame type 1 in the
+ * second flag byte, no data section, chunk-file numbers as offsets); the
+ * chunks go to their own files (sparse_write). */
+static ina_rc_t frame_write(const iarray_container_t *c, bool sparse, uint8_t **out, int64_t *out_len) {
   if (c->viewed)
     return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_STORAGE),
                          "Container must be stored on a blosc schunk and must not be a view");
@@ -447,11 +494,13 @@ static ina_rc_t frame_write(const iarray_container_t *c, uint8_t **out, int64_t 
     fp.coding[5] = (uint8_t)(f == IAMX_FILTER_SHUFFLE ? 1 : f == IAMX_FILTER_BITSHUFFLE ? 2 : 0);
     fp.coding[6] = (uint8_t)codec;
   }
+  fp.flags[1] = sparse ? 1 : 0;
+  const int64_t dbytes = sparse ? 0 : cbytes;
 
   const int64_t meta_at = 87, cat_off = meta_at + 32 + 3, ia_off = cat_off + 5 + catlen;
   const int64_t hsize = ia_off + 5 + (int64_t)sizeof(ia);
   const int64_t olen = 32 + 8 * n, tlen = 35;
-  const int64_t fsize = hsize + cbytes + olen + tlen;
+  const int64_t fsize = hsize + dbytes + olen + tlen;
   uint8_t *d = (uint8_t *)malloc((size_t)fsize);
   if (!d) return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
   uint8_t *p = d;
@@ -467,7 +516,7 @@ static ina_rc_t frame_write(const iarray_container_t *c, uint8_t **out, int64_t 
   *p++ = 0xd3;
   p = put(p, (uint64_t)(n * c->chunk_nbytes), 8);
   *p++ = 0xd3;
-  p = put(p, (uint64_t)cbytes, 8);
+  p = put(p, (uint64_t)dbytes, 8);
   const int64_t i32s[3] = {c->itemsize, c->blocksize, c->chunk_nbytes};
   for (int i = 0; i < 3; i++) {
     *p++ = 0xd2;
@@ -505,9 +554,13 @@ static ina_rc_t frame_write(const iarray_container_t *c, uint8_t **out, int64_t 
   memcpy(p, ia, sizeof(ia));
   p += sizeof(ia);
   /* chunks */
-  uint8_t *offs = d + hsize + cbytes + 32;
+  uint8_t *offs = d + hsize + dbytes + 32;
   int64_t at = 0;
   for (int64_t i = 0; i < n; i++) {
+    if (sparse) {
+      put_le(offs + 8 * i, (uint64_t)i, 8);
+      continue;
+    }
     memcpy(p, c->chunks[i], (size_t)c->csizes[i]);
     p += c->csizes[i];
     put_le(offs + 8 * i, (uint64_t)at, 8);
@@ -549,10 +602,65 @@ static ina_rc_t frame_write(const iarray_container_t *c, uint8_t **out, int64_t 
   return INA_SUCCESS;
 }
 
+static int write_file(const char *path, const uint8_t *d, int64_t len) {
+  FILE *f = fopen(path, "wb");
+  int ok = f && fwrite(d, 1, (size_t)len, f) == (size_t)len;
+  if (f && fclose(f) != 0) ok = 0;
+  return ok;
+}
+
+static ina_rc_t write_contiguous(const iarray_container_t *c, const char *urlpath) {
+  uint8_t *d;
+  int64_t len;
+  ina_rc_t rc = frame_write(c, false, &d, &len);
+  if (INA_FAILED(rc)) return rc;
+  const int ok = write_file(urlpath, d, len);
+  free(d);
+  if (!ok) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "cannot write %s", urlpath);
+  return INA_SUCCESS;
+}
+
+/* A sparse frame directory: chunks.b2frame (the index frame) and one
+ * %08X.chunk file per chunk; whatever the path held before is replaced. */
+static ina_rc_t write_sparse(const iarray_container_t *c, const char *urlpath) {
+  ina_rc_t rc = iarray_container_remove((char *)urlpath);
+  if (INA_FAILED(rc)) return rc;
+  if (mkdir(urlpath, 0755) != 0) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "cannot create %s", urlpath);
+  uint8_t *d;
+  int64_t len;
+  rc = frame_write(c, true, &d, &len);
+  if (INA_FAILED(rc)) return rc;
+  char path[4096];
+  int ok = snprintf(path, sizeof(path), "%s/chunks.b2frame", urlpath) < (int)sizeof(path) && write_file(path, d, len);
+  free(d);
+  for (int64_t i = 0; ok && i < c->nchunks; i++)
+    ok = snprintf(path, sizeof(path), "%s/%08X.chunk", urlpath, (unsigned)i) < (int)sizeof(path) &&
+         write_file(path, c->chunks[i], c->csizes[i]);
+  if (!ok) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "cannot write %s", urlpath);
+  return INA_SUCCESS;
+}
+
+/* A container created with storage.urlpath keeps its chunks in memory (and
+ * HBM) and writes them through to its frame once they are complete: a
+ * contiguous frame file, or a sparse frame directory when
+ * storage.contiguous is false (src/iarray_constructor.c:16-45 hands the
+ * storage to Caterva, which writes the super-chunk to that path).  Chunks
+ * not written yet are stored as uninitialized special chunks. */
+ina_rc_t iax_persist(iarray_container_t *c) {
+  if (!c || c->viewed || !c->storage.urlpath) return INA_SUCCESS;
+  for (int64_t i = 0; i < c->nchunks; i++)
+    if (!c->chunks[i]) {
+      if (!(c->chunks[i] = iax_special_chunk(4, (int32_t)c->chunk_nbytes, c->blocksize, c->itemsize)))
+        return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+      c->csizes[i] = 32;
+    }
+  return c->storage.contiguous ? write_contiguous(c, c->storage.urlpath) : write_sparse(c, c->storage.urlpath);
+}
+
 INA_API(ina_rc_t) iarray_to_cframe(iarray_context_t *ctx, iarray_container_t *src, uint8_t **frame,
                                    int64_t *frame_len, bool *needs_free) {
   if (!ctx || !src || !frame || !frame_len || !needs_free) return INA_ERROR(INA_ERR_NULL_POINTER);
-  const ina_rc_t rc = frame_write(src, frame, frame_len);
+  const ina_rc_t rc = frame_write(src, false, frame, frame_len);
   *needs_free = INA_SUCCEED(rc);
   return rc;
 }
@@ -560,16 +668,7 @@ INA_API(ina_rc_t) iarray_to_cframe(iarray_context_t *ctx, iarray_container_t *sr
 INA_API(ina_rc_t) iarray_container_save(iarray_context_t *ctx, iarray_container_t *container, char *urlpath) {
   if (!ctx || !container || !urlpath) return INA_ERROR(INA_ERR_NULL_POINTER);
   if (container->on_disk && !container->viewed) return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_STORAGE), "Container is already on disk");
-  uint8_t *d;
-  int64_t len;
-  ina_rc_t rc = frame_write(container, &d, &len);
-  if (INA_FAILED(rc)) return rc;
-  FILE *f = fopen(urlpath, "wb");
-  int ok = f && fwrite(d, 1, (size_t)len, f) == (size_t)len;
-  if (f && fclose(f) != 0) ok = 0;
-  free(d);
-  if (!ok) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "cannot write %s", urlpath);
-  return INA_SUCCESS;
+  return write_contiguous(container, urlpath);
 }
 
 INA_API(ina_rc_t) iarray_container_remove(char *urlpath) {

@@ -254,7 +254,7 @@ static ina_rc_t new_output(iarray_context_t *ctx, iarray_container_t *a, iarray_
   for (int i = 0; i < dts.ndim; i++)
     if (dts.shape[i] < storage->chunkshape[i])
       return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_CHUNKSHAPE), "output chunkshape exceeds the output shape");
-  return iarray_empty(ctx, &dts, storage, c);
+  return iax_container_new(ctx, &dts, storage, c);
 }
 
 static ina_rc_t matmul_impl(iarray_context_t *ctx, iarray_container_t *a, iarray_container_t *b,
@@ -273,6 +273,7 @@ static ina_rc_t matmul_impl(iarray_context_t *ctx, iarray_container_t *a, iarray
   iamx_prof_reset();
   rc = b->dtshape.ndim == 2 ? run_gemm(a, b, *c, params) : run_gemv(a, b, *c);
   iax_last_device_ms = iamx_prof_span_ms();
+  if (INA_SUCCEED(rc)) rc = iax_persist(*c);
   /* a failed product leaves no partly written output behind */
   if (INA_FAILED(rc)) iarray_container_free(ctx, c);
   return rc;

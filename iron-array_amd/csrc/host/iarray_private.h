@@ -160,6 +160,8 @@ ina_rc_t iax_operand(iarray_container_t *c, int slot, int tmp_slot, void **d_out
 void iax_operand_forget(int slot);
 ina_rc_t iax_check_decode_status(void);
 uint8_t *iax_special_chunk(int kind, int32_t nbytes, int32_t blocksize, int typesize);
+/* writes an on-disk container (storage.urlpath) through to its frame */
+ina_rc_t iax_persist(iarray_container_t *c);
 /* Write C from a dense device buffer (logical shape, ld = shape1): GPU pack +
  * host LZ4 encode; fills c->chunks. */
 ina_rc_t iax_write_from_dense(iarray_container_t *c, const void *d_dense);

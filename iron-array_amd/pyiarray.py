@@ -183,10 +183,12 @@ def _dtshape(dtype, shape):
     return d
 
 
-def _storage(chunks, blocks):
+def _storage(chunks, blocks, urlpath=None, contiguous=False):
+    """iarray_storage_t: with a urlpath the container is also kept on disk, as
+    one frame file (contiguous) or a sparse frame directory."""
     s = Storage()
-    s.urlpath = None
-    s.contiguous = False
+    s.urlpath = os.fsencode(urlpath) if urlpath is not None else None
+    s.contiguous = bool(contiguous)
     for i, c in enumerate(chunks):
         s.chunkshape[i] = c
     for i, b in enumerate(blocks):
@@ -235,40 +237,42 @@ class Container:
             _lib.iarray_container_free(self.ctx.ptr, ctypes.byref(self.ptr))
 
     @classmethod
-    def _new(cls, fn, ctx, dtype, shape, chunks, blocks, *extra):
+    def _new(cls, fn, ctx, dtype, shape, chunks, blocks, *extra, urlpath=None, contiguous=False):
         p = ctypes.c_void_p()
         d = _dtshape(dtype, shape)
-        s = _storage(chunks, blocks)
+        s = _storage(chunks, blocks, urlpath, contiguous)
         _check(fn(ctx.ptr, ctypes.byref(d), *extra, ctypes.byref(s), ctypes.byref(p)))
         return cls(p, ctx, dtype, shape)
 
     @classmethod
-    def from_numpy(cls, ctx, arr, chunks, blocks):
+    def from_numpy(cls, ctx, arr, chunks, blocks, urlpath=None, contiguous=False):
         arr = np.ascontiguousarray(arr)
         p = ctypes.c_void_p()
         d = _dtshape(arr.dtype, arr.shape)
-        s = _storage(chunks, blocks)
+        s = _storage(chunks, blocks, urlpath, contiguous)
         _check(lib().iarray_from_buffer(ctx.ptr, ctypes.byref(d), arr.ctypes.data_as(ctypes.c_void_p), arr.nbytes,
                                         ctypes.byref(s), ctypes.byref(p)))
         return cls(p, ctx, arr.dtype, arr.shape)
 
     @classmethod
-    def arange(cls, ctx, dtype, shape, chunks, blocks, start=0.0, step=1.0):
+    def arange(cls, ctx, dtype, shape, chunks, blocks, start=0.0, step=1.0, urlpath=None, contiguous=False):
         return cls._new(lib().iarray_arange, ctx, dtype, shape, chunks, blocks, ctypes.c_double(start),
-                        ctypes.c_double(step))
+                        ctypes.c_double(step), urlpath=urlpath, contiguous=contiguous)
 
     @classmethod
-    def linspace(cls, ctx, dtype, shape, chunks, blocks, start, stop):
+    def linspace(cls, ctx, dtype, shape, chunks, blocks, start, stop, urlpath=None, contiguous=False):
         return cls._new(lib().iarray_linspace, ctx, dtype, shape, chunks, blocks, ctypes.c_double(start),
-                        ctypes.c_double(stop))
+                        ctypes.c_double(stop), urlpath=urlpath, contiguous=contiguous)
 
     @classmethod
-    def zeros(cls, ctx, dtype, shape, chunks, blocks):
-        return cls._new(lib().iarray_zeros, ctx, dtype, shape, chunks, blocks)
+    def zeros(cls, ctx, dtype, shape, chunks, blocks, urlpath=None, contiguous=False):
+        return cls._new(lib().iarray_zeros, ctx, dtype, shape, chunks, blocks, urlpath=urlpath,
+                        contiguous=contiguous)
 
     @classmethod
-    def empty(cls, ctx, dtype, shape, chunks, blocks):
-        return cls._new(lib().iarray_empty, ctx, dtype, shape, chunks, blocks)
+    def empty(cls, ctx, dtype, shape, chunks, blocks, urlpath=None, contiguous=False):
+        return cls._new(lib().iarray_empty, ctx, dtype, shape, chunks, blocks, urlpath=urlpath,
+                        contiguous=contiguous)
 
     @classmethod
     def from_chunks(cls, ctx, dtype, shape, chunks_shape, blocks, chunk_bytes):
@@ -376,9 +380,9 @@ def _out_shape(a, b):
     return (a.shape[0],) if len(b.shape) == 1 else (a.shape[0], b.shape[1])
 
 
-def matmul(a, b, chunks, blocks, precision=PLAIN, slices=0):
+def matmul(a, b, chunks, blocks, precision=PLAIN, slices=0, urlpath=None, contiguous=False):
     p = ctypes.c_void_p()
-    s = _storage(chunks, blocks)
+    s = _storage(chunks, blocks, urlpath, contiguous)
     if precision == PLAIN and slices == 0:
         _check(lib().iarray_linalg_matmul(a.ctx.ptr, a.ptr, b.ptr, ctypes.byref(s), ctypes.byref(p)))
     else:
@@ -388,30 +392,30 @@ def matmul(a, b, chunks, blocks, precision=PLAIN, slices=0):
     return Container(p, a.ctx, a.dtype, _out_shape(a, b))
 
 
-def opt_gemm(a, b, chunks, blocks):
+def opt_gemm(a, b, chunks, blocks, urlpath=None, contiguous=False):
     p = ctypes.c_void_p()
-    s = _storage(chunks, blocks)
+    s = _storage(chunks, blocks, urlpath, contiguous)
     _check(lib().iarray_opt_gemm(a.ctx.ptr, a.ptr, b.ptr, ctypes.byref(s), ctypes.byref(p)))
     return Container(p, a.ctx, a.dtype, _out_shape(a, b))
 
 
-def opt_gemm_a(a, b, chunks, blocks):
+def opt_gemm_a(a, b, chunks, blocks, urlpath=None, contiguous=False):
     p = ctypes.c_void_p()
-    s = _storage(chunks, blocks)
+    s = _storage(chunks, blocks, urlpath, contiguous)
     _check(lib().iarray_opt_gemm_a(a.ctx.ptr, a.ptr, b.ptr, ctypes.byref(s), ctypes.byref(p)))
     return Container(p, a.ctx, a.dtype, _out_shape(a, b))
 
 
-def opt_gemm_b(a, b, chunks, blocks):
+def opt_gemm_b(a, b, chunks, blocks, urlpath=None, contiguous=False):
     p = ctypes.c_void_p()
-    s = _storage(chunks, blocks)
+    s = _storage(chunks, blocks, urlpath, contiguous)
     _check(lib().iarray_opt_gemm_b(a.ctx.ptr, a.ptr, b.ptr, ctypes.byref(s), ctypes.byref(p)))
     return Container(p, a.ctx, a.dtype, _out_shape(a, b))
 
 
-def opt_gemv(a, b, chunks, blocks):
+def opt_gemv(a, b, chunks, blocks, urlpath=None, contiguous=False):
     p = ctypes.c_void_p()
-    s = _storage(chunks, blocks)
+    s = _storage(chunks, blocks, urlpath, contiguous)
     _check(lib().iarray_opt_gemv(a.ctx.ptr, a.ptr, b.ptr, ctypes.byref(s), ctypes.byref(p)))
     return Container(p, a.ctx, a.dtype, _out_shape(a, b))
 

@@ -474,9 +474,47 @@ def test_constructor_errors():
     assert _err(lambda: ia.Container.arange(ctx, np.float64, (4, 4, 4), (2, 2, 2), (1, 1, 1))) == ia.ERR_INVALID_NDIM
     assert _err(lambda: ia.Container.arange(ctx, np.float64, (8, 8), (4, 4), (5, 2))) == ia.ERR_INVALID_BLOCKSHAPE
     s = ia._storage((4, 4), (2, 2))
-    s.urlpath = b"x.iarr"
+    s.urlpath = b""
     rc = ia.lib().iarray_empty(ctx.ptr, ctypes.byref(d), ctypes.byref(s), ctypes.byref(p))
     assert rc % (1 << 64) - ia.ERR_BIT == ia.ERR_INVALID_STORAGE
+
+
+@pytest.mark.parametrize("contiguous", [True, False], ids=["frame-file", "sparse-dir"])
+def test_storage_urlpath_persists(orc, tmp_path, contiguous):
+    """A container created with storage.urlpath (src/iarray_constructor.c:
+    16-45) is written through to a contiguous frame file, or to a sparse frame
+    directory (chunks.b2frame index + one %08X.chunk file per chunk) when
+    contiguous is false; loading the path gives the same shape, dtype and
+    chunks, the oracle decodes them to the data, it counts as stored on disk
+    (save refuses, as in src/iarray_container.c:57-66), and creating at the
+    same path again replaces it."""
+    ctx = _ctx()
+    rng = np.random.default_rng(4)
+    x = rng.integers(-50, 50, (130, 70)).astype(np.float64)
+    path = str(tmp_path / "x.iarr")
+    c = ia.Container.from_numpy(ctx, x, (40, 30), (20, 15), urlpath=path, contiguous=contiguous)
+    if contiguous:
+        assert os.path.isfile(path)
+    else:
+        names = sorted(os.listdir(path))
+        assert names[-1] == "chunks.b2frame" and len(names) == 1 + 4 * 3
+        assert names[0] == "00000000.chunk" and "0000000B.chunk" in names
+    d = ia.Container.load(ctx, path)
+    assert d.shape == x.shape and d.dtype == x.dtype and d.chunks() == c.chunks()
+    assert np.array_equal(orc.Array(d.chunks(), x.shape, (40, 30), (20, 15), 8).to_dense(), x)
+    assert _err(lambda: c.save(str(tmp_path / "copy.iarr"))) == ia.ERR_INVALID_STORAGE
+    # the other constructors and a second container at the same path
+    r = ia.Container.arange(ctx, np.float32, (50,), (20,), (5,), urlpath=path, contiguous=contiguous)
+    e = ia.Container.load(ctx, path)
+    assert e.shape == (50,) and e.dtype == np.float32 and e.chunks() == r.chunks()
+    assert np.array_equal(orc.Array(e.chunks(), (50,), (20,), (5,), 4).to_dense(), np.arange(50, dtype=np.float32))
+    z = ia.Container.zeros(ctx, np.float64, (30, 30), (10, 10), (5, 5), urlpath=path, contiguous=contiguous)
+    zz = ia.Container.load(ctx, path)
+    assert zz.chunks() == z.chunks() and not orc.Array(zz.chunks(), (30, 30), (10, 10), (5, 5), 8).to_dense().any()
+    ia.Container.empty(ctx, np.float64, (30, 30), (10, 10), (5, 5), urlpath=path, contiguous=contiguous)
+    assert ia.Container.load(ctx, path).shape == (30, 30)
+    ia.remove(path)
+    assert not os.path.exists(path)
 
 
 def test_arange_linspace_values(orc):
