@@ -152,6 +152,20 @@ IAMX_API int iamx_gemm_prepare_b(int dtype, int precision, int slices, int64_t N
                                  int64_t ldb);
 IAMX_API int iamx_gemm_rows(int64_t M, const void *A, int64_t lda, void *C, int64_t ldc);
 
+/* The block-pair product of iarray_opt_gemm / _gemm_a / _gemm_b when a
+ * zero block is present (src/linalg/iarray_gemm.c:143-165,
+ * src/linalg/iarray_gemm_a.c:100-175): C block (gr, gc) = sum over K blocks
+ * gk, ascending, of A block (gr, gk) x B block (gk, gc), skipping the pairs
+ * with d_zero_a[gr * nbk + gk] or d_zero_b[gk * nbc + gc] set, so Inf/NaN
+ * facing a zero block never reaches C.  d_rows / d_ks / d_cols hold [first,
+ * end) of every block row, K block and block column (int32 pairs, device
+ * memory); only block rows [br0, br1) are computed; max_rows / max_cols bound
+ * one block's extent.  A, B, C dense row-major. */
+IAMX_API int iamx_gemm_blockpairs(int dtype, const void *A, int64_t lda, const void *B, int64_t ldb, void *C,
+                                  int64_t ldc, const int32_t *d_rows, int nbr, const int32_t *d_ks, int nbk,
+                                  const int32_t *d_cols, int nbc, const uint8_t *d_zero_a, const uint8_t *d_zero_b,
+                                  int br0, int br1, int max_rows, int max_cols);
+
 /* Replaces the per-C-block cblas_?gemv of the gemv branch
  * (src/linalg/matmul/gemv.c:92-98) and of iarray_opt_gemv
  * (src/linalg/iarray_gemv.c:212-224): y(M) = A(MxK) x(K). */
@@ -164,11 +178,20 @@ IAMX_API int iamx_gemv(int dtype, int64_t M, int64_t K, const void *A, int64_t l
  * `partial` (nbc x M) and are added in block-column order into y.
  * d_blk_rc holds 4 int32 per block job: first row and first column of the
  * tile in A, the tile's column index in the global block grid (partial row),
- * and 0; nbc is the number of block columns; bs0/bs1 are the block shape
+ * and flags (bit 0: a zero block, used by iamx_opt_gemv_fused only); nbc is the number of block columns; bs0/bs1 are the block shape
  * (bs1 <= 1024). */
 IAMX_API int iamx_gemv_fused(int dtype, const iamx_lz4_job_t *d_lz4, int n_lz4, const iamx_block_job_t *d_blocks,
                              const int32_t *d_blk_rc, int n_blocks, int nbc, int bs0, int bs1, const void *x,
                              void *y, int64_t M, void *d_scratch, void *d_partial);
+
+/* iamx_gemv_fused for iarray_opt_gemv, which skips zero blocks
+ * (src/linalg/iarray_gemv.c:125-160): a block job whose d_blk_rc flag word
+ * has bit 0 set (a zero block of A), or whose block column c has d_xzero[c]
+ * set (a zero chunk or block of x), contributes nothing, so an Inf or NaN
+ * facing a zero block never reaches y.  d_partial must start at zero. */
+IAMX_API int iamx_opt_gemv_fused(int dtype, const iamx_lz4_job_t *d_lz4, int n_lz4, const iamx_block_job_t *d_blocks,
+                                 const int32_t *d_blk_rc, int n_blocks, int nbc, int bs0, int bs1, const void *x,
+                                 void *y, int64_t M, void *d_scratch, void *d_partial, const uint8_t *d_xzero);
 
 /* iamx_gemv_fused without any allocation: the LZ4 scratch and the partial
  * sums live in a fixed device array of iamx_gemv_fused_noalloc_limit() bytes

@@ -77,7 +77,7 @@ typedef struct iarray_device_plan_s {
   const void *lz4;       /* iamx_lz4_job_t[n_lz4], device memory */
   int n_lz4;
   const void *blocks;    /* iamx_block_job_t[n_blocks], device memory */
-  const int32_t *blk_rc; /* 4 int32 per block job (first row, first column, block column, 0), device memory */
+  const int32_t *blk_rc; /* 4 int32 per block job (first row, first column, block column, flags: bit 0 a zero block), device memory */
   int n_blocks;
   int nbcols;            /* block columns of the global block grid */
   int64_t scratch_bytes; /* decoded-stream scratch the LZ4 jobs need */

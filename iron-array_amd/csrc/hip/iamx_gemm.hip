@@ -177,9 +177,101 @@ __global__ __launch_bounds__(256) void k_gemv(int64_t M, int64_t K, const T *__r
   if (lane == 0) y[row] = s;
 }
 
+// ---- block-pair GEMM of the opt paths ------------------------------------------------------
+// The reference's opt_gemm / opt_gemm_a / opt_gemm_b compute each C block as
+// the sum, over the K blocks in ascending order, of A block x B block, and
+// skip a pair when either block is a zero block (src/linalg/iarray_gemm.c:
+// 143-165, iarray_gemm_a.c / _b.c:100-175): an Inf or NaN facing a zero
+// block never reaches C.  Here each workgroup computes a 64 x 64 piece of one
+// C block (block row gr, block column gc) from the dense operand images,
+// skipping the flagged pairs; rows/ks/cols give each block row, K block and
+// block column as [first, end) (trimmed at chunk and array edges).
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm_blockpairs(const T *__restrict__ A, int64_t lda,
+                                                         const T *__restrict__ B, int64_t ldb, T *__restrict__ C,
+                                                         int64_t ldc, const int32_t *__restrict__ rows,
+                                                         const int32_t *__restrict__ ks, int nbk,
+                                                         const int32_t *__restrict__ cols, int nbc,
+                                                         const uint8_t *__restrict__ zero_a,
+                                                         const uint8_t *__restrict__ zero_b, int br0, int sub_r,
+                                                         int sub_c) {
+  __shared__ T As[16][64 + 1];
+  __shared__ T Bs[16][64];
+  const int gc = (int)blockIdx.x / sub_c, sc = (int)blockIdx.x % sub_c;
+  const int gr = br0 + (int)blockIdx.y / sub_r, sr = (int)blockIdx.y % sub_r;
+  const int r0 = rows[2 * gr] + 64 * sr, r1 = min(r0 + 64, rows[2 * gr + 1]);
+  const int c0 = cols[2 * gc] + 64 * sc, c1 = min(c0 + 64, cols[2 * gc + 1]);
+  if (r0 >= r1 || c0 >= c1) return;
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  T acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = (T)0;
+  for (int gk = 0; gk < nbk; gk++) {
+    if (zero_a[(int64_t)gr * nbk + gk] | zero_b[(int64_t)gk * nbc + gc]) continue;  // uniform
+    const int k1 = ks[2 * gk + 1];
+    for (int kk = ks[2 * gk]; kk < k1; kk += 16) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int e = tid + 256 * q;  // 1024 = 64 x 16
+        const int ar = e >> 4, ak = e & 15;
+        As[ak][ar] = (r0 + ar < r1 && kk + ak < k1) ? A[(int64_t)(r0 + ar) * lda + kk + ak] : (T)0;
+        const int bk = e >> 6, bc = e & 63;
+        Bs[bk][bc] = (kk + bk < k1 && c0 + bc < c1) ? B[(int64_t)(kk + bk) * ldb + c0 + bc] : (T)0;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 16; k++) {
+        T a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) a[i] = As[k][ty * 4 + i];
+#pragma unroll
+        for (int j = 0; j < 4; j++) b[j] = Bs[k][tx * 4 + j];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int j = 0; j < 4; j++) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+      }
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int r = r0 + ty * 4 + i, c = c0 + tx * 4 + j;
+      if (r < r1 && c < c1) C[(int64_t)r * ldc + c] = acc[i][j];
+    }
+}
+
 }  // namespace
 
 extern "C" {
+
+IAMX_API int iamx_gemm_blockpairs(int dtype, const void *A, int64_t lda, const void *B, int64_t ldb, void *C,
+                                  int64_t ldc, const int32_t *d_rows, int nbr, const int32_t *d_ks, int nbk,
+                                  const int32_t *d_cols, int nbc, const uint8_t *d_zero_a, const uint8_t *d_zero_b,
+                                  int br0, int br1, int max_rows, int max_cols) {
+  int rc = iamx::ensure_init();
+  if (rc) return rc;
+  if (br0 < 0 || br1 > nbr || br0 > br1 || nbk < 0 || nbc <= 0 || max_rows <= 0 || max_cols <= 0)
+    return iamx::set_error(IAMX_EINVAL, "iamx_gemm_blockpairs: bad block ranges");
+  if (br0 == br1) return IAMX_OK;
+  const int sub_r = (max_rows + 63) / 64, sub_c = (max_cols + 63) / 64;
+  const dim3 grid((unsigned)(nbc * sub_c), (unsigned)((br1 - br0) * sub_r));
+  if (dtype == 0)
+    IAMX_LAUNCH("gemm_blockpairs_f64", k_gemm_blockpairs<double>, grid, dim3(256), 0, (const double *)A, lda,
+                (const double *)B, ldb, (double *)C, ldc, d_rows, d_ks, nbk, d_cols, nbc, d_zero_a, d_zero_b, br0,
+                sub_r, sub_c);
+  else if (dtype == 1)
+    IAMX_LAUNCH("gemm_blockpairs_f32", k_gemm_blockpairs<float>, grid, dim3(256), 0, (const float *)A, lda,
+                (const float *)B, ldb, (float *)C, ldc, d_rows, d_ks, nbk, d_cols, nbc, d_zero_a, d_zero_b, br0,
+                sub_r, sub_c);
+  else
+    return iamx::set_error(IAMX_EINVAL, "iamx_gemm_blockpairs: dtype %d", dtype);
+  return IAMX_OK;
+}
 
 int iamx_gemm_split_bf16(int64_t M, int64_t N, int64_t K, const float *A, int64_t lda, const float *B, int64_t ldb,
                          float *C, int64_t ldc);

@@ -232,7 +232,7 @@ __global__ __launch_bounds__(256) void k_gemv_fused(const iamx_block_job_t *__re
                                                     const int32_t *__restrict__ blk_rc, int njobs,
                                                     const T *__restrict__ x, const uint8_t *__restrict__ scratch,
                                                     T *__restrict__ partial, int64_t M, int64_t K, int nbc,
-                                                    int64_t cap, int *status) {
+                                                    int64_t cap, int *status, const uint8_t *__restrict__ xzero) {
   __shared__ iamx_block_job_t J;
   __shared__ Src S;
   __shared__ T xs[kMaxCols + kMaxCols / 32];
@@ -243,6 +243,9 @@ __global__ __launch_bounds__(256) void k_gemv_fused(const iamx_block_job_t *__re
   const int tid = threadIdx.x;
   const int64_t row0 = blk_rc[4 * b], col0 = blk_rc[4 * b + 1];
   const int bcol = blk_rc[4 * b + 2];
+  // opt_gemv (xzero given) skips a zero block of A or of x: its partial sums
+  // stay at the zero they start from
+  if (xzero && bcol >= 0 && bcol < nbc && ((blk_rc[4 * b + 3] & 1) | xzero[bcol])) return;
   if (tid == 0) {
     J = jobs[b];
     // every range the job names must lie inside the buffers of this call:
@@ -417,7 +420,7 @@ __global__ __launch_bounds__(256) void k_zero_words(uint32_t *__restrict__ p, in
 // check every job against them (IAMX_STATUS_RANGE).
 int gemv_fused_launch(int dtype, const iamx_lz4_job_t *d_lz4, int n_lz4, const iamx_block_job_t *d_blocks,
                       const int32_t *d_blk_rc, int n_blocks, int nbc, const void *x, void *y, int64_t M, int64_t K,
-                      int64_t cap, void *d_scratch, void *d_partial) {
+                      int64_t cap, void *d_scratch, void *d_partial, const uint8_t *d_xzero = nullptr) {
   if (n_lz4 > 0) {
     const int rc = iamx::launch_lz4("lz4_streams_gemv", d_lz4, n_lz4, d_scratch, cap);
     if (rc) return rc;
@@ -426,12 +429,12 @@ int gemv_fused_launch(int dtype, const iamx_lz4_job_t *d_lz4, int n_lz4, const i
   int *st = iamx::status_word();
   if (dtype == 0) {
     IAMX_LAUNCH("gemv_fused_f64", k_gemv_fused<double>, dim3(n_blocks), dim3(256), 0, d_blocks, d_blk_rc, n_blocks,
-                (const double *)x, (const uint8_t *)d_scratch, (double *)d_partial, M, K, nbc, cap, st);
+                (const double *)x, (const uint8_t *)d_scratch, (double *)d_partial, M, K, nbc, cap, st, d_xzero);
     IAMX_LAUNCH("gemv_reduce_f64", k_reduce_cols<double>, rgrid, dim3(256), 0, (const double *)d_partial, nbc, M,
                 (double *)y);
   } else if (dtype == 1) {
     IAMX_LAUNCH("gemv_fused_f32", k_gemv_fused<float>, dim3(n_blocks), dim3(256), 0, d_blocks, d_blk_rc, n_blocks,
-                (const float *)x, (const uint8_t *)d_scratch, (float *)d_partial, M, K, nbc, cap, st);
+                (const float *)x, (const uint8_t *)d_scratch, (float *)d_partial, M, K, nbc, cap, st, d_xzero);
     IAMX_LAUNCH("gemv_reduce_f32", k_reduce_cols<float>, rgrid, dim3(256), 0, (const float *)d_partial, nbc, M,
                 (float *)y);
   } else {
@@ -490,6 +493,18 @@ IAMX_API int iamx_gemv_fused(int dtype, const iamx_lz4_job_t *d_lz4, int n_lz4, 
   if (n_blocks <= 0 || nbc <= 0) return iamx::set_error(IAMX_EINVAL, "iamx_gemv_fused: no blocks");
   return gemv_fused_launch(dtype, d_lz4, n_lz4, d_blocks, d_blk_rc, n_blocks, nbc, x, y, M, INT64_MAX, INT64_MAX,
                            d_scratch, d_partial);
+}
+
+IAMX_API int iamx_opt_gemv_fused(int dtype, const iamx_lz4_job_t *d_lz4, int n_lz4, const iamx_block_job_t *d_blocks,
+                                 const int32_t *d_blk_rc, int n_blocks, int nbc, int bs0, int bs1, const void *x,
+                                 void *y, int64_t M, void *d_scratch, void *d_partial, const uint8_t *d_xzero) {
+  int rc = iamx::ensure_init();
+  if (rc) return rc;
+  (void)bs0;
+  if (bs1 > kMaxCols) return iamx::set_error(IAMX_EINVAL, "iamx_opt_gemv_fused: block width %d > %d", bs1, kMaxCols);
+  if (n_blocks <= 0 || nbc <= 0 || !d_xzero) return iamx::set_error(IAMX_EINVAL, "iamx_opt_gemv_fused: no blocks");
+  return gemv_fused_launch(dtype, d_lz4, n_lz4, d_blocks, d_blk_rc, n_blocks, nbc, x, y, M, INT64_MAX, INT64_MAX,
+                           d_scratch, d_partial, d_xzero);
 }
 
 IAMX_API int64_t iamx_gemv_fused_noalloc_limit(void) { return (int64_t)kNoAllocBytes; }

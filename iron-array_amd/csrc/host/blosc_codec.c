@@ -71,6 +71,34 @@ int iax_parse_header(const uint8_t *c, int32_t csize, iax_header_t *h) {
   return 0;
 }
 
+/* The reference's zero tests (src/linalg/iarray_gemm.c:17-77): a chunk is
+ * zeros when it is a special chunk of kind "zeros"; a block is zeros when
+ * every one of its streams has csize 0 (memcpyed and special chunks have no
+ * such blocks).  The stream count follows the don't-split bit (0x10); the
+ * reference reads bit 0x04, which extended headers always set, see the
+ * known divergences. */
+int iax_chunk_is_zeros(const uint8_t *c, int32_t csize) {
+  iax_header_t h;
+  if (iax_parse_header(c, csize, &h) != 0) return 0;
+  return h.special == 1;
+}
+
+int iax_block_is_zeros(const uint8_t *c, int32_t csize, int nb) {
+  iax_header_t h;
+  if (iax_parse_header(c, csize, &h) != 0 || h.memcpyed || h.special) return 0;
+  if (h.blosc2_flags & 0x08) return 0; /* lazy chunks hold no block data */
+  if (nb < 0 || nb >= h.nblocks) return 0;
+  const int leftover = (nb == h.nblocks - 1) && h.leftover > 0;
+  const int nstreams = (!h.dont_split && !leftover) ? h.typesize : 1;
+  int32_t p = (int32_t)rd32(c + h.header_len + 4 * nb);
+  for (int s = 0; s < nstreams; s++) {
+    if (p < h.header_len || p > csize - 4) return 0;
+    if ((int32_t)rd32(c + p) != 0) return 0;
+    p += 4;
+  }
+  return 1;
+}
+
 /* ---- filters ------------------------------------------------------------------ */
 void iax_shuffle(int ts, int32_t bsize, const uint8_t *src, uint8_t *dst) {
   const int32_t n = bsize / ts;

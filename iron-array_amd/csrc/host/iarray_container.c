@@ -352,7 +352,7 @@ static int push_lz4(jobs_t *j, iamx_lz4_job_t v) {
   return 0;
 }
 
-static int push_blk(jobs_t *j, const iamx_block_job_t *v, int32_t row0, int32_t col0, int32_t bcol) {
+static int push_blk(jobs_t *j, const iamx_block_job_t *v, int32_t row0, int32_t col0, int32_t bcol, int32_t flags) {
   if (j->n_blk == j->cap_blk) {
     j->cap_blk = j->cap_blk ? 2 * j->cap_blk : 1024;
     void *p = realloc(j->blk, (size_t)j->cap_blk * sizeof(*v));
@@ -364,7 +364,7 @@ static int push_blk(jobs_t *j, const iamx_block_job_t *v, int32_t row0, int32_t 
   j->rc4[4 * j->n_blk] = row0;
   j->rc4[4 * j->n_blk + 1] = col0;
   j->rc4[4 * j->n_blk + 2] = bcol;
-  j->rc4[4 * j->n_blk + 3] = 0;
+  j->rc4[4 * j->n_blk + 3] = flags;
   j->blk[j->n_blk++] = *v;
   return 0;
 }
@@ -551,7 +551,10 @@ ina_rc_t iax_ensure_plan(iarray_container_t *c) {
         J.vcols = (int32_t)vc;
         J.dst_off = row0 * ld + col0;
         J.dst_ld = ld;
-        if (push_blk(&jobs, &J, (int32_t)row0, (int32_t)col0, (int32_t)(ci1 * c->nblocks_ax[1] + bi1)))
+        /* bit 0: a zero block by the reference's test (special zero chunk, or
+         * every stream of the block empty), skipped by opt_gemv */
+        const int32_t zflag = ch && (iax_chunk_is_zeros(ch, c->csizes[nc]) || iax_block_is_zeros(ch, c->csizes[nc], nb));
+        if (push_blk(&jobs, &J, (int32_t)row0, (int32_t)col0, (int32_t)(ci1 * c->nblocks_ax[1] + bi1), zflag))
           rc = INA_ERROR(INA_ERR_OUT_OF_MEMORY);
       }
   }
@@ -1421,7 +1424,7 @@ INA_API(ina_rc_t) iarray_decode_chunks_flat(iarray_context_t *ctx, int64_t nchun
       J.vcols = n;
       J.dst_off = elem0 + (int64_t)nb * (h.blocksize / h.typesize);
       J.dst_ld = n;
-      if (push_blk(&jobs, &J, 0, 0, 0)) rc = INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+      if (push_blk(&jobs, &J, 0, 0, 0, 0)) rc = INA_ERROR(INA_ERR_OUT_OF_MEMORY);
     }
     elem0 += h.nbytes / h.typesize;
   }

@@ -154,8 +154,135 @@ static int64_t panel_rows(int64_t M, int64_t N) {
 /* C = A B with A, B (views allowed) already validated; c allocated.  C is
  * computed in row panels, each packed and downloaded as soon as it is done,
  * so the host LZ4 encode of a panel overlaps the GEMM of the next. */
+/* ---- zero-block skipping of the opt paths ----------------------------------------------------
+ * opt: 0 a plain matmul (nothing skipped, as gemm_blosc / gemv_blosc), 1
+ * iarray_opt_gemm, 2 iarray_opt_gemm_a, 3 iarray_opt_gemm_b, 4
+ * iarray_opt_gemv.  A pair (A block, B block) is skipped when either is a
+ * zero block by the reference's block test, or lies in a zero chunk of the
+ * operand whose chunks the variant tests: both for opt_gemm
+ * (src/linalg/iarray_gemm.c:143-165), B's for opt_gemm_a and A's for
+ * opt_gemm_b (iarray_gemm_a.c / _b.c:100-175), A's and x's for opt_gemv
+ * (iarray_gemv.c:125-160). */
+enum { OPT_NONE = 0, OPT_GEMM = 1, OPT_GEMM_A = 2, OPT_GEMM_B = 3, OPT_GEMV = 4 };
+
+/* [first, end) of every block along one axis of a container's tiling,
+ * trimmed at chunk and array edges (all-padding blocks are empty). */
+static int32_t *axis_blocks(const iarray_container_t *c, int ax, int64_t *n) {
+  const int64_t nc = c->nchunks_ax[ax], nb = c->nblocks_ax[ax];
+  int32_t *r = (int32_t *)malloc(sizeof(int32_t) * 2 * (size_t)(nc * nb > 0 ? nc * nb : 1));
+  if (!r) return NULL;
+  for (int64_t ci = 0; ci < nc; ci++)
+    for (int64_t bi = 0; bi < nb; bi++) {
+      const int64_t first = ci * c->chunk[ax] + bi * c->block[ax];
+      int64_t end = first + c->block[ax];
+      if (end > ci * c->chunk[ax] + c->chunk[ax]) end = ci * c->chunk[ax] + c->chunk[ax];
+      if (end > c->shape[ax]) end = c->shape[ax];
+      if (end < first) end = first;
+      r[2 * (ci * nb + bi)] = (int32_t)first;
+      r[2 * (ci * nb + bi) + 1] = (int32_t)end;
+    }
+  *n = nc * nb;
+  return r;
+}
+
+/* zero flag of block (g0, g1) of the global block grid of c */
+static uint8_t zero_block(const iarray_container_t *c, int64_t g0, int64_t g1, int chunk_rule) {
+  const int64_t ci0 = g0 / c->nblocks_ax[0], bi0 = g0 % c->nblocks_ax[0];
+  const int64_t ci1 = g1 / c->nblocks_ax[1], bi1 = g1 % c->nblocks_ax[1];
+  const int64_t nc = ci0 * c->nchunks_ax[1] + ci1;
+  const uint8_t *ch = c->chunks[nc];
+  if (!ch) return 0;
+  if (chunk_rule && iax_chunk_is_zeros(ch, c->csizes[nc])) return 1;
+  return (uint8_t)iax_block_is_zeros(ch, c->csizes[nc], (int)(bi0 * c->nblocks_ax[1] + bi1));
+}
+
+typedef struct {
+  int dt;
+  const uint8_t *A, *B;
+  uint8_t *C;
+  int64_t K, N;
+  int32_t *rows; /* host copy, for the block-row range of a panel */
+  int64_t nbr, nbk, nbc;
+  const int32_t *d_rows, *d_ks, *d_cols;
+  const uint8_t *d_za, *d_zb;
+  int max_rows, max_cols;
+} bp_ctx_t;
+
+static int blockpair_rows(void *v, int64_t row0, int64_t nrows) {
+  const bp_ctx_t *x = (const bp_ctx_t *)v;
+  int64_t br0 = 0, br1;
+  while (br0 < x->nbr && x->rows[2 * br0] < row0) br0++;
+  br1 = br0;
+  while (br1 < x->nbr && x->rows[2 * br1] < row0 + nrows) br1++;
+  return iamx_gemm_blockpairs(x->dt, x->A, x->K, x->B, x->N, x->C, x->N, x->d_rows, (int)x->nbr, x->d_ks,
+                              (int)x->nbk, x->d_cols, (int)x->nbc, x->d_za, x->d_zb, (int)br0, (int)br1, x->max_rows,
+                              x->max_cols);
+}
+
+/* Block-pair product when any pair is skipped; *used = 0 (and nothing
+ * launched) when no zero block exists, so the plain GEMM gives the same C. */
+static ina_rc_t run_blockpairs(iarray_container_t *a, iarray_container_t *b, iarray_container_t *c, int opt,
+                               const void *dA, const void *dB, void *dC, int *used) {
+  *used = 0;
+  int64_t nbr, nbk, nbk2, nbc;
+  int32_t *rows = axis_blocks(a, 0, &nbr), *ks = axis_blocks(a, 1, &nbk), *cols = axis_blocks(b, 1, &nbc);
+  int32_t *ks2 = axis_blocks(b, 0, &nbk2);
+  ina_rc_t rc = INA_SUCCESS;
+  uint8_t *za = NULL, *zb = NULL;
+  if (!rows || !ks || !cols || !ks2) rc = INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+  else if (nbk2 != nbk || memcmp(ks, ks2, sizeof(int32_t) * 2 * (size_t)nbk) != 0)
+    rc = iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_BLOCKSHAPE), "A's column blocks differ from B's row blocks");
+  if (INA_SUCCEED(rc)) {
+    za = (uint8_t *)calloc((size_t)(nbr * nbk > 0 ? nbr * nbk : 1), 1);
+    zb = (uint8_t *)calloc((size_t)(nbk * nbc > 0 ? nbk * nbc : 1), 1);
+    if (!za || !zb) rc = INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+  }
+  int any = 0;
+  if (INA_SUCCEED(rc)) {
+    const int rule_a = opt == OPT_GEMM || opt == OPT_GEMM_B, rule_b = opt == OPT_GEMM || opt == OPT_GEMM_A;
+    for (int64_t i = 0; i < nbr; i++)
+      for (int64_t k = 0; k < nbk; k++) any |= za[i * nbk + k] = zero_block(a, i, k, rule_a);
+    for (int64_t k = 0; k < nbk; k++)
+      for (int64_t j = 0; j < nbc; j++) any |= zb[k * nbc + j] = zero_block(b, k, j, rule_b);
+  }
+  if (INA_SUCCEED(rc) && any) {
+    const size_t o_ks = (size_t)(2 * nbr) * 4, o_cols = o_ks + (size_t)(2 * nbk) * 4,
+                 o_za = o_cols + (size_t)(2 * nbc) * 4, o_zb = o_za + (size_t)(nbr * nbk),
+                 total = o_zb + (size_t)(nbk * nbc);
+    uint8_t *h = (uint8_t *)malloc(total);
+    uint8_t *d = (uint8_t *)iamx_workspace(15, total + 16);
+    if (!h) rc = INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+    else if (!d) rc = iax_dev_error(IAMX_ENOMEM, "block-pair tables");
+    if (INA_SUCCEED(rc)) {
+      memcpy(h, rows, o_ks);
+      memcpy(h + o_ks, ks, o_cols - o_ks);
+      memcpy(h + o_cols, cols, o_za - o_cols);
+      memcpy(h + o_za, za, o_zb - o_za);
+      memcpy(h + o_zb, zb, total - o_zb);
+      int e = iamx_h2d(d, h, total);
+      if (e) rc = iax_dev_error(e, "block-pair tables");
+    }
+    free(h);
+    if (INA_SUCCEED(rc)) {
+      bp_ctx_t x = {dtype_code(a->dtshape.dtype), (const uint8_t *)dA, (const uint8_t *)dB, (uint8_t *)dC,
+                    a->shape[1], b->shape[1], rows, nbr, nbk, nbc, (const int32_t *)d,
+                    (const int32_t *)(d + o_ks), (const int32_t *)(d + o_cols), d + o_za, d + o_zb,
+                    (int)a->block[0], (int)b->block[1]};
+      *used = 1;
+      rc = iax_write_from_dense_rows(c, dC, panel_rows(a->shape[0], b->shape[1]), blockpair_rows, &x);
+    }
+  }
+  free(rows);
+  free(ks);
+  free(ks2);
+  free(cols);
+  free(za);
+  free(zb);
+  return rc;
+}
+
 static ina_rc_t run_gemm(iarray_container_t *a, iarray_container_t *b, iarray_container_t *c,
-                         const iarray_matmul_params_t *params) {
+                         const iarray_matmul_params_t *params, int opt) {
   const int dt = dtype_code(a->dtshape.dtype);
   const int64_t M = a->dtshape.shape[0], K = a->dtshape.shape[1], N = b->dtshape.shape[1];
   int prec = params ? (int)params->precision : 0;
@@ -171,6 +298,12 @@ static ina_rc_t run_gemm(iarray_container_t *a, iarray_container_t *b, iarray_co
   if (INA_FAILED(rc)) return rc;
   void *dC = iamx_workspace(4, (size_t)(M * N * a->itemsize) + 16);
   if (!dC) return iax_dev_error(IAMX_ENOMEM, "C buffer");
+  if (opt != OPT_NONE && prec == IARRAY_MATMUL_PLAIN) {
+    int used = 0;
+    rc = run_blockpairs(a, b, c, opt, dA, dB, dC, &used);
+    if (INA_FAILED(rc)) return rc;
+    if (used) return check_status();
+  }
   if (prec == IARRAY_MATMUL_OZAKI_INT8 || prec == IARRAY_MATMUL_OZAKI_CRT) {
     iax_last_slice_err = -1.0;
     if (slices == 0) {
@@ -207,7 +340,7 @@ static ina_rc_t run_gemm(iarray_container_t *a, iarray_container_t *b, iarray_co
 }
 
 /* y = A x ; fused block decode + GEMV when A is a plain container. */
-static ina_rc_t run_gemv(iarray_container_t *a, iarray_container_t *x, iarray_container_t *y) {
+static ina_rc_t run_gemv(iarray_container_t *a, iarray_container_t *x, iarray_container_t *y, int opt) {
   const int dt = dtype_code(a->dtshape.dtype);
   const int64_t M = a->dtshape.shape[0], K = a->dtshape.shape[1];
   void *dx = NULL;
@@ -226,7 +359,28 @@ static ina_rc_t run_gemv(iarray_container_t *a, iarray_container_t *x, iarray_co
     void *partial = iamx_workspace(6, pbytes + 16);
     if (!scratch || !partial) return iax_dev_error(IAMX_ENOMEM, "gemv scratch");
     e = iamx_memset(partial, 0, pbytes);
-    if (!e && p->n_blocks > 0)
+    uint8_t *d_xzero = NULL;
+    if (!e && opt == OPT_GEMV && p->n_blocks > 0) {
+      /* zero flags of x's blocks (its chunk test and block test), one per
+       * block column of A: the two tilings agree along K */
+      const int64_t nbx = x->nchunks_ax[1] * x->nblocks_ax[1];
+      if (nbx != p->nbcols)
+        return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_BLOCKSHAPE), "x's blocks differ from A's block columns");
+      uint8_t *hz = (uint8_t *)malloc((size_t)nbx);
+      d_xzero = (uint8_t *)iamx_workspace(15, (size_t)nbx + 16);
+      if (!hz) return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+      if (!d_xzero) {
+        free(hz);
+        return iax_dev_error(IAMX_ENOMEM, "x zero flags");
+      }
+      for (int64_t g = 0; g < nbx; g++) hz[g] = zero_block(x, 0, g, 1);
+      e = iamx_h2d(d_xzero, hz, (size_t)nbx);
+      free(hz);
+    }
+    if (!e && p->n_blocks > 0 && d_xzero)
+      e = iamx_opt_gemv_fused(dt, p->d_lz4, p->n_lz4, p->d_blocks, p->d_blk_rc, p->n_blocks, (int)p->nbcols,
+                              (int)a->block[0], (int)a->block[1], dx, dy, M, scratch, partial, d_xzero);
+    else if (!e && p->n_blocks > 0)
       e = iamx_gemv_fused(dt, p->d_lz4, p->n_lz4, p->d_blocks, p->d_blk_rc, p->n_blocks, (int)p->nbcols,
                           (int)a->block[0], (int)a->block[1], dx, dy, M, scratch, partial);
     else if (!e)
@@ -258,7 +412,7 @@ static ina_rc_t new_output(iarray_context_t *ctx, iarray_container_t *a, iarray_
 }
 
 static ina_rc_t matmul_impl(iarray_context_t *ctx, iarray_container_t *a, iarray_container_t *b,
-                            iarray_storage_t *storage, const iarray_matmul_params_t *params,
+                            iarray_storage_t *storage, const iarray_matmul_params_t *params, int opt,
                             iarray_container_t **c) {
   if (!ctx || !a || !b || !storage || !c) return INA_ERROR(INA_ERR_NULL_POINTER);
   if (a->dtshape.ndim != 2 || (b->dtshape.ndim != 1 && b->dtshape.ndim != 2))
@@ -271,7 +425,7 @@ static ina_rc_t matmul_impl(iarray_context_t *ctx, iarray_container_t *a, iarray
   ina_rc_t rc = new_output(ctx, a, b, storage, c);
   if (INA_FAILED(rc)) return rc;
   iamx_prof_reset();
-  rc = b->dtshape.ndim == 2 ? run_gemm(a, b, *c, params) : run_gemv(a, b, *c);
+  rc = b->dtshape.ndim == 2 ? run_gemm(a, b, *c, params, opt) : run_gemv(a, b, *c, opt);
   iax_last_device_ms = iamx_prof_span_ms();
   if (INA_SUCCEED(rc)) rc = iax_persist(*c);
   /* a failed product leaves no partly written output behind */
@@ -281,13 +435,13 @@ static ina_rc_t matmul_impl(iarray_context_t *ctx, iarray_container_t *a, iarray
 
 INA_API(ina_rc_t) iarray_linalg_matmul(iarray_context_t *ctx, iarray_container_t *a, iarray_container_t *b,
                                        iarray_storage_t *storage, iarray_container_t **c) {
-  return matmul_impl(ctx, a, b, storage, NULL, c);
+  return matmul_impl(ctx, a, b, storage, NULL, OPT_NONE, c);
 }
 
 INA_API(ina_rc_t) iarray_linalg_matmul_params(iarray_context_t *ctx, iarray_container_t *a, iarray_container_t *b,
                                               iarray_storage_t *storage, const iarray_matmul_params_t *params,
                                               iarray_container_t **c) {
-  return matmul_impl(ctx, a, b, storage, params, c);
+  return matmul_impl(ctx, a, b, storage, params, OPT_NONE, c);
 }
 
 INA_API(ina_rc_t) iarray_opt_gemm(iarray_context_t *ctx, iarray_container_t *a, iarray_container_t *b,
@@ -308,7 +462,7 @@ INA_API(ina_rc_t) iarray_opt_gemm(iarray_context_t *ctx, iarray_container_t *a, 
     return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_BLOCKSHAPE), "a->blockshape[1] != b->blockshape[0]");
   if (b->block[1] != storage->blockshape[1])
     return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_BLOCKSHAPE), "b->blockshape[1] != c->blockshape[1]");
-  return matmul_impl(ctx, a, b, storage, NULL, c);
+  return matmul_impl(ctx, a, b, storage, NULL, OPT_GEMM, c);
 }
 
 #define PART_ERR(code, msg) return iax_set_error(INA_ERROR(code), msg)
@@ -350,7 +504,7 @@ INA_API(ina_rc_t) iarray_opt_gemm_a(iarray_context_t *ctx, iarray_container_t *a
   if (b->shape[1] > storage->chunkshape[1]) PART_ERR(IARRAY_ERR_INVALID_CHUNKSHAPE, "b->shape[1] > c->chunkshape[1]");
   rc = opt_ab_blocks(a, b, storage);
   if (INA_FAILED(rc)) return rc;
-  return matmul_impl(ctx, a, b, storage, NULL, c);
+  return matmul_impl(ctx, a, b, storage, NULL, OPT_GEMM_A, c);
 }
 
 INA_API(ina_rc_t) iarray_opt_gemm_b(iarray_context_t *ctx, iarray_container_t *a, iarray_container_t *b,
@@ -362,7 +516,7 @@ INA_API(ina_rc_t) iarray_opt_gemm_b(iarray_context_t *ctx, iarray_container_t *a
   if (b->chunk[1] != storage->chunkshape[1]) PART_ERR(IARRAY_ERR_INVALID_CHUNKSHAPE, "b->chunkshape[1] != c->chunkshape[1]");
   rc = opt_ab_blocks(a, b, storage);
   if (INA_FAILED(rc)) return rc;
-  return matmul_impl(ctx, a, b, storage, NULL, c);
+  return matmul_impl(ctx, a, b, storage, NULL, OPT_GEMM_B, c);
 }
 
 INA_API(ina_rc_t) iarray_opt_gemv(iarray_context_t *ctx, iarray_container_t *a, iarray_container_t *b,
@@ -379,5 +533,5 @@ INA_API(ina_rc_t) iarray_opt_gemv(iarray_context_t *ctx, iarray_container_t *a, 
     return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_BLOCKSHAPE), "a->blockshape[0] != c->blockshape[0]");
   if (a->block[1] != b->block[1])
     return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_BLOCKSHAPE), "a->blockshape[1] != b->blockshape[0]");
-  return matmul_impl(ctx, a, b, storage, NULL, c);
+  return matmul_impl(ctx, a, b, storage, NULL, OPT_GEMV, c);
 }

@@ -111,6 +111,8 @@ typedef struct {
 } iax_header_t;
 
 int iax_parse_header(const uint8_t *chunk, int32_t csize, iax_header_t *h);
+int iax_chunk_is_zeros(const uint8_t *chunk, int32_t csize);
+int iax_block_is_zeros(const uint8_t *chunk, int32_t csize, int nb);
 int iax_lz4_compress(const uint8_t *src, int n, uint8_t *dst, int cap, int accel, uint32_t *hashtab);
 void iax_shuffle(int ts, int32_t bsize, const uint8_t *src, uint8_t *dst);
 void iax_bitshuffle(int ts, int32_t bsize, const uint8_t *src, uint8_t *dst);

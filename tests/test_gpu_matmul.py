@@ -413,3 +413,98 @@ def test_batch_ragged_matches_per_product(ctx):
         cs = ia.matmul_batch(pairs, cs_list, bs_list)
         for c, want in zip(cs, outs):
             assert np.array_equal(c.to_numpy(), want.astype(np.float32)), f"replay {rep}"
+
+
+# ---- zero-block skipping of the opt paths ------------------------------------------------------
+def _special_zero_chunk(like: bytes) -> bytes:
+    import struct
+    h = bytearray(like[:32])
+    h[2] = 0x05 | 0x10 | (h[2] & 0xE0)
+    h[12:16] = struct.pack("<i", 32)
+    h[16:31] = bytes(15)
+    h[31] = 1 << 4
+    return bytes(h)
+
+
+def _with_zero_chunks(ctx, x, cs, bs, zero_chunks):
+    c = ia.Container.from_numpy(ctx, x, cs, bs)
+    ch = list(c.chunks())
+    for i in zero_chunks:
+        ch[i] = _special_zero_chunk(ch[i])
+    out = ia.Container.from_chunks(ctx, x.dtype, x.shape, cs, bs, ch)
+    out._cs, out._bs = cs, bs
+    return out
+
+
+SKIP_CASES = [
+    # variant, M, K, N, A (chunk, block), B (chunk, block), C (chunk, block)
+    ("gemm", 100, 90, 70, ((40, 30), (20, 10)), ((30, 35), (10, 15)), ((40, 35), (20, 15))),
+    ("gemm", 64, 128, 96, ((32, 64), (16, 32)), ((64, 48), (32, 16)), ((32, 48), (16, 16))),
+    ("a", 70, 45, 50, ((30, 45), (30, 20)), ((45, 20), (20, 20)), ((30, 50), (30, 20))),
+    ("b", 70, 45, 50, ((30, 45), (30, 20)), ((45, 20), (20, 20)), ((70, 20), (30, 20))),
+]
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+@pytest.mark.parametrize("case", SKIP_CASES, ids=[f"{c[0]}-{c[1]}x{c[2]}x{c[3]}" for c in SKIP_CASES])
+def test_opt_gemm_skips_zero_blocks_on_gpu(ctx, orc, case, dt):
+    """iarray_opt_gemm / _a / _b on the GPU skip the same (A block, B block)
+    pairs as the reference (src/linalg/iarray_gemm.c:143-165,
+    iarray_gemm_a.c / _b.c:100-175): with a special zero chunk and zero
+    blocks planted and Inf wherever the other operand faces them, the result
+    equals the oracle's bit for bit, NaN and Inf included; the plain matmul
+    skips nothing and gets NaN there, as gemm_blosc does."""
+    var, M, K, N, (acs, abs_), (bcs, bbs), (ccs, cbs) = case
+    print(f"case: opt skip {var} {M}x{K}x{N} {np.dtype(dt).name}", flush=True)
+    rng = np.random.default_rng(M + 2 * K + 3 * N)
+    a = rng.integers(-8, 9, (M, K)).astype(dt)
+    b = rng.integers(-8, 9, (K, N)).astype(dt)
+    za, zb = [], []
+    if var in ("gemm", "b"):
+        a[:acs[0], :acs[1]] = 0              # A chunk (0, 0): a special zero chunk
+        za = [0]
+    if var in ("gemm", "a"):
+        b[:bcs[0], :bcs[1]] = 0              # B chunk (0, 0)
+        zb = [0]
+    a[acs[0]:acs[0] + abs_[0], :abs_[1]] = 0  # a zero block (streams all empty) of A's second chunk row
+    b[:bbs[0], bcs[1]:bcs[1] + bbs[1]] = 0    # and of B's second chunk column
+    a[-1, -1] = np.inf
+    b[bbs[0]:2 * bbs[0], :] = np.inf          # B rows that meet A's zero blocks
+    b[:, -1] = -np.inf
+    A = _with_zero_chunks(ctx, a, acs, abs_, za)
+    B = _with_zero_chunks(ctx, b, bcs, bbs, zb)
+    fn = {"gemm": ia.opt_gemm, "a": ia.opt_gemm_a, "b": ia.opt_gemm_b}[var]
+    ofn = {"gemm": orc.opt_gemm, "a": orc.opt_gemm_a, "b": orc.opt_gemm_b}[var]
+    with np.errstate(invalid="ignore"):
+        got = fn(A, B, ccs, cbs).to_numpy()
+        want = ofn(_oracle(orc, A), _oracle(orc, B), ccs, cbs)
+        plain_want = orc.gemm_blosc(_oracle(orc, A), _oracle(orc, B), ccs, cbs)
+    assert np.isnan(plain_want).sum() > np.isnan(want).sum()   # the planted cases do differ
+    assert np.array_equal(got, want, equal_nan=True)
+    plain = ia.matmul(A, B, ccs, cbs).to_numpy()
+    assert np.array_equal(np.isnan(plain), np.isnan(plain_want))
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_opt_gemv_skips_zero_blocks_on_gpu(ctx, orc, dt):
+    """iarray_opt_gemv skips zero chunks and zero blocks of A and of x
+    (src/linalg/iarray_gemv.c:125-160): an Inf in x facing only A's zero
+    blocks, and an Inf in A facing only x's zero block, never reach y."""
+    M, K = 130, 90
+    acs, abs_, xcs, xbs = (64, 45), (16, 15), (45,), (15,)
+    rng = np.random.default_rng(33)
+    a = rng.integers(-8, 9, (M, K)).astype(dt)
+    x = rng.integers(-4, 5, K).astype(dt)
+    a[:acs[0], :acs[1]] = 0                       # A chunk (0, 0): a special zero chunk
+    a[acs[0]:acs[0] + abs_[0], :abs_[1]] = 0      # a zero block of chunk (1, 0)
+    x[:abs_[1]] = np.inf                          # meets only those zeros for rows < 80
+    x[xcs[0] + xbs[0]:xcs[0] + 2 * xbs[0]] = 0    # a zero block of x's second chunk
+    a[70, xcs[0] + xbs[0]] = np.inf               # faces only that zero block (row 70 also meets
+    #                                               x's Inf only through A's zero block)
+    A = _with_zero_chunks(ctx, a, acs, abs_, [0])
+    X = _with_zero_chunks(ctx, x, xcs, xbs, [])
+    with np.errstate(invalid="ignore"):
+        got = ia.opt_gemv(A, X, (64,), (16,)).to_numpy()
+        want = orc.opt_gemv(_oracle(orc, A), _oracle(orc, X), (64,), (16,))
+    assert np.isfinite(want[:acs[0] + abs_[0]]).all() and not np.isfinite(want[acs[0] + abs_[0]:]).all()
+    assert np.array_equal(got, want, equal_nan=True)
