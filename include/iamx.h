@@ -90,15 +90,24 @@ IAMX_API double iamx_prof_span_ms(void); /* first start .. last stop since reset
  * each into scratch, then every block is un-shuffled / bit-un-shuffled and
  * written into its tile of a dense row-major destination (edge-trimmed). */
 typedef struct iamx_lz4_job_s {
-  uint64_t src;    /* device address of the LZ4 payload */
+  uint64_t src;    /* device address of the compressed payload */
   int64_t dst_off; /* byte offset in scratch of the decoded stream */
   int32_t csize;   /* compressed bytes */
   int32_t nbytes;  /* decoded bytes (the stream's neblock) */
+  int32_t codec;   /* IAMX_CODEC_LZ4 (LZ4 and LZ4HC streams) or IAMX_CODEC_BLOSCLZ */
+  int32_t reserved;
 } iamx_lz4_job_t;
+/* The stream formats of a job (a chunk's codec, bits 5-7 of its flags, 1 or 0). */
+enum { IAMX_CODEC_LZ4 = 0, IAMX_CODEC_BLOSCLZ = 1 };
 
 #define IAMX_MAX_STREAMS 8
 enum { IAMX_SRC_CONST = 0, IAMX_SRC_ADDR = 1, IAMX_SRC_SCRATCH = 2 };
 enum { IAMX_FILTER_NONE = 0, IAMX_FILTER_SHUFFLE = 1, IAMX_FILTER_BITSHUFFLE = 2, IAMX_FILTER_BITSHUFFLE_V2 = 3 };
+/* OR'ed into a block job's filter: the tile is written transposed, element
+ * (r, c) of the tile at dst_off + c * dst_ld + r (the decoded image of a
+ * transposed view, written in place of the omatcopy of
+ * src/iarray_container.c:718-723). */
+#define IAMX_JOB_TRANSPOSE 0x100
 
 typedef struct iamx_block_job_s {
   uint64_t src[IAMX_MAX_STREAMS]; /* device address or scratch offset of stream s */
@@ -316,6 +325,8 @@ IAMX_API int iamx_decode_status(void);
 /* Status bit of a job table entry that points outside the buffers it was
  * given (the job is skipped); bits 1..7 are malformed LZ4 streams. */
 #define IAMX_STATUS_RANGE (1 << 8)
+/* Status bit of a malformed BloscLZ stream. */
+#define IAMX_STATUS_BLOSCLZ (1 << 9)
 
 #ifdef __cplusplus
 }

@@ -424,6 +424,116 @@ __device__ __forceinline__ void lz4_stream(const iamx_lz4_job_t job, uint8_t *__
   if (bad && lane == 0) atomicOr(status, 1 << bad);
 }
 
+// One BloscLZ stream by one wavefront (the stream format is restated in the
+// oracle, oracle/blosclz_block.c, from c-blosc's compressor output): a
+// control byte per token, literal runs of up to 32 bytes, matches of 3 or
+// more bytes at distances up to 8191 + 1, or 8192 + 65535 through the
+// far-distance escape.  The parse is uniform (a 256-byte register window of
+// the input, bytes taken with readlane); every output byte is written to the
+// scratch slot and to the LDS history ring.  Matches within RING - 64 bytes
+// copy ring to ring in steps of min(distance, 64) bytes; farther ones read
+// the slot in global memory after a release/acquire fence (again before
+// every step that reads bytes of the same match).
+template <int RING>
+__device__ __forceinline__ void blosclz_stream(const iamx_lz4_job_t job, uint8_t *__restrict__ scratch,
+                                               int *status, uint8_t *ring) {
+  constexpr int32_t RM = RING - 1;
+  const int lane = threadIdx.x;
+  const uint64_t abase = job.src & ~3ull;
+  const IAMX_GLOBAL uint32_t *__restrict__ sw = iamx::gptr<const uint32_t>(abase);
+  const IAMX_GLOBAL uint8_t *__restrict__ sb = iamx::gptr<const uint8_t>(abase);
+  IAMX_GLOBAL uint8_t *dst = iamx::gptr<uint8_t>((uint64_t)(scratch + job.dst_off));
+  int32_t ip = (int32_t)(job.src & 3);
+  const int32_t iend = ip + job.csize, oend = job.nbytes;
+  const int32_t lastw = (iend - 1) >> 2;
+  int32_t wb = 0;
+  uint32_t w = sw[min(lane, lastw)];
+  auto byte_at = [&](int32_t pos) -> uint32_t {  // pos uniform, pos < iend
+    if (pos - wb >= 256) {
+      wb = pos & ~3;
+      w = sw[min((wb >> 2) + lane, lastw)];
+    }
+    const int32_t q = pos - wb;
+    return (rdlane(w, q >> 2) >> ((q & 3) << 3)) & 0xffu;
+  };
+  auto fence = [&]() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  };
+  int32_t op = 0;
+  int bad = 0;
+  uint32_t ctrl = byte_at(ip++) & 31u;
+  for (;;) {
+    if (ctrl >= 32) {
+      int32_t len = (int32_t)(ctrl >> 5) - 1;
+      int32_t dist = (int32_t)(ctrl & 31) << 8;
+      if (len == 6) {
+        uint32_t code;
+        do {
+          if (ip >= iend || len > oend) { bad = 1; break; }
+          code = byte_at(ip++);
+          len += (int32_t)code;
+        } while (code == 255);
+        if (bad) break;
+      }
+      if (ip >= iend) { bad = 1; break; }
+      const uint32_t code = byte_at(ip++);
+      len += 3;
+      if (code == 255 && dist == (31 << 8)) {
+        if (iend - ip < 2) { bad = 1; break; }
+        dist = (int32_t)((byte_at(ip) << 8) | byte_at(ip + 1)) + 8192;
+        ip += 2;
+      } else {
+        dist += (int32_t)code + 1;
+      }
+      if (dist > op || len > oend - op) { bad = 1; break; }
+      if (dist <= RING - 64) {
+        const int32_t step = min(dist, 64);
+        for (int32_t k = 0; k < len; k += step) {
+          const int32_t j = k + lane;
+          const bool on = lane < step && j < len;
+          uint8_t v = 0;
+          if (on) v = ring[(op + j - dist) & RM];
+          __builtin_amdgcn_wave_barrier();
+          if (on) {
+            ring[(op + j) & RM] = v;
+            dst[op + j] = v;
+          }
+          __builtin_amdgcn_wave_barrier();
+        }
+      } else {
+        fence();
+        for (int32_t k = 0; k < len; k += 64) {
+          if (k + 64 > dist) fence();  // this step reads bytes of this match
+          const int32_t j = k + lane;
+          if (j < len) {
+            const uint8_t v = dst[op + j - dist];
+            ring[(op + j) & RM] = v;
+            dst[op + j] = v;
+          }
+          __builtin_amdgcn_wave_barrier();
+        }
+      }
+      op += len;
+    } else {
+      const int32_t len = (int32_t)ctrl + 1;
+      if (len > iend - ip || len > oend - op) { bad = 1; break; }
+      if (lane < len) {
+        const uint8_t v = sb[ip + lane];
+        ring[(op + lane) & RM] = v;
+        dst[op + lane] = v;
+      }
+      __builtin_amdgcn_wave_barrier();
+      ip += len;
+      op += len;
+    }
+    if (ip >= iend) break;
+    ctrl = byte_at(ip++);
+  }
+  if (!bad && op != oend) bad = 1;
+  if (bad && lane == 0) atomicOr(status, IAMX_STATUS_BLOSCLZ);
+}
+
 template <int RING>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RING <= 4096 ? 8 : 1, 8))) void k_lz4_streams(
     const iamx_lz4_job_t *__restrict__ jobs, int njobs, uint8_t *__restrict__ scratch, int64_t cap, int *status) {
@@ -434,6 +544,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RING <= 4096
   // the decoded stream, padded to 16 bytes, must lie inside the scratch
   if (job.dst_off < 0 || job.nbytes < 0 || job.dst_off > cap - (((int64_t)job.nbytes + 15) & ~(int64_t)15)) {
     if (threadIdx.x == 0) atomicOr(status, IAMX_STATUS_RANGE);
+    return;
+  }
+  if (job.codec == IAMX_CODEC_BLOSCLZ) {
+    blosclz_stream<RING>(job, scratch, status, ring);
     return;
   }
   lz4_stream<RING>(job, scratch, status, ring);
@@ -488,8 +602,12 @@ __global__ __launch_bounds__(256) void k_unfilter(const iamx_block_job_t *__rest
   const int32_t cols = J.cols;
   const int64_t ld = J.dst_ld;
   const int64_t base = J.dst_off;
-  if (J.filter == IAMX_FILTER_BITSHUFFLE || J.filter == IAMX_FILTER_BITSHUFFLE_V2) {
-    const int32_t n8 = J.filter == IAMX_FILTER_BITSHUFFLE ? n - n % 8 : n;
+  // a transposed destination takes tile element (r, c) at base + c * ld + r
+  const bool tr = (J.filter & IAMX_JOB_TRANSPOSE) != 0;
+  const int64_t sr = tr ? 1 : ld, sc = tr ? ld : 1;
+  const int filter = J.filter & ~IAMX_JOB_TRANSPOSE;
+  if (filter == IAMX_FILTER_BITSHUFFLE || filter == IAMX_FILTER_BITSHUFFLE_V2) {
+    const int32_t n8 = filter == IAMX_FILTER_BITSHUFFLE ? n - n % 8 : n;
     const int32_t rowlen = n8 / 8;
     const int32_t ngroups = rowlen + (n - n8);  // 8-element groups, then tail elements one by one
     for (int32_t g = threadIdx.x; g < ngroups; g += blockDim.x) {
@@ -520,14 +638,14 @@ __global__ __launch_bounds__(256) void k_unfilter(const iamx_block_job_t *__rest
       }
       int32_t r = e0 / cols, c = e0 - r * cols;
       for (int i = 0; i < cnt; i++) {
-        if (r < J.vrows && c < J.vcols) store_elem(dst, ts, base + (int64_t)r * ld + c, elems[i]);
+        if (r < J.vrows && c < J.vcols) store_elem(dst, ts, base + (int64_t)r * sr + (int64_t)c * sc, elems[i]);
         if (++c == cols) { c = 0; ++r; }
       }
     }
     return;
   }
   // byte shuffle (planes) or no filter
-  const bool shuffled = J.filter == IAMX_FILTER_SHUFFLE && ts > 1;
+  const bool shuffled = filter == IAMX_FILTER_SHUFFLE && ts > 1;
   const IAMX_GLOBAL uint8_t *plane[8];
   uint8_t pconst[8], pfill[8];
   if (shuffled) {
@@ -548,7 +666,7 @@ __global__ __launch_bounds__(256) void k_unfilter(const iamx_block_job_t *__rest
       } else {
         for (int j = 0; j < ts; j++) v |= (uint64_t)block_byte(P, J.neblock, e * ts + j) << (8 * j);
       }
-      store_elem(dst, ts, base + (int64_t)r * ld + c, v);
+      store_elem(dst, ts, base + (int64_t)r * sr + (int64_t)c * sc, v);
     }
   }
 }

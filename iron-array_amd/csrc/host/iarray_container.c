@@ -161,6 +161,8 @@ INA_API(void) iarray_container_free(iarray_context_t *ctx, iarray_container_t **
     free(c->csizes);
     free(c->arena_off);
     free(c->storage.urlpath); /* the frame on disk stays */
+  } else {
+    iamx_free(c->d_tblocks);
   }
   free(c);
   *cp = NULL;
@@ -447,8 +449,11 @@ static ina_rc_t block_streams(const uint8_t *ch, int32_t csize, uint64_t dev, co
       p += cs;
     } else {
       if (p + cs > csize) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "truncated LZ4 stream");
-      if (h->compformat != 1) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "codec format %d", h->compformat);
-      iamx_lz4_job_t lj = {dev + (uint64_t)p, jobs->scratch, cs, neblock};
+      /* codec formats 1 (LZ4, LZ4HC) and 0 (BloscLZ) decode on the GPU */
+      if (h->compformat != 1 && h->compformat != 0)
+        return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "codec format %d has no GPU decoder", h->compformat);
+      iamx_lz4_job_t lj = {dev + (uint64_t)p, jobs->scratch, cs, neblock,
+                           h->compformat == 0 ? IAMX_CODEC_BLOSCLZ : IAMX_CODEC_LZ4, 0};
       if (push_lz4(jobs, lj)) return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
       J->kind[s] = IAMX_SRC_SCRATCH;
       J->src[s] = (uint64_t)jobs->scratch;
@@ -615,6 +620,44 @@ void iax_operand_forget(int slot) {
   }
 }
 
+/* Decode the base of transposed view v into the view's row-major image
+ * (base cols x base rows): the base's block jobs with their tiles written
+ * transposed, so no separate transpose pass reads and writes the image. */
+static ina_rc_t decode_transposed(iarray_container_t *v, iarray_container_t *base, void *dst) {
+  ina_rc_t rc = iax_ensure_plan(base);
+  if (INA_FAILED(rc)) return rc;
+  const iax_plan_t *p = &base->plan;
+  if (!v->d_tblocks || v->t_gen != base->generation) {
+    iamx_free(v->d_tblocks);
+    v->d_tblocks = NULL;
+    const int n = p->n_blocks;
+    iamx_block_job_t *h = (iamx_block_job_t *)malloc(sizeof(*h) * (size_t)(n > 0 ? n : 1));
+    if (!h) return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+    const int64_t ld = base->shape[1], tld = base->shape[0];
+    for (int i = 0; i < n; i++) {
+      h[i] = p->h_blocks[i];
+      const int64_t row0 = h[i].dst_off / ld, col0 = h[i].dst_off % ld;
+      h[i].dst_off = col0 * tld + row0;
+      h[i].dst_ld = tld;
+      h[i].filter |= IAMX_JOB_TRANSPOSE;
+    }
+    v->d_tblocks = (iamx_block_job_t *)iamx_malloc(sizeof(*h) * (size_t)(n > 0 ? n : 1));
+    int e = v->d_tblocks ? iamx_h2d(v->d_tblocks, h, sizeof(*h) * (size_t)n) : IAMX_ENOMEM;
+    free(h);
+    if (e) {
+      iamx_free(v->d_tblocks);
+      v->d_tblocks = NULL;
+      return iax_dev_error(e, "transposed block jobs");
+    }
+    v->t_gen = base->generation;
+  }
+  void *scratch = iamx_workspace(0, (size_t)p->scratch_bytes + 16);
+  if (!scratch) return iax_dev_error(IAMX_ENOMEM, "decode scratch");
+  int e = iamx_decode(p->d_lz4, p->n_lz4, v->d_tblocks, p->n_blocks, scratch, dst);
+  if (e) return iax_dev_error(e, "decode");
+  return INA_SUCCESS;
+}
+
 ina_rc_t iax_operand(iarray_container_t *c, int slot, int tmp_slot, void **d_out) {
   iarray_container_t *base = c->viewed ? c->viewed : c;
   const size_t bytes = (size_t)(base->shape[0] * base->shape[1] * base->itemsize);
@@ -654,18 +697,19 @@ ina_rc_t iax_operand(iarray_container_t *c, int slot, int tmp_slot, void **d_out
     *d_out = dst;
     return INA_SUCCESS;
   }
+  /* the rectangle of base rows x cols the view transposes */
+  const int64_t rows = c->dtshape.shape[1], cols = c->dtshape.shape[0];
+  if (rows == base->shape[0] && cols == base->shape[1]) {
+    /* the whole base: its blocks decoded straight into the transposed image */
+    ina_rc_t rc = decode_transposed(c, base, dst);
+    if (INA_FAILED(rc)) return rc;
+    *d_out = dst;
+    return INA_SUCCESS;
+  }
   void *tmp = iamx_workspace(tmp_slot, bytes + 16);
   if (!tmp) return iax_dev_error(IAMX_ENOMEM, "transpose buffer");
   ina_rc_t rc = iax_decode_dense(base, tmp, 0);
   if (INA_FAILED(rc)) return rc;
-  /* the rectangle of base rows x cols the view transposes */
-  const int64_t rows = c->dtshape.shape[1], cols = c->dtshape.shape[0];
-  if (rows == base->shape[0] && cols == base->shape[1]) {
-    int e = iamx_transpose(base->itemsize, base->shape[0], base->shape[1], tmp, dst);
-    if (e) return iax_dev_error(e, "transpose");
-    *d_out = dst;
-    return INA_SUCCESS;
-  }
   /* a slice of a transposed view: cut the rectangle into dst, transpose it back into tmp */
   const size_t is = (size_t)base->itemsize;
   if (c->voff[0] < 0 || c->voff[1] < 0 || c->voff[0] + rows > base->shape[0] || c->voff[1] + cols > base->shape[1])
@@ -686,7 +730,7 @@ ina_rc_t iax_check_decode_status(void) {
   int st = iamx_decode_status();
   if (st) iax_operand_forget(-1);
   if (st < 0) return iax_dev_error(st, "decode status");
-  if (st) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "malformed LZ4 stream (device flags 0x%x)", st);
+  if (st) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "malformed compressed stream (device flags 0x%x)", st);
   return INA_SUCCESS;
 }
 
@@ -1237,6 +1281,8 @@ INA_API(ina_rc_t) iarray_linalg_transpose(iarray_context_t *ctx, iarray_containe
   v->frame.valid = 0;
   v->on_disk = false;
   v->storage.urlpath = NULL;
+  v->d_tblocks = NULL;
+  v->t_gen = 0;
   memset(v->tuned, 0, sizeof(v->tuned));
   *b = v;
   return INA_SUCCESS;
@@ -1296,6 +1342,8 @@ INA_API(ina_rc_t) iarray_get_slice(iarray_context_t *ctx, iarray_container_t *sr
   v->frame.valid = 0;
   v->on_disk = false;
   v->storage.urlpath = NULL;
+  v->d_tblocks = NULL;
+  v->t_gen = 0;
   memset(v->tuned, 0, sizeof(v->tuned));
   *container = v;
   return INA_SUCCESS;

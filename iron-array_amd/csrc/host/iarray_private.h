@@ -87,6 +87,11 @@ struct iarray_container_s {
   iarray_container_t *viewed;
   bool transposed;
   int64_t voff[2];
+  /* a transposed view's block jobs: the base's, writing their tiles
+   * transposed (IAMX_JOB_TRANSPOSE), valid while the base's generation is
+   * t_gen */
+  iamx_block_job_t *d_tblocks;
+  uint64_t t_gen;
   /* frames: header fields of the frame it was read from; opened (rather than
    * loaded) containers count as stored on disk */
   iax_frame_params_t frame;

@@ -12,6 +12,11 @@
  * Parity pins (each function below names the checks that pin it):
  *   - orc_lz4_decode ........ liblz4 1.9.3 LZ4_compress_default outputs (black box),
  *                              known-answer SHA-256s of the reference's frame fixtures.
+ *   - orc_blosclz_decode .... c-blosc 1.21.0 chunks written with compressor "blosclz"
+ *                              (black box): literal runs, short and long matches, the
+ *                              far-distance escape, split and unsplit streams; c-blosc2's
+ *                              own BloscLZ output is not in the image (parity unpinned
+ *                              beyond the shared stream format).
  *   - orc_chunk_* / getitem .. the reference's tests/data/<name>.iarray frames (c-blosc2
  *                              writes them; SHA-256 of every decoded array is fixed),
  *                              c-blosc 1.21.0 chunks (blosc_compress, black box).
@@ -49,6 +54,8 @@ extern "C" {
 
 /* ---- LZ4 block (raw, no frame) ---------------------------------------- */
 int orc_lz4_decode(const uint8_t *src, int srclen, uint8_t *dst, int dstcap);
+/* ---- BloscLZ stream (blosclz_block.c) ------------------------------------ */
+int orc_blosclz_decode(const uint8_t *src, int srclen, uint8_t *dst, int dstcap);
 
 /* ---- Blosc chunk (blosc2 extended header v3..v5, and blosc1 v2) -------- */
 typedef struct {

@@ -322,3 +322,91 @@ def test_crafted_lz4_streams(ctx, orc, plan, n):
     assert orc.decompress_chunk(chunk, n) == want
     got = ia.decode_chunks_flat(c, [chunk], n)
     assert got == want
+
+
+# ---- BloscLZ streams and chunks of mixed codecs ------------------------------------------------
+def _cblosc_compress(b, compressor, raw, clevel, shuf, ts, bs):
+    b.blosc_set_compressor(compressor)
+    b.blosc_set_blocksize(bs)
+    dst = ctypes.create_string_buffer(len(raw) + 64)
+    cs = b.blosc_compress(clevel, shuf, ts, len(raw), raw, dst, len(raw) + 64)
+    b.blosc_set_compressor(b"lz4")
+    b.blosc_set_blocksize(0)
+    assert cs > 0
+    return dst.raw[:cs]
+
+
+def _lz77_structured(rng, n):
+    out = bytearray(rng.integers(0, 256, 64, dtype=np.uint8).tobytes())
+    while len(out) < n:
+        if rng.random() < 0.4:
+            out += rng.integers(0, 256, int(rng.integers(1, 80)), dtype=np.uint8).tobytes()
+        else:
+            d = int(rng.choice([rng.integers(1, 300), rng.integers(300, 8200), rng.integers(8000, 8500),
+                                rng.integers(8192, 74000)]))
+            d = min(d, len(out))
+            for _ in range(int(rng.integers(3, 1200))):
+                out.append(out[len(out) - d])
+    return bytes(out[:n])
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_blosclz_chunks_decode_on_gpu(ctx, orc, seed):
+    """BloscLZ streams (codec 0) written by c-blosc 1.21.0 (black box) decode
+    on the GPU to their input, bit for bit: structured data with far
+    matches and long lengths, random bytes and smooth floats, unfiltered,
+    shuffled and bitshuffled, clevels 1/5/9, forced block sizes."""
+    b = _libblosc()
+    if b is None:
+        pytest.skip("c-blosc 1.21 not present")
+    rng = np.random.default_rng(200 + seed)
+    n = 160000
+    datas = [_lz77_structured(rng, n), rng.integers(0, 256, n, dtype=np.uint8).tobytes(),
+             (np.sin(np.arange(n // 8) * 1e-3) + rng.normal(0, 1e-4, n // 8)).tobytes()]
+    for di, raw in enumerate(datas):
+        for clevel in (1, 5, 9):
+            for ts, shuf in ((1, 0), (4, 1), (8, 1), (4, 2)):
+                bs = int(rng.choice([0, 16384, 65536]))
+                print(f"case: blosclz seed={seed} data={di} clevel={clevel} ts={ts} shuf={shuf} bs={bs}", flush=True)
+                chunk = _cblosc_compress(b, b"blosclz", raw, clevel, shuf, ts, bs)
+                assert chunk[2] >> 5 == 0
+                assert ia.decode_chunks_flat(ctx, [chunk], len(raw)) == raw
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_mixed_codec_operands_matmul(orc, dt):
+    """Operands whose chunks change codec from chunk to chunk, as BTune may
+    leave them (src/iarray.c:477-498): each chunk re-compressed by c-blosc
+    1.21 with BloscLZ or LZ4 (alternating).  c-blosc picks its own block
+    size, which for chunks this small is the whole chunk, so the tilings use
+    one block per chunk.  The GPU decodes them to the data and the product
+    equals the oracle's on the same chunks (integer data: exact)."""
+    b = _libblosc()
+    if b is None:
+        pytest.skip("c-blosc 1.21 not present")
+    ctx = ia.Context(clevel=5, filters=ia.COMP_SHUFFLE)
+    rng = np.random.default_rng(9)
+    isz = np.dtype(dt).itemsize
+
+    def mixed(x, cs, bs):
+        c = ia.Container.from_numpy(ctx, x, cs, bs)
+        chunks = []
+        for i, ch in enumerate(c.chunks()):
+            raw = orc.decompress_chunk(ch, struct.unpack("<i", ch[4:8])[0])
+            comp = b"blosclz" if i % 2 == 0 else b"lz4"
+            cc = _cblosc_compress(b, comp, raw, 5, 1, isz, 0)
+            assert struct.unpack("<i", cc[8:12])[0] == bs[0] * bs[1] * isz
+            chunks.append(cc)
+        out = ia.Container.from_chunks(ctx, dt, x.shape, cs, bs, chunks)
+        return out, chunks
+
+    a = rng.integers(-20, 21, (300, 260)).astype(dt)
+    w = rng.integers(-20, 21, (260, 180)).astype(dt)
+    A, ach = mixed(a, (40, 50), (40, 50))
+    W, wch = mixed(w, (50, 40), (50, 40))
+    assert {c[2] >> 5 for c in ach} == {0, 1}
+    assert np.array_equal(A.to_numpy(), a)
+    z = ia.matmul(A, W, (100, 90), (50, 45)).to_numpy()
+    o = orc.gemm_blosc(orc.Array(ach, a.shape, (40, 50), (40, 50), isz),
+                       orc.Array(wch, w.shape, (50, 40), (50, 40), isz), (100, 90), (50, 45))
+    assert np.array_equal(z, o) and np.array_equal(z, a @ w)

@@ -508,3 +508,27 @@ def test_opt_gemv_skips_zero_blocks_on_gpu(ctx, orc, dt):
         want = orc.opt_gemv(_oracle(orc, A), _oracle(orc, X), (64,), (16,))
     assert np.isfinite(want[:acs[0] + abs_[0]]).all() and not np.isfinite(want[acs[0] + abs_[0]:]).all()
     assert np.array_equal(got, want, equal_nan=True)
+
+
+@pytest.mark.parametrize("dt,filt", [(np.float32, ia.COMP_BITSHUFFLE), (np.float64, ia.COMP_SHUFFLE),
+                                     (np.float32, 0)])
+def test_transposed_view_decodes_in_place(orc, dt, filt):
+    """A transposed view's image is decoded with every tile written transposed
+    (no separate transpose pass): ragged chunks and blocks, bitshuffled,
+    shuffled and unfiltered streams; the image equals x.T and a product with
+    it matches the oracle's transposed read."""
+    c2 = ia.Context(clevel=5, filters=filt)
+    rng = np.random.default_rng(17)
+    x = rng.integers(-100, 100, (173, 91)).astype(dt)
+    cx = ia.Container.from_numpy(c2, x, (64, 40), (24, 16))
+    t = cx.transpose()
+    assert np.array_equal(t.to_numpy(), x.T)
+    y = rng.integers(-10, 10, (173, 30)).astype(dt)
+    cy = ia.Container.from_numpy(c2, y, (50, 30), (25, 10))
+    z = ia.matmul(t, cy, (45, 30), (15, 10)).to_numpy()
+    assert np.array_equal(z, x.T @ y)
+    o = orc.gemm_blosc(orc.Array(cx.chunks(), x.shape, (64, 40), (24, 16), x.itemsize, transposed=True),
+                       orc.Array(cy.chunks(), y.shape, (50, 30), (25, 10), y.itemsize), (45, 30), (15, 10))
+    assert np.array_equal(z, o)
+    # the table is kept for the next call, and follows a rebuilt base
+    assert np.array_equal(t.to_numpy(), x.T)

@@ -119,6 +119,60 @@ def test_against_cblosc1(orc, dtype, shuf, n):
             assert orc.decompress_chunk(dst.raw[:cs], x.nbytes) == x.tobytes()
 
 
+def _lz77_structured(rng, n):
+    """Bytes built of literal pieces and copies from earlier output at
+    distances up to 74000 (BloscLZ's short, 13-bit and far-distance forms)
+    with lengths up to 1200 (its long-length form)."""
+    out = bytearray(rng.integers(0, 256, 64, dtype=np.uint8).tobytes())
+    while len(out) < n:
+        if rng.random() < 0.4:
+            out += rng.integers(0, 256, int(rng.integers(1, 80)), dtype=np.uint8).tobytes()
+        else:
+            d = int(rng.choice([rng.integers(1, 300), rng.integers(300, 8200), rng.integers(8000, 8500),
+                                rng.integers(8192, 74000)]))
+            d = min(d, len(out))
+            for _ in range(int(rng.integers(3, 1200))):
+                out.append(out[len(out) - d])
+    return bytes(out[:n])
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_blosclz_against_cblosc1(orc, seed):
+    """BloscLZ streams (codec 0) written by c-blosc 1.21.0's blosc_compress
+    with compressor "blosclz" (black box) decode through the oracle to the
+    input: structured LZ77 data with far matches, random bytes, smooth
+    floats, every clevel band, unfiltered, shuffled and bitshuffled, forced
+    block sizes."""
+    b = _libblosc()
+    if b is None:
+        pytest.skip("c-blosc 1.21 not present")
+    b.blosc_set_compressor(b"blosclz")
+    try:
+        rng = np.random.default_rng(100 + seed)
+        n = int(rng.integers(4000, 160000)) // 8 * 8
+        datas = [np.frombuffer(_lz77_structured(rng, n), np.uint8),
+                 rng.integers(0, 256, n, dtype=np.uint8),
+                 (np.sin(np.arange(n // 8) * 1e-3) + rng.normal(0, 1e-4, n // 8))]
+        seen_lz = 0
+        for x in datas:
+            for clevel in (1, 5, 9):
+                for ts, shuf in ((1, 0), (4, 1), (8, 1), (4, 2)):
+                    bs = int(rng.choice([0, 16384, 65536]))
+                    b.blosc_set_blocksize(bs)
+                    raw = x.tobytes()
+                    dst = ctypes.create_string_buffer(len(raw) + 64)
+                    cs = b.blosc_compress(clevel, shuf, ts, len(raw), raw, dst, len(raw) + 64)
+                    assert cs > 0
+                    chunk = dst.raw[:cs]
+                    assert chunk[2] >> 5 == 0   # codec 0: BloscLZ
+                    seen_lz += not (chunk[2] & 0x2)
+                    assert orc.decompress_chunk(chunk, len(raw)) == raw
+        assert seen_lz > 0
+    finally:
+        b.blosc_set_compressor(b"lz4")
+        b.blosc_set_blocksize(0)
+
+
 @pytest.mark.parametrize("ts,n", [(4, 64), (8, 40), (4, 62500), (2, 24), (4, 13)])
 def test_bitshuffle_permutation(orc, ts, n):
     rng = np.random.default_rng(ts * 1000 + n)
