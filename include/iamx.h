@@ -73,7 +73,7 @@ IAMX_API int iamx_use_stream(int side);
  * tuner, 1 and 10..11 to the mixed-precision products (B planes of one-shot
  * products, B planes of the row-panel interface, planes of the current A
  * panel) */
-#define IAMX_WORKSPACES 16
+#define IAMX_WORKSPACES 17 /* slot 16: the plain GEMM's split-K partial tiles */
 IAMX_API void *iamx_workspace(int slot, size_t bytes);
 IAMX_API void iamx_shutdown(void);
 

@@ -10,6 +10,8 @@
 // MFMA operand layout (16x16x4, one block): lane l supplies A[l%16][l/16] and
 // B[l/16][l%16]; result register v holds D[4*(l/16)+v][l%16] for float32 and
 // D[4*v+l/16][l%16] for float64 (measured with a one-hot probe on gfx950).
+#include <stdlib.h>
+
 #include "iamx_internal.h"
 
 namespace {
@@ -45,12 +47,9 @@ constexpr int BM = 128, BN = 128, PAD = 4;
 // that share A rows then share one L2.  XCD x receives nt / 8 workgroups, one
 // more when x < nt % 8, and its band starts after the bands of XCDs 0..x-1, so
 // the map is a bijection for every tile count.
-__device__ __forceinline__ void tile_coords(int &tm, int &tn, int tiles_m, int tiles_n) {
-  const int nt = tiles_m * tiles_n;
+__device__ __forceinline__ int band_id(int nt) {
   const int q = nt / 8, r = nt % 8, x = (int)blockIdx.x % 8;
-  const int id = x * q + min(x, r) + (int)blockIdx.x / 8;
-  tm = id / tiles_n;
-  tn = id % tiles_n;
+  return x * q + min(x, r) + (int)blockIdx.x / 8;
 }
 
 // K slab depth: 32 for float32 (LDS 2 x 2 x 32 x 132 x 4 B = 68 KB), 16 for
@@ -63,7 +62,8 @@ struct Slab {
 template <typename T>
 __global__ __launch_bounds__(256) void k_gemm(int64_t M, int64_t N, int64_t K, const T *__restrict__ A, int64_t lda,
                                               const T *__restrict__ B, int64_t ldb, T *__restrict__ C, int64_t ldc,
-                                              int tiles_m, int tiles_n) {
+                                              int tiles_n, int tile_base, int ntiles, int splits, int64_t kc,
+                                              T *__restrict__ partial) {
   constexpr int BK = Slab<T>::BK;
   constexpr int PER = BK * BM / 256;  // elements per thread per operand and slab (16 f32, 8 f64)
   constexpr int VEC = 16 / sizeof(T);  // elements per 16-byte load
@@ -71,9 +71,13 @@ __global__ __launch_bounds__(256) void k_gemm(int64_t M, int64_t N, int64_t K, c
   __shared__ T Bs[2][BK][BN + PAD];
   typedef typename Mfma<T>::acc_t acc_t;
   typedef T vec_t __attribute__((ext_vector_type(VEC)));
-  int tm, tn;
-  tile_coords(tm, tn, tiles_m, tiles_n);
+  // work item id = (tile tile_base + t, K part s) for t < ntiles, s < splits
+  const int id = band_id(ntiles * splits);
+  const int tl = id / splits, part = id - tl * splits;
+  const int tile = tile_base + tl;
+  const int tm = tile / tiles_n, tn = tile % tiles_n;
   const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
+  const int64_t kb = (int64_t)part * kc, ke = min(K, kb + kc);  // this item's K range (kc: a multiple of BK)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
 
@@ -83,7 +87,7 @@ __global__ __launch_bounds__(256) void k_gemm(int64_t M, int64_t N, int64_t K, c
   const bool rows_in = m0 + BM <= M && n0 + BN <= N;
   T ra[PER], rb[PER];
   auto gload = [&](int64_t k0) {
-    if (rows_in && k0 + BK <= K) {
+    if (rows_in && k0 + BK <= ke) {
       // interior slab: 16-byte loads
       const T *pa = A + (m0 + a_r) * lda + k0 + a_k;
       const T *pb = B + (k0 + b_k) * ldb + n0 + b_c;
@@ -104,13 +108,13 @@ __global__ __launch_bounds__(256) void k_gemm(int64_t M, int64_t N, int64_t K, c
 #pragma unroll
     for (int i = 0; i < PER; i++) {
       const int64_t gk = k0 + a_k + i;
-      ra[i] = (gr < M && gk < K) ? A[gr * lda + gk] : (T)0;
+      ra[i] = (gr < M && gk < ke) ? A[gr * lda + gk] : (T)0;
     }
     const int64_t gk = k0 + b_k;
 #pragma unroll
     for (int i = 0; i < PER; i++) {
       const int64_t gc = n0 + b_c + i;
-      rb[i] = (gk < K && gc < N) ? B[gk * ldb + gc] : (T)0;
+      rb[i] = (gk < ke && gc < N) ? B[gk * ldb + gc] : (T)0;
     }
   };
   auto sstore = [&](int buf) {
@@ -126,13 +130,13 @@ __global__ __launch_bounds__(256) void k_gemm(int64_t M, int64_t N, int64_t K, c
 #pragma unroll
     for (int j = 0; j < 4; j++) acc[i][j] = acc_t{0, 0, 0, 0};
 
-  const int64_t nk = (K + BK - 1) / BK;
-  gload(0);
+  const int64_t nk = (ke - kb + BK - 1) / BK;
+  gload(kb);
   sstore(0);
   __syncthreads();
   for (int64_t kt = 0; kt < nk; kt++) {
     const int cur = (int)(kt & 1);
-    if (kt + 1 < nk) gload((kt + 1) * BK);
+    if (kt + 1 < nk) gload(kb + (kt + 1) * BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       T a[4], b[4];
@@ -149,6 +153,18 @@ __global__ __launch_bounds__(256) void k_gemm(int64_t M, int64_t N, int64_t K, c
     if (kt + 1 < nk) sstore(cur ^ 1);
     __syncthreads();
   }
+  if (partial) {  // a K part of a split tile: the whole 128 x 128 partial tile
+    T *pt = partial + ((int64_t)part * ntiles + tl) * (BM * BN);
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int col = wc * 64 + j * 16 + (lane & 15);
+#pragma unroll
+        for (int v = 0; v < 4; v++) pt[(wr * 64 + i * 16 + Mfma<T>::row(lane, v)) * BN + col] = acc[i][j][v];
+      }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < 4; i++)
 #pragma unroll
@@ -160,6 +176,23 @@ __global__ __launch_bounds__(256) void k_gemm(int64_t M, int64_t N, int64_t K, c
         if (row < M && col < N) C[row * ldc + col] = acc[i][j][v];
       }
     }
+}
+
+// C of the split tiles: the sum of their `splits` partial tiles, parts in
+// ascending order (the same bits every run).
+template <typename T>
+__global__ __launch_bounds__(256) void k_sum_parts(const T *__restrict__ partial, int splits, int ntiles,
+                                                   int tile_base, int tiles_n, int64_t M, int64_t N,
+                                                   T *__restrict__ C, int64_t ldc) {
+  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (int64_t)ntiles * BM * BN) return;
+  const int tl = (int)(idx / (BM * BN)), e = (int)(idx % (BM * BN));
+  const int tile = tile_base + tl;
+  const int64_t row = (int64_t)(tile / tiles_n) * BM + e / BN, col = (int64_t)(tile % tiles_n) * BN + e % BN;
+  if (row >= M || col >= N) return;
+  T sum = partial[(int64_t)tl * (BM * BN) + e];
+  for (int p = 1; p < splits; p++) sum += partial[((int64_t)p * ntiles + tl) * (BM * BN) + e];
+  C[row * ldc + col] = sum;
 }
 
 // y = A x : one wavefront per row, lanes stride over k, tree reduction.
@@ -304,17 +337,55 @@ IAMX_API int iamx_gemm(int dtype, int precision, int slices, int64_t M, int64_t 
                                slices > 0 ? slices : 14);
   }
   if (precision != 0) return iamx::set_error(IAMX_EINVAL, "iamx_gemm: precision %d not available", precision);
+  if (dtype != 0 && dtype != 1) return iamx::set_error(IAMX_EINVAL, "iamx_gemm: dtype %d", dtype);
   const int64_t tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
-  if (tiles_m * tiles_n > INT32_MAX) return iamx::set_error(IAMX_EINVAL, "iamx_gemm: too many tiles");
-  dim3 grid((unsigned)(tiles_m * tiles_n));
-  if (dtype == 0)
-    IAMX_LAUNCH("gemm_f64_mfma", k_gemm<double>, grid, dim3(256), 0, M, N, K, (const double *)A, lda,
-                (const double *)B, ldb, (double *)C, ldc, (int)tiles_m, (int)tiles_n);
-  else if (dtype == 1)
-    IAMX_LAUNCH("gemm_f32_mfma", k_gemm<float>, grid, dim3(256), 0, M, N, K, (const float *)A, lda,
-                (const float *)B, ldb, (float *)C, ldc, (int)tiles_m, (int)tiles_n);
-  else
-    return iamx::set_error(IAMX_EINVAL, "iamx_gemm: dtype %d", dtype);
+  if (tiles_m * tiles_n > INT32_MAX / 8) return iamx::set_error(IAMX_EINVAL, "iamx_gemm: too many tiles");
+  // Tiles run as rounds of `slots` workgroups (two per CU).  The tiles past
+  // the last full round would leave most CUs idle for a whole round, so they
+  // are split into K parts (enough to fill a round, at most 8) whose partial
+  // tiles are summed in order by a second kernel; a product with fewer tiles
+  // than one round is split the same way.  IAMX_GEMM_SPLITK=0 turns this off.
+  static const bool splitk = [] {
+    const char *e = getenv("IAMX_GEMM_SPLITK");
+    return !(e && atoi(e) == 0);
+  }();
+  const int nt = (int)(tiles_m * tiles_n);
+  const int slots = 2 * (iamx::g_num_cus > 0 ? iamx::g_num_cus : 256);
+  const int full = nt / slots * slots, rem = nt - full;
+  const int64_t bk = dtype == 0 ? Slab<double>::BK : Slab<float>::BK;
+  int S = rem > 0 ? slots / rem : 1;
+  if (S > 8) S = 8;
+  int64_t kc = (K + bk - 1) / bk;  // K slabs
+  kc = (kc + S - 1) / S * bk;       // slabs per part, as elements
+  if (kc < 4 * bk) kc = 4 * bk;     // at least four slabs a part
+  S = kc > 0 ? (int)((K + kc - 1) / kc) : 1;
+  const bool split = splitk && rem > 0 && S >= 2 && rem * 4 <= slots * 3;
+  const int head = split ? full : nt;
+  const int64_t isz = dtype == 0 ? 8 : 4;
+  if (head > 0) {
+    const dim3 grid((unsigned)head);
+    if (dtype == 0)
+      IAMX_LAUNCH("gemm_f64_mfma", k_gemm<double>, grid, dim3(256), 0, M, N, K, (const double *)A, lda,
+                  (const double *)B, ldb, (double *)C, ldc, (int)tiles_n, 0, head, 1, K, (double *)nullptr);
+    else
+      IAMX_LAUNCH("gemm_f32_mfma", k_gemm<float>, grid, dim3(256), 0, M, N, K, (const float *)A, lda,
+                  (const float *)B, ldb, (float *)C, ldc, (int)tiles_n, 0, head, 1, K, (float *)nullptr);
+  }
+  if (!split) return IAMX_OK;
+  void *part = iamx_workspace(16, (size_t)S * rem * BM * BN * isz + 256);
+  if (!part) return iamx::set_error(IAMX_ENOMEM, "iamx_gemm: split-K partial tiles");
+  const dim3 pgrid((unsigned)(rem * S)), rgrid((unsigned)(((int64_t)rem * BM * BN + 255) / 256));
+  if (dtype == 0) {
+    IAMX_LAUNCH("gemm_f64_mfma", k_gemm<double>, pgrid, dim3(256), 0, M, N, K, (const double *)A, lda,
+                (const double *)B, ldb, (double *)C, ldc, (int)tiles_n, full, rem, S, kc, (double *)part);
+    IAMX_LAUNCH("gemm_sum_parts", k_sum_parts<double>, rgrid, dim3(256), 0, (const double *)part, S, rem, full,
+                (int)tiles_n, M, N, (double *)C, ldc);
+  } else {
+    IAMX_LAUNCH("gemm_f32_mfma", k_gemm<float>, pgrid, dim3(256), 0, M, N, K, (const float *)A, lda,
+                (const float *)B, ldb, (float *)C, ldc, (int)tiles_n, full, rem, S, kc, (float *)part);
+    IAMX_LAUNCH("gemm_sum_parts", k_sum_parts<float>, rgrid, dim3(256), 0, (const float *)part, S, rem, full,
+                (int)tiles_n, M, N, (float *)C, ldc);
+  }
   return IAMX_OK;
 }
 
