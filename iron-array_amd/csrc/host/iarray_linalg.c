@@ -138,15 +138,21 @@ static int gemm_rows(void *v, int64_t row0, int64_t nrows) {
  * 1024-row panels against 27.6 ms with 2048 and 36.4 ms with one panel), but
  * at least 2^23/N rows, 512 tiles of 128x128, so a panel still fills the 256
  * CUs (2000^2 products: 4 panels of 500 rows took 0.375 ms each, one launch
- * for all of C 0.39 ms); IARRAY_GEMM_PANEL_ROWS overrides (0 = one panel) */
-static int64_t panel_rows(int64_t M, int64_t N) {
+ * for all of C 0.39 ms).  Split-bf16 and Ozaki-I launch one workgroup per
+ * 256x256 tile, so their panels hold at least 2^24/N rows, 256 such tiles
+ * (8192^2 split-bf16: 0.453 ms per 1024 rows with 2048-row panels against
+ * 0.618 ms with 1024); the CRT form launches a workgroup per tile and
+ * modulus and keeps the plain rule.  IARRAY_GEMM_PANEL_ROWS overrides (0 =
+ * one panel) */
+static int64_t panel_rows(int64_t M, int64_t N, int prec) {
   const char *env = getenv("IARRAY_GEMM_PANEL_ROWS");
   if (env) {
     const long long v = atoll(env);
     return v > 0 ? (int64_t)v : M;
   }
   int64_t r = M / 8;
-  const int64_t fill = ((int64_t)1 << 23) / (N > 0 ? N : 1);
+  const int big = prec == IARRAY_MATMUL_SPLIT_BF16 || prec == IARRAY_MATMUL_OZAKI_INT8;
+  const int64_t fill = ((int64_t)1 << (big ? 24 : 23)) / (N > 0 ? N : 1);
   if (r < fill) r = fill;
   return r;
 }
@@ -269,7 +275,7 @@ static ina_rc_t run_blockpairs(iarray_container_t *a, iarray_container_t *b, iar
                     (const int32_t *)(d + o_ks), (const int32_t *)(d + o_cols), d + o_za, d + o_zb,
                     (int)a->block[0], (int)b->block[1]};
       *used = 1;
-      rc = iax_write_from_dense_rows(c, dC, panel_rows(a->shape[0], b->shape[1]), blockpair_rows, &x);
+      rc = iax_write_from_dense_rows(c, dC, panel_rows(a->shape[0], b->shape[1], IARRAY_MATMUL_PLAIN), blockpair_rows, &x);
     }
   }
   free(rows);
@@ -334,7 +340,7 @@ static ina_rc_t run_gemm(iarray_container_t *a, iarray_container_t *b, iarray_co
   int e = iamx_gemm_prepare_b(dt, prec, slices, N, K, dB, N);
   if (e) return iax_dev_error(e, "gemm");
   rows_ctx_t rx = {(const uint8_t *)dA, (uint8_t *)dC, K, N, a->itemsize};
-  rc = iax_write_from_dense_rows(c, dC, panel_rows(M, N), gemm_rows, &rx);
+  rc = iax_write_from_dense_rows(c, dC, panel_rows(M, N, prec), gemm_rows, &rx);
   if (INA_FAILED(rc)) return rc;
   return check_status();
 }
