@@ -56,6 +56,8 @@ def parse_args():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--no-mixed", action="store_true", help="skip the mixed-vs-plain block")
     p.add_argument("--mixed-steps", type=int, default=5)
+    p.add_argument("--mixed-warmup", type=int, default=5,
+                   help="untimed calls per form (the C write's GPU share settles in these)")
     p.add_argument("--cpu-threads", type=int, default=0)
     return p.parse_args()
 
@@ -293,7 +295,7 @@ def mixed_vs_plain(ia, args):
         for prec in ("plain",) + mixed:
             pc = PRECISIONS[prec]
             step = lambda: ia.matmul(w["a"], w["b"], w["c_chunks"], w["c_blocks"], precision=pc, slices=0)  # noqa: E731
-            ms, c, prof = timed_steps(ia, step, 1, args.mixed_steps)
+            ms, c, prof = timed_steps(ia, step, args.mixed_warmup, args.mixed_steps)
             S = ia.last_ozaki_slices()[0] if prec.startswith("ozaki") else None
             roof = roofline(ia, w, prof, prec, npairs(prec, S))
             pms = product_ms(prof)

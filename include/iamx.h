@@ -67,6 +67,19 @@ IAMX_API int iamx_d2h_chain(void *dst, const void *src, size_t bytes, int event)
  * it, so that work overlaps what the library stream runs next. */
 IAMX_API int iamx_side_wait(void);
 IAMX_API int iamx_use_stream(int side);
+/* Streamed uploads, for operands kept off the device between uses (larger
+ * than the device budget): two stages, each a pinned host buffer.
+ * iamx_upload_host waits for the stage's previous upload to finish and
+ * returns its buffer (grown to `bytes`); iamx_upload copies `bytes` of it to
+ * d_dst on an upload stream once the library stream has released the stage
+ * (iamx_upload_release marks that point: after the work queued so far), and
+ * makes the library stream wait for the copy.  No counterpart in the
+ * reference, whose operands live in host memory or on disk. */
+IAMX_API void *iamx_upload_host(int stage, size_t bytes);
+IAMX_API int iamx_upload(int stage, void *d_dst, size_t bytes);
+IAMX_API int iamx_upload_release(int stage);
+/* hipMemGetInfo of the library's device */
+IAMX_API int iamx_mem_info(size_t *free_bytes, size_t *total_bytes);
 /* Scratch that lives until iamx_shutdown; grown (never shrunk) on demand.
  * Each slot is independent of the others. */
 /* slots 0..7 and 12..14 belong to the host library, 8..9 to the Ozaki slice

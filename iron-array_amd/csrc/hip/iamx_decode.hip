@@ -571,6 +571,32 @@ __device__ __forceinline__ void put_elem(uint8_t *dst, int64_t idx, uint64_t v) 
   reinterpret_cast<T *>(dst)[idx] = (T)v;
 }
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// 4x4 byte transpose: r[g] = bytes g of (a, b, c, d)
+__device__ __forceinline__ void bytes4x4(uint32_t a, uint32_t b, uint32_t c, uint32_t d, uint32_t r[4]) {
+  const uint32_t t0 = __builtin_amdgcn_perm(b, a, 0x05010400u);  // a0 b0 a1 b1
+  const uint32_t t1 = __builtin_amdgcn_perm(b, a, 0x07030602u);  // a2 b2 a3 b3
+  const uint32_t t2 = __builtin_amdgcn_perm(d, c, 0x05010400u);  // c0 d0 c1 d1
+  const uint32_t t3 = __builtin_amdgcn_perm(d, c, 0x07030602u);  // c2 d2 c3 d3
+  r[0] = __builtin_amdgcn_perm(t2, t0, 0x05040100u);
+  r[1] = __builtin_amdgcn_perm(t2, t0, 0x07060302u);
+  r[2] = __builtin_amdgcn_perm(t3, t1, 0x05040100u);
+  r[3] = __builtin_amdgcn_perm(t3, t1, 0x07060302u);
+}
+
+// The 4 bytes at p as a little-endian dword, p of any alignment: the aligned
+// dword(s) holding them, funnel-shifted.  The second dword is read only when
+// p is unaligned, and then it holds p[3], so no read leaves the page of the
+// last byte asked for.
+__device__ __forceinline__ uint32_t load_u32_any(const IAMX_GLOBAL uint8_t *p) {
+  const uint64_t a = (uint64_t)p;
+  const IAMX_GLOBAL uint32_t *w = iamx::gptr<const uint32_t>(a & ~3ull);
+  const uint32_t sh = (uint32_t)(a & 3);
+  if (sh == 0) return w[0];
+  return __builtin_amdgcn_alignbyte(w[1], w[0], sh);
+}
+
 __device__ __forceinline__ void store_elem(uint8_t *dst, int ts, int64_t idx, uint64_t v) {
   switch (ts) {
     case 8: put_elem<uint64_t>(dst, idx, v); break;
@@ -657,6 +683,61 @@ __global__ __launch_bounds__(256) void k_unfilter(const iamx_block_job_t *__rest
       }
     }
   }
+  if (shuffled && (ts == 4 || ts == 8) && (J.nstreams == ts || J.nstreams == 1)) {
+    // Four consecutive elements per thread: one dword from each plane (a last
+    // group of fewer than 4 elements reads its bytes one by one, so nothing
+    // past the planes is read) and a 4x4 byte transpose per 4 planes.  The
+    // four are stored together when they share a row of an untransposed
+    // destination, one by one otherwise.
+    for (int32_t q = threadIdx.x; 4 * q < n; q += blockDim.x) {
+      const int32_t e0 = 4 * q;
+      const int32_t ne = min(4, n - e0);
+      uint32_t w[8];
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        if (j >= ts) continue;
+        if (pconst[j]) {
+          w[j] = 0x01010101u * pfill[j];
+        } else if (ne == 4) {
+          w[j] = load_u32_any(plane[j] + e0);
+        } else {
+          uint32_t v = 0;
+          for (int i = 0; i < ne; i++) v |= (uint32_t)plane[j][e0 + i] << (8 * i);
+          w[j] = v;
+        }
+      }
+      uint32_t lo[4], hi[4] = {0, 0, 0, 0};
+      bytes4x4(w[0], w[1], w[2], w[3], lo);
+      if (ts == 8) bytes4x4(w[4], w[5], w[6], w[7], hi);
+      int32_t r = e0 / cols, c = e0 - r * cols;
+      if (!tr && ne == 4 && r < J.vrows && c + 4 <= J.vcols) {
+        const int64_t idx = base + (int64_t)r * ld + c;
+        if (((uint64_t)(dst + idx * ts) & 15) == 0) {
+          if (ts == 4) {
+            *iamx::gptr<u32x4>((uint64_t)(dst + idx * 4)) = u32x4{lo[0], lo[1], lo[2], lo[3]};
+          } else {
+            IAMX_GLOBAL u32x4 *o = iamx::gptr<u32x4>((uint64_t)(dst + idx * 8));
+            o[0] = u32x4{lo[0], hi[0], lo[1], hi[1]};
+            o[1] = u32x4{lo[2], hi[2], lo[3], hi[3]};
+          }
+        } else {
+#pragma unroll
+          for (int i = 0; i < 4; i++) store_elem(dst, ts, idx + i, ((uint64_t)hi[i] << 32) | lo[i]);
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          if (i < ne && r < J.vrows && c < J.vcols)
+            store_elem(dst, ts, base + (int64_t)r * sr + (int64_t)c * sc, ((uint64_t)hi[i] << 32) | lo[i]);
+          if (++c == cols) {
+            c = 0;
+            ++r;
+          }
+        }
+      }
+    }
+    return;
+  }
   for (int32_t r = threadIdx.x / 64; r < J.vrows; r += blockDim.x / 64) {
     for (int32_t c = threadIdx.x % 64; c < J.vcols; c += 64) {
       const int32_t e = r * cols + c;
@@ -723,6 +804,56 @@ __global__ __launch_bounds__(256) void k_pack(iamx_tiling_t t, const uint8_t *__
     return;
   }
   const bool shuffled = t.filter == IAMX_FILTER_SHUFFLE && ts > 1;
+  if (shuffled && (ts == 4 || ts == 8)) {
+    // Four consecutive elements per thread, read together where they share a
+    // row inside the valid extent, one by one otherwise; their bytes j form
+    // 4 bytes of plane j (a 4x4 byte transpose per 4 planes), stored as one
+    // dword where it is aligned.
+    for (int32_t q = threadIdx.x; 4 * q < n; q += blockDim.x) {
+      const int32_t e0 = 4 * q;
+      const int32_t ne = min(4, n - e0);
+      const int32_t r = e0 / cols, c = e0 - r * cols;
+      uint32_t lo[4], hi[4] = {0, 0, 0, 0};
+      const int64_t idx = (gr0 + r) * ld + gc0 + c;
+      // vc counts valid columns from the block's first column to the chunk's
+      // (or array's) edge and can pass the block width: the four elements
+      // share a row only when they also end inside the block
+      if (ne == 4 && r < vr && c + 4 <= vc && c + 4 <= cols && ((uint64_t)(src + idx * ts) & 15) == 0) {
+        const IAMX_GLOBAL u32x4 *s = iamx::gptr<const u32x4>((uint64_t)(src + idx * ts));
+        const u32x4 a = s[0];
+        if (ts == 4) {
+          lo[0] = a.x; lo[1] = a.y; lo[2] = a.z; lo[3] = a.w;
+        } else {
+          const u32x4 b = s[1];
+          lo[0] = a.x; hi[0] = a.y; lo[1] = a.z; hi[1] = a.w;
+          lo[2] = b.x; hi[2] = b.y; lo[3] = b.z; hi[3] = b.w;
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int32_t e = e0 + i, er = e / cols, ec = e - er * cols;
+          uint64_t v = 0;
+          if (i < ne && er < vr && ec < vc) v = load_elem(src, ts, (gr0 + er) * ld + gc0 + ec);
+          lo[i] = (uint32_t)v;
+          hi[i] = (uint32_t)(v >> 32);
+        }
+      }
+      uint32_t w[8];
+      bytes4x4(lo[0], lo[1], lo[2], lo[3], w);
+      if (ts == 8) bytes4x4(hi[0], hi[1], hi[2], hi[3], w + 4);
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        if (j >= ts) continue;
+        uint8_t *d = o + (int64_t)j * n + e0;
+        if (ne == 4 && ((uint64_t)d & 3) == 0) {
+          *iamx::gptr<uint32_t>((uint64_t)d) = w[j];
+        } else {
+          for (int i = 0; i < ne; i++) d[i] = (uint8_t)(w[j] >> (8 * i));
+        }
+      }
+    }
+    return;
+  }
   for (int32_t e = threadIdx.x; e < n; e += blockDim.x) {
     const uint64_t v = elem(e);
     if (shuffled) {

@@ -97,6 +97,7 @@ constexpr int kSub = 4096;
 constexpr int kSubHlog = 10;
 constexpr int kSubSlot = kSub + 256;  // a sub-block's worst output, rounded
 constexpr int kInfo = 8;              // int32 per info record
+constexpr int kGiveUp = 1024;         // bytes scanned without a match before a sub-block stays literal
 enum { I_HAS = 0, I_POS, I_OFF, I_ML, I_LEN, I_TRAIL, I_NZ, I_FAIL };
 
 // Sub-block sbid of the call (stream e0 + sbid / K, sub-block sbid % K).
@@ -166,6 +167,10 @@ __device__ __forceinline__ void encode_sub(const EncGeom &G, const uint8_t *__re
   };
 
   for (int32_t base = 0; base < mflimit; base += 64) {
+    // A sub-block whose first kGiveUp bytes hold no match is left as
+    // literals (random mantissa planes: LZ4 would store the stream raw), the
+    // wavefront analogue of LZ4's search acceleration after misses.
+    if (first && base >= kGiveUp) break;
     const int32_t pos = base + lane;
     const bool act = pos < mflimit;
     const int32_t lp = min(pos, mflimit - 1);

@@ -225,10 +225,12 @@ static hipEvent_t g_copy_ev[IAMX_COPY_EVENTS], g_ready_ev[IAMX_COPY_EVENTS];
 static bool g_copy_ev_made = false;
 static hipStream_t g_copy_stream = nullptr;
 
+static hipStream_t g_up_stream_ref();
 static void sync_streams() {
   (void)hipStreamSynchronize(g_stream);
   if (g_side) (void)hipStreamSynchronize(g_side);
   if (g_copy_stream) (void)hipStreamSynchronize(g_copy_stream);
+  if (g_up_stream_ref()) (void)hipStreamSynchronize(g_up_stream_ref());
 }
 
 IAMX_API void *iamx_host_buffer(size_t bytes) {
@@ -354,6 +356,7 @@ IAMX_API int iamx_sync(void) {
   IAMX_HIP_CHECK(hipStreamSynchronize(g_stream));
   if (g_side) IAMX_HIP_CHECK(hipStreamSynchronize(g_side));
   if (g_copy_stream) IAMX_HIP_CHECK(hipStreamSynchronize(g_copy_stream));
+  if (g_up_stream_ref()) IAMX_HIP_CHECK(hipStreamSynchronize(g_up_stream_ref()));
   return IAMX_OK;
 }
 
@@ -373,6 +376,89 @@ IAMX_API int iamx_side_wait(void) {
   if (!ev) IAMX_HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
   IAMX_HIP_CHECK(hipEventRecord(ev, g_stream));
   IAMX_HIP_CHECK(hipStreamWaitEvent(g_side, ev, 0));
+  return IAMX_OK;
+}
+
+// ---- streamed uploads ------------------------------------------------------------------------
+// Operands too large to keep resident travel in panels through two stages:
+// each stage is a pinned host buffer copied to a device buffer on an upload
+// stream of its own (default priority).  The copy of a stage waits (on the
+// device) for the library stream to release that stage, the library stream
+// waits for the copy, and the host waits for a stage's previous copy before
+// refilling its pinned buffer.  Only events, no device-side waits.
+static void *g_up_host[2] = {nullptr, nullptr};
+static size_t g_up_bytes[2] = {0, 0};
+static hipEvent_t g_up_done[2], g_up_free[2];
+static bool g_up_done_rec[2] = {false, false};
+static hipStream_t g_up_stream = nullptr;
+
+static int upload_setup() {
+  if (g_up_stream) return IAMX_OK;
+  for (int i = 0; i < 2; i++) {
+    IAMX_HIP_CHECK(hipEventCreateWithFlags(&g_up_done[i], hipEventDisableTiming));
+    IAMX_HIP_CHECK(hipEventCreateWithFlags(&g_up_free[i], hipEventDisableTiming));
+  }
+  IAMX_HIP_CHECK(hipStreamCreateWithFlags(&g_up_stream, hipStreamNonBlocking));
+  return IAMX_OK;
+}
+
+IAMX_API void *iamx_upload_host(int stage, size_t bytes) {
+  if (stage < 0 || stage > 1 || ensure_init() || upload_setup()) return nullptr;
+  if (g_up_done_rec[stage] && hipEventSynchronize(g_up_done[stage]) != hipSuccess) {
+    set_error(IAMX_EINVAL, "iamx_upload_host: waiting for stage %d", stage);
+    return nullptr;
+  }
+  if (g_up_host[stage] && g_up_bytes[stage] >= bytes) return g_up_host[stage];
+  if (g_up_host[stage]) {
+    (void)hipHostFree(g_up_host[stage]);
+    g_up_host[stage] = nullptr;
+    g_up_bytes[stage] = 0;
+  }
+  const size_t want = bytes + bytes / 8 + 4096;
+  if (hipHostMalloc(&g_up_host[stage], want, hipHostMallocDefault) != hipSuccess) {
+    g_up_host[stage] = nullptr;
+    set_error(IAMX_ENOMEM, "hipHostMalloc(%zu) failed", want);
+    return nullptr;
+  }
+  g_up_bytes[stage] = want;
+  return g_up_host[stage];
+}
+
+IAMX_API int iamx_upload(int stage, void *d_dst, size_t bytes) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (stage < 0 || stage > 1 || !g_up_host[stage] || bytes > g_up_bytes[stage])
+    return set_error(IAMX_EINVAL, "iamx_upload: stage %d, %zu bytes", stage, bytes);
+  if (capturing()) return set_error(IAMX_EINVAL, "iamx_upload: not available under graph capture");
+  rc = upload_setup();
+  if (rc) return rc;
+  IAMX_HIP_CHECK(hipStreamWaitEvent(g_up_stream, g_up_free[stage], 0));
+  if (bytes) IAMX_HIP_CHECK(hipMemcpyAsync(d_dst, g_up_host[stage], bytes, hipMemcpyHostToDevice, g_up_stream));
+  IAMX_HIP_CHECK(hipEventRecord(g_up_done[stage], g_up_stream));
+  g_up_done_rec[stage] = true;
+  IAMX_HIP_CHECK(hipStreamWaitEvent(g_cur, g_up_done[stage], 0));
+  return IAMX_OK;
+}
+
+static hipStream_t g_up_stream_ref() { return g_up_stream; }
+
+IAMX_API int iamx_upload_release(int stage) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (stage < 0 || stage > 1) return set_error(IAMX_EINVAL, "iamx_upload_release: stage %d", stage);
+  rc = upload_setup();
+  if (rc) return rc;
+  IAMX_HIP_CHECK(hipEventRecord(g_up_free[stage], g_cur));
+  return IAMX_OK;
+}
+
+IAMX_API int iamx_mem_info(size_t *free_bytes, size_t *total_bytes) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  size_t f = 0, t = 0;
+  IAMX_HIP_CHECK(hipMemGetInfo(&f, &t));
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
   return IAMX_OK;
 }
 
@@ -411,6 +497,11 @@ IAMX_API void iamx_shutdown(void) {
   g_direct_bytes = 0;
   if (g_status_host) (void)hipHostFree(g_status_host);
   g_status_host = g_status_hdev = nullptr;
+  for (int i = 0; i < 2; i++) {
+    if (g_up_host[i]) (void)hipHostFree(g_up_host[i]);
+    g_up_host[i] = nullptr;
+    g_up_bytes[i] = 0;
+  }
 }
 
 IAMX_API void iamx_prof_enable(int on) { g_prof_on = on != 0; }

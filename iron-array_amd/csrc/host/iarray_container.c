@@ -23,6 +23,8 @@
  * src/linalg/transpose.c:18-45.
  */
 #include <math.h>
+#include <pthread.h>
+#include <sys/mman.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -155,7 +157,11 @@ INA_API(void) iarray_container_free(iarray_context_t *ctx, iarray_container_t **
   if (!cp || !*cp) return;
   iarray_container_t *c = *cp;
   if (!c->viewed) {
-    for (int64_t i = 0; i < c->nchunks; i++) free(c->chunks[i]);
+    const uint8_t *m0 = c->lazy.map, *m1 = c->lazy.map + c->lazy.map_len;
+    for (int64_t i = 0; i < c->nchunks; i++)
+      if (!(m0 && c->chunks[i] >= m0 && c->chunks[i] < m1)) free(c->chunks[i]); /* mapped chunks are borrowed */
+    if (c->lazy.map) munmap(c->lazy.map, c->lazy.map_len);
+    free(c->lazy.off);
     drop_device(c);
     free(c->chunks);
     free(c->csizes);
@@ -309,9 +315,33 @@ INA_API(ina_rc_t) iarray_linspace(iarray_context_t *ctx, iarray_dtshape_t *dtsha
 }
 
 /* ---- device mirror ------------------------------------------------------------------------ */
+ina_rc_t iax_materialize(iarray_container_t *c) {
+  if (c->viewed) c = c->viewed;
+  if (!c->lazy.pending) return INA_SUCCESS;
+  const int64_t hsize = c->lazy.hsize, opos = c->lazy.opos;
+  for (int64_t i = 0; i < c->nchunks; i++) {
+    const int64_t off = c->lazy.off[i];
+    if (off < 0) continue;
+    /* the offset was bounded at open (off <= opos - hsize - 32); the chunk's
+     * size comes from its own header and must end inside the data section */
+    const int64_t pos = hsize + off;
+    int32_t cs;
+    memcpy(&cs, c->lazy.map + pos + 12, 4);
+    if (cs < 32 || cs > opos - pos)
+      return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "frame: chunk %lld size", (long long)i);
+    c->chunks[i] = c->lazy.map + pos;
+    c->csizes[i] = cs;
+    c->lazy.off[i] = -1;
+  }
+  c->lazy.pending = false;
+  return INA_SUCCESS;
+}
+
 ina_rc_t iax_ensure_device(iarray_container_t *c) {
   if (c->viewed) return iax_ensure_device(c->viewed);
   if (c->device_valid) return INA_SUCCESS;
+  ina_rc_t mrc = iax_materialize(c);
+  if (INA_FAILED(mrc)) return mrc;
   int64_t total = 0;
   for (int64_t i = 0; i < c->nchunks; i++) {
     c->arena_off[i] = total;
@@ -506,6 +536,58 @@ static void free_jobs(jobs_t *j) {
   free(j->rc4);
 }
 
+/* Block jobs of chunk nc (host bytes ch of csize, or NULL for a
+ * never-written chunk; its device copy at dev) appended to `jobs`. */
+static ina_rc_t chunk_jobs(const iarray_container_t *c, int64_t nc, const uint8_t *ch, int32_t csize, uint64_t dev,
+                           jobs_t *jobs) {
+  ina_rc_t rc = INA_SUCCESS;
+  const int64_t ld = c->shape[1];
+  const int64_t ci0 = nc / c->nchunks_ax[1], ci1 = nc % c->nchunks_ax[1];
+  iax_header_t h;
+  if (ch) {
+    if (iax_parse_header(ch, csize, &h) || h.nbytes != c->chunk_nbytes || h.blocksize != c->blocksize ||
+        h.nblocks != c->nblocks_ax[0] * c->nblocks_ax[1])
+      return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "chunk %lld does not match the tiling", (long long)nc);
+  }
+  for (int64_t bi0 = 0; bi0 < c->nblocks_ax[0] && INA_SUCCEED(rc); bi0++)
+    for (int64_t bi1 = 0; bi1 < c->nblocks_ax[1] && INA_SUCCEED(rc); bi1++) {
+      const int64_t r_in = bi0 * c->block[0], c_in = bi1 * c->block[1];
+      const int64_t row0 = ci0 * c->chunk[0] + r_in, col0 = ci1 * c->chunk[1] + c_in;
+      int64_t vr = c->block[0], vc = c->block[1];
+      if (c->chunk[0] - r_in < vr) vr = c->chunk[0] - r_in;
+      if (c->shape[0] - row0 < vr) vr = c->shape[0] - row0;
+      if (c->chunk[1] - c_in < vc) vc = c->chunk[1] - c_in;
+      if (c->shape[1] - col0 < vc) vc = c->shape[1] - col0;
+      if (vr <= 0 || vc <= 0) continue; /* an all-padding block holds no data */
+      iamx_block_job_t J;
+      memset(&J, 0, sizeof(J));
+      const int nb = (int)(bi0 * c->nblocks_ax[1] + bi1);
+      if (ch) {
+        rc = block_streams(ch, csize, dev, &h, nb, c->itemsize, jobs, &J);
+        if (INA_FAILED(rc)) break;
+      } else { /* never-written chunk of an empty container reads as zeros */
+        J.typesize = (int16_t)c->itemsize;
+        J.bsize = c->blocksize;
+        J.nstreams = 1;
+        J.neblock = c->blocksize;
+        J.filter = IAMX_FILTER_NONE;
+        J.kind[0] = IAMX_SRC_CONST;
+      }
+      J.rows = (int32_t)c->block[0];
+      J.cols = (int32_t)c->block[1];
+      J.vrows = (int32_t)vr;
+      J.vcols = (int32_t)vc;
+      J.dst_off = row0 * ld + col0;
+      J.dst_ld = ld;
+      /* bit 0: a zero block by the reference's test (special zero chunk, or
+       * every stream of the block empty), skipped by opt_gemv */
+      const int32_t zflag = ch && (iax_chunk_is_zeros(ch, csize) || iax_block_is_zeros(ch, csize, nb));
+      if (push_blk(jobs, &J, (int32_t)row0, (int32_t)col0, (int32_t)(ci1 * c->nblocks_ax[1] + bi1), zflag))
+        rc = INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+    }
+  return rc;
+}
+
 ina_rc_t iax_ensure_plan(iarray_container_t *c) {
   if (c->viewed) return iax_ensure_plan(c->viewed);
   if (c->plan_valid) return INA_SUCCESS;
@@ -513,56 +595,9 @@ ina_rc_t iax_ensure_plan(iarray_container_t *c) {
   if (INA_FAILED(rc)) return rc;
   jobs_t jobs;
   memset(&jobs, 0, sizeof(jobs));
-  const int64_t ld = c->shape[1];
-  for (int64_t nc = 0; nc < c->nchunks && INA_SUCCEED(rc); nc++) {
-    const int64_t ci0 = nc / c->nchunks_ax[1], ci1 = nc % c->nchunks_ax[1];
-    iax_header_t h;
-    const uint8_t *ch = c->chunks[nc];
-    if (ch) {
-      if (iax_parse_header(ch, c->csizes[nc], &h) || h.nbytes != c->chunk_nbytes || h.blocksize != c->blocksize ||
-          h.nblocks != c->nblocks_ax[0] * c->nblocks_ax[1]) {
-        rc = iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "chunk %lld does not match the tiling", (long long)nc);
-        break;
-      }
-    }
-    for (int64_t bi0 = 0; bi0 < c->nblocks_ax[0] && INA_SUCCEED(rc); bi0++)
-      for (int64_t bi1 = 0; bi1 < c->nblocks_ax[1] && INA_SUCCEED(rc); bi1++) {
-        const int64_t r_in = bi0 * c->block[0], c_in = bi1 * c->block[1];
-        const int64_t row0 = ci0 * c->chunk[0] + r_in, col0 = ci1 * c->chunk[1] + c_in;
-        int64_t vr = c->block[0], vc = c->block[1];
-        if (c->chunk[0] - r_in < vr) vr = c->chunk[0] - r_in;
-        if (c->shape[0] - row0 < vr) vr = c->shape[0] - row0;
-        if (c->chunk[1] - c_in < vc) vc = c->chunk[1] - c_in;
-        if (c->shape[1] - col0 < vc) vc = c->shape[1] - col0;
-        if (vr <= 0 || vc <= 0) continue; /* an all-padding block holds no data */
-        iamx_block_job_t J;
-        memset(&J, 0, sizeof(J));
-        const int nb = (int)(bi0 * c->nblocks_ax[1] + bi1);
-        if (ch) {
-          rc = block_streams(ch, c->csizes[nc], (uint64_t)(uintptr_t)(c->d_arena + c->arena_off[nc]), &h, nb,
-                             c->itemsize, &jobs, &J);
-          if (INA_FAILED(rc)) break;
-        } else { /* never-written chunk of an empty container reads as zeros */
-          J.typesize = (int16_t)c->itemsize;
-          J.bsize = c->blocksize;
-          J.nstreams = 1;
-          J.neblock = c->blocksize;
-          J.filter = IAMX_FILTER_NONE;
-          J.kind[0] = IAMX_SRC_CONST;
-        }
-        J.rows = (int32_t)c->block[0];
-        J.cols = (int32_t)c->block[1];
-        J.vrows = (int32_t)vr;
-        J.vcols = (int32_t)vc;
-        J.dst_off = row0 * ld + col0;
-        J.dst_ld = ld;
-        /* bit 0: a zero block by the reference's test (special zero chunk, or
-         * every stream of the block empty), skipped by opt_gemv */
-        const int32_t zflag = ch && (iax_chunk_is_zeros(ch, c->csizes[nc]) || iax_block_is_zeros(ch, c->csizes[nc], nb));
-        if (push_blk(&jobs, &J, (int32_t)row0, (int32_t)col0, (int32_t)(ci1 * c->nblocks_ax[1] + bi1), zflag))
-          rc = INA_ERROR(INA_ERR_OUT_OF_MEMORY);
-      }
-  }
+  for (int64_t nc = 0; nc < c->nchunks && INA_SUCCEED(rc); nc++)
+    rc = chunk_jobs(c, nc, c->chunks[nc], c->csizes[nc], (uint64_t)(uintptr_t)(c->d_arena + c->arena_off[nc]),
+                    &jobs);
   if (INA_SUCCEED(rc)) rc = upload_plan(&jobs, &c->plan);
   c->plan.nbcols = c->nchunks_ax[1] * c->nblocks_ax[1];
   c->plan.h_lz4 = jobs.lz4;
@@ -573,6 +608,153 @@ ina_rc_t iax_ensure_plan(iarray_container_t *c) {
     return rc;
   }
   c->plan_valid = 1;
+  return INA_SUCCESS;
+}
+
+/* ---- operands streamed through the device ----------------------------------------------------- *
+ * A GEMV whose matrix does not fit the device budget (IARRAY_DEVICE_BUDGET
+ * bytes, else half of the device's free memory) never holds it resident: its
+ * chunk rows travel in panels through two upload stages.  For panel k the
+ * host copies the panel's compressed chunks into stage k%2's pinned buffer
+ * (from the container's chunks, which an opened frame pages in from its file
+ * only now), appends the panel's decode jobs (device addresses inside the
+ * stage's device buffer), and queues the upload; the library stream waits
+ * for it, decodes and multiplies the panel's blocks into the partial sums and
+ * releases the stage, so the upload of panel k+1 overlaps the GEMV of panel
+ * k.  The blocks, their jobs and the order of every sum are those of the
+ * resident GEMV, so y is the same to the bit. */
+static int64_t stream_budget(void) {
+  const char *e = getenv("IARRAY_DEVICE_BUDGET");
+  if (e && atoll(e) > 0) return (int64_t)atoll(e);
+  size_t f = 0, t = 0;
+  if (iamx_mem_info(&f, &t) != 0) return INT64_MAX;
+  return (int64_t)(f / 2);
+}
+
+/* bytes the resident GEMV keeps on the device for A: the compressed arena and
+ * the decoded LZ4 streams (at most the decoded chunks) */
+static int64_t resident_bytes(const iarray_container_t *a) {
+  int64_t cb = 0;
+  for (int64_t i = 0; i < a->nchunks; i++) cb += (a->csizes[i] + 15) & ~15;
+  return cb + a->nchunks * a->chunk_nbytes;
+}
+
+int iax_gemv_should_stream(iarray_container_t *a) {
+  if (a->viewed || a->device_valid) return 0;
+  if (INA_FAILED(iax_materialize(a))) return 0;
+  return resident_bytes(a) > stream_budget();
+}
+
+static size_t al16(size_t v) { return (v + 15) & ~(size_t)15; }
+
+ina_rc_t iax_gemv_streamed(iarray_container_t *a, int dt, const void *dx, void *dy, const uint8_t *d_xzero) {
+  ina_rc_t rc = iax_materialize(a);
+  if (INA_FAILED(rc)) return rc;
+  const int64_t M = a->shape[0];
+  const int64_t ncr = a->nchunks_ax[0], ncc = a->nchunks_ax[1];
+  const int64_t budget = stream_budget();
+  /* panels of whole chunk rows: compressed bytes and decoded scratch each at
+   * most a quarter of the budget (one chunk row at least) */
+  int64_t *pstart = (int64_t *)malloc(sizeof(int64_t) * (size_t)(ncr + 1));
+  if (!pstart) return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
+  int np = 0;
+  size_t max_stage = 0;
+  int64_t max_scratch = 0;
+  for (int64_t r = 0; r < ncr;) {
+    pstart[np++] = r;
+    int64_t cb = 0, db = 0;
+    do {
+      int64_t rcb = 0;
+      for (int64_t j = 0; j < ncc; j++) rcb += al16((size_t)a->csizes[r * ncc + j]);
+      const int64_t rdb = ncc * a->chunk_nbytes;
+      if (cb > 0 && (cb + rcb > budget / 4 || db + rdb > budget / 4)) break;
+      cb += rcb;
+      db += rdb;
+      r++;
+    } while (r < ncr);
+    if ((size_t)cb > max_stage) max_stage = (size_t)cb;
+    if (db > max_scratch) max_scratch = db;
+  }
+  pstart[np] = ncr;
+  const int64_t nbc = ncc * a->nblocks_ax[1];
+  /* the stage's device buffer: compressed chunks, then the job tables (a
+   * panel's blocks: at most its chunks' blocks; LZ4 jobs: at most 8 a block) */
+  int64_t max_blocks = 0;
+  for (int k = 0; k < np; k++) {
+    const int64_t nb = (pstart[k + 1] - pstart[k]) * ncc * a->nblocks_ax[0] * a->nblocks_ax[1];
+    if (nb > max_blocks) max_blocks = nb;
+  }
+  const size_t tables = al16((size_t)max_blocks * IAMX_MAX_STREAMS * sizeof(iamx_lz4_job_t)) +
+                        al16((size_t)max_blocks * sizeof(iamx_block_job_t)) + al16((size_t)max_blocks * 16);
+  const size_t stage_bytes = al16(max_stage) + tables;
+  uint8_t *d_stage[2] = {(uint8_t *)iamx_workspace(0, 2 * stage_bytes + 16), NULL};
+  void *scratch = iamx_workspace(7, (size_t)max_scratch + 16);
+  iax_operand_forget(0);
+  iax_operand_forget(7);
+  iax_operand_forget(6);
+  void *partial = iamx_workspace(6, (size_t)(nbc * M * a->itemsize) + 16);
+  if (!d_stage[0] || !scratch || !partial) {
+    free(pstart);
+    return iax_dev_error(IAMX_ENOMEM, "streamed gemv buffers");
+  }
+  d_stage[1] = d_stage[0] + stage_bytes;
+  int e = iamx_memset(partial, 0, (size_t)(nbc * M * a->itemsize));
+  jobs_t jobs;
+  memset(&jobs, 0, sizeof(jobs));
+  int launched = 0;
+  for (int k = 0; k < np && !e && INA_SUCCEED(rc); k++) {
+    const int s = k & 1;
+    uint8_t *h = (uint8_t *)iamx_upload_host(s, stage_bytes);
+    if (!h) {
+      e = IAMX_ENOMEM;
+      break;
+    }
+    /* the panel's chunks, 16-byte aligned, and their jobs */
+    jobs.n_lz4 = jobs.n_blk = 0;
+    jobs.scratch = 0;
+    size_t off = 0;
+    for (int64_t nc = pstart[k] * ncc; nc < pstart[k + 1] * ncc && INA_SUCCEED(rc); nc++) {
+      const uint8_t *ch = a->chunks[nc];
+      if (ch) memcpy(h + off, ch, (size_t)a->csizes[nc]);
+      rc = chunk_jobs(a, nc, ch, a->csizes[nc], (uint64_t)(uintptr_t)(d_stage[s] + off), &jobs);
+      off += ch ? al16((size_t)a->csizes[nc]) : 0;
+    }
+    if (INA_FAILED(rc)) break;
+    if ((size_t)jobs.n_lz4 > (size_t)max_blocks * IAMX_MAX_STREAMS || jobs.n_blk > max_blocks ||
+        jobs.scratch > max_scratch) {
+      rc = iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "streamed panel %d exceeds its buffers", k);
+      break;
+    }
+    iax_sort_lz4_jobs(jobs.lz4, jobs.n_lz4);
+    const size_t o_lz4 = al16(max_stage);
+    const size_t o_blk = o_lz4 + al16((size_t)max_blocks * IAMX_MAX_STREAMS * sizeof(iamx_lz4_job_t));
+    const size_t o_rc = o_blk + al16((size_t)max_blocks * sizeof(iamx_block_job_t));
+    memcpy(h + o_lz4, jobs.lz4, (size_t)jobs.n_lz4 * sizeof(iamx_lz4_job_t));
+    memcpy(h + o_blk, jobs.blk, (size_t)jobs.n_blk * sizeof(iamx_block_job_t));
+    memcpy(h + o_rc, jobs.rc4, (size_t)jobs.n_blk * 16);
+    e = iamx_upload(s, d_stage[s], o_rc + (size_t)jobs.n_blk * 16);
+    if (e || jobs.n_blk == 0) {
+      if (!e) e = iamx_upload_release(s);
+      continue;
+    }
+    const iamx_lz4_job_t *dl = (const iamx_lz4_job_t *)(d_stage[s] + o_lz4);
+    const iamx_block_job_t *db = (const iamx_block_job_t *)(d_stage[s] + o_blk);
+    const int32_t *drc = (const int32_t *)(d_stage[s] + o_rc);
+    e = d_xzero ? iamx_opt_gemv_fused(dt, dl, (int)jobs.n_lz4, db, drc, (int)jobs.n_blk, (int)nbc, (int)a->block[0],
+                                      (int)a->block[1], dx, dy, M, scratch, partial, d_xzero)
+                : iamx_gemv_fused(dt, dl, (int)jobs.n_lz4, db, drc, (int)jobs.n_blk, (int)nbc, (int)a->block[0],
+                                  (int)a->block[1], dx, dy, M, scratch, partial);
+    launched = 1;
+    if (!e) e = iamx_upload_release(s);
+  }
+  if (!e && INA_SUCCEED(rc) && !launched) e = iamx_memset(dy, 0, (size_t)(M * a->itemsize));
+  free_jobs(&jobs);
+  free(pstart);
+  if (INA_FAILED(rc)) {
+    (void)iamx_sync();
+    return rc;
+  }
+  if (e) return iax_dev_error(e, "streamed gemv");
   return INA_SUCCESS;
 }
 
@@ -953,12 +1135,12 @@ static void hybrid_task(void *v, int64_t nc, int worker) {
 }
 
 static ina_rc_t write_hybrid(iarray_container_t *c, const iamx_tiling_t *t, const void *d_dense, void *d_packed,
-                             const int64_t *pf, int npanels, iax_rows_fn rows, void *rctx, double share) {
+                             const int64_t *pf, int npanels, iax_rows_fn rows, void *rctx, int ngpu) {
   const int split = iax_split(&c->cfg, c->itemsize, c->blocksize);
   const int64_t cap = iamx_chunk_capacity(t, split);
   if (cap < c->chunk_nbytes) return iax_dev_error(IAMX_EINVAL, "chunk capacity");
-  int ngpu = (int)(share * npanels + 0.5);
   if (ngpu > npanels) ngpu = npanels;
+  if (ngpu < 0) ngpu = 0;
   uint8_t on_gpu[IAMX_COPY_EVENTS];
   for (int k = 0; k < npanels; k++) on_gpu[k] = k >= npanels - ngpu; /* the last panels: a shorter tail */
   void *d_scratch = NULL;
@@ -1044,6 +1226,86 @@ static double gpu_encode_share(const iarray_container_t *c, int npanels) {
   return v > 1.0 ? 1.0 : v;
 }
 
+/* ---- adaptive GPU share of the C write ------------------------------------------------------- *
+ * How many panels the GPU should encode depends on how long the GPU's own
+ * work takes against the host threads' encode, which differs by product form
+ * (a split-bf16 product leaves the GPU idle sooner than a plain one) and by
+ * machine.  Without IARRAY_GPU_ENCODE, each output class (element size, bytes,
+ * panel count, and the product form the caller names in iax_write_form)
+ * keeps a hill climb over the number of GPU panels: a call
+ * runs with `cur` and times the whole panel stage; the next call tries one
+ * panel more (or fewer) than the best count so far, and after a move in each
+ * direction has failed to beat the best, the class stays at the best count.
+ * The encoded bytes of a chunk differ between the two encoders (both decode
+ * to the same C). */
+#define IAX_SHARE_CLASSES 16
+int iax_write_form = 0;
+
+typedef struct {
+  int64_t key[4];
+  int used;
+  int cur, best, dir, fails;
+  double best_s;
+} share_state_t;
+static share_state_t g_share[IAX_SHARE_CLASSES];
+static pthread_mutex_t g_share_mu = PTHREAD_MUTEX_INITIALIZER;
+
+static share_state_t *share_class(const iarray_container_t *c, int npanels, int ngpu0) {
+  const int64_t key[4] = {c->itemsize, c->nchunks * c->chunk_nbytes, npanels, iax_write_form};
+  share_state_t *free_slot = NULL;
+  for (int i = 0; i < IAX_SHARE_CLASSES; i++) {
+    share_state_t *s = &g_share[i];
+    if (s->used && !memcmp(s->key, key, sizeof key)) return s;
+    if (!s->used && !free_slot) free_slot = s;
+  }
+  if (!free_slot) free_slot = &g_share[(c->nchunks + npanels) % IAX_SHARE_CLASSES]; /* evict one */
+  memset(free_slot, 0, sizeof *free_slot);
+  memcpy(free_slot->key, key, sizeof key);
+  free_slot->used = 1;
+  free_slot->cur = free_slot->best = ngpu0;
+  free_slot->dir = 1;
+  free_slot->best_s = -1.0;
+  return free_slot;
+}
+
+/* GPU panels for this call (the class's current probe) */
+static int adaptive_gpu_panels(const iarray_container_t *c, int npanels, double share0) {
+  int n0 = (int)(share0 * npanels + 0.5);
+  pthread_mutex_lock(&g_share_mu);
+  const int n = share_class(c, npanels, n0)->cur;
+  pthread_mutex_unlock(&g_share_mu);
+  return n;
+}
+
+/* record the panel stage's time `s` of a call that ran `ngpu` GPU panels and
+ * choose the next probe */
+static void adaptive_record(const iarray_container_t *c, int npanels, int ngpu, double s) {
+  pthread_mutex_lock(&g_share_mu);
+  share_state_t *st = share_class(c, npanels, ngpu);
+  if (st->cur == ngpu) {
+    if (st->best_s < 0 || s < st->best_s) {
+      if (st->best_s >= 0 && ngpu != st->best) st->fails = 0; /* a move that paid: keep going */
+      st->best_s = s;
+      st->best = ngpu;
+    } else if (ngpu != st->best) {
+      st->fails++;
+      st->dir = -st->dir;
+    }
+    int next = st->best;
+    if (st->fails < 2) {
+      next = st->best + st->dir;
+      if (next < 0 || next > npanels) { /* the edge: turn round */
+        st->fails++;
+        st->dir = -st->dir;
+        next = st->best + st->dir;
+        if (next < 0 || next > npanels || st->fails >= 2) next = st->best;
+      }
+    }
+    st->cur = next;
+  }
+  pthread_mutex_unlock(&g_share_mu);
+}
+
 /* Outputs up to this size are packed straight into mapped host memory (the
  * pack kernels' byte stores then cross the host link); IARRAY_DIRECT_WRITE=0
  * sends every output through the pipelined downloads. */
@@ -1083,9 +1345,12 @@ ina_rc_t iax_write_from_dense_rows(iarray_container_t *c, const void *d_dense, i
   const double share = gpu_encode_share(c, npanels);
   iax_last_gpu_enc_in = iax_last_gpu_enc_out = 0;
   if (share > 0.0) {
+    const int adaptive = getenv("IARRAY_GPU_ENCODE") == NULL;
+    const int ngpu = adaptive ? adaptive_gpu_panels(c, npanels, share) : (int)(share * npanels + 0.5);
     const double t0 = iax_now();
-    ina_rc_t rc = write_hybrid(c, &t, d_dense, d_packed, pf, npanels, rows, rctx, share);
+    ina_rc_t rc = write_hybrid(c, &t, d_dense, d_packed, pf, npanels, rows, rctx, ngpu);
     iax_last_encode_s = iax_now() - t0;
+    if (adaptive && !INA_FAILED(rc)) adaptive_record(c, npanels, ngpu, iax_last_encode_s);
     free(ends);
     free(pf);
     if (INA_FAILED(rc)) return rc;
@@ -1204,6 +1469,8 @@ INA_API(ina_rc_t) iarray_get_slice_buffer(iarray_context_t *ctx, iarray_containe
 INA_API(ina_rc_t) iarray_container_info(iarray_container_t *c, int64_t *nbytes, int64_t *cbytes) {
   FAIL_NULL(c);
   iarray_container_t *b = c->viewed ? c->viewed : c;
+  ina_rc_t rc = iax_materialize(b);
+  if (INA_FAILED(rc)) return rc;
   int64_t cb = 0;
   for (int64_t i = 0; i < b->nchunks; i++) cb += b->csizes[i];
   if (nbytes) *nbytes = nitems(c) * c->itemsize;
@@ -1383,6 +1650,8 @@ INA_API(ina_rc_t) iarray_container_chunk(iarray_container_t *c, int64_t nchunk, 
   FAIL_NULL(csize);
   iarray_container_t *b = c->viewed ? c->viewed : c;
   if (nchunk < 0 || nchunk >= b->nchunks) return INA_ERROR(INA_ERR_INVALID_ARGUMENT);
+  ina_rc_t rc = iax_materialize(b);
+  if (INA_FAILED(rc)) return rc;
   *chunk = b->chunks[nchunk];
   *csize = b->csizes[nchunk];
   return INA_SUCCESS;

@@ -29,6 +29,8 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <fcntl.h>
+#include <sys/mman.h>
 #include <sys/stat.h>
 #include <unistd.h>
 
@@ -206,8 +208,11 @@ static uint8_t *read_chunk_file(const char *dir, int64_t id, int32_t *csize) {
 /* `dir` non-NULL: d is the index frame (chunks.b2frame) of a sparse frame
  * directory; its offsets chunk holds chunk-file numbers instead of byte
  * offsets and follows the header directly (the index has no data section). */
+/* `map` non-NULL: d is that read-only mapping of the frame file (size
+ * bytes), and the container is made lazy: its data chunks stay in the file
+ * until iax_materialize locates them. */
 static ina_rc_t container_from_frame(iarray_context_t *ctx, const uint8_t *d, int64_t size, const char *dir,
-                                     iarray_container_t **container) {
+                                     uint8_t *map, iarray_container_t **container) {
   if (size < 64 || memcmp(d, frame_magic, sizeof(frame_magic)) != 0) return frame_error("no b2frame header");
   mp_t m = {d + 10, d + size, 0};
   const int64_t hsize = mp_int(&m);
@@ -302,10 +307,12 @@ static ina_rc_t container_from_frame(iarray_context_t *ctx, const uint8_t *d, in
   const uint8_t **ptrs = (const uint8_t **)malloc(sizeof(*ptrs) * (size_t)(nchunks > 0 ? nchunks : 1));
   int32_t *csz = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nchunks > 0 ? nchunks : 1));
   uint8_t **special = (uint8_t **)calloc((size_t)(nchunks > 0 ? nchunks : 1), sizeof(uint8_t *));
-  if (!ptrs || !csz || !special) {
+  int64_t *loff = map ? (int64_t *)malloc(sizeof(int64_t) * (size_t)(nchunks > 0 ? nchunks : 1)) : NULL;
+  if (!ptrs || !csz || !special || (map && !loff)) {
     free(ptrs);
     free(csz);
     free(special);
+    free(loff);
     return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
   }
   rc = INA_SUCCESS;
@@ -330,6 +337,7 @@ static ina_rc_t container_from_frame(iarray_context_t *ctx, const uint8_t *d, in
       }
       ptrs[i] = special[i];
       csz[i] = 32;
+      if (loff) loff[i] = -1;
       continue;
     }
     if (dir) {
@@ -344,6 +352,12 @@ static ina_rc_t container_from_frame(iarray_context_t *ctx, const uint8_t *d, in
       rc = frame_error("chunk offset");
       break;
     }
+    if (loff) { /* located in the mapping on first use */
+      loff[i] = off;
+      ptrs[i] = NULL;
+      csz[i] = 0;
+      continue;
+    }
     const int64_t pos = hsize + off;
     int32_t cs;
     memcpy(&cs, d + pos + 12, 4);
@@ -354,13 +368,59 @@ static ina_rc_t container_from_frame(iarray_context_t *ctx, const uint8_t *d, in
     ptrs[i] = d + pos;
     csz[i] = cs;
   }
-  if (INA_SUCCEED(rc)) rc = iarray_from_chunks(ctx, &dts, &st, nchunks, ptrs, csz, container);
+  if (INA_SUCCEED(rc) && map) {
+    rc = iax_container_new(ctx, &dts, &st, container);
+    if (INA_SUCCEED(rc) && (*container)->nchunks != nchunks) {
+      iarray_container_free(ctx, container);
+      rc = frame_error("chunk count");
+    }
+    if (INA_SUCCEED(rc)) {
+      iarray_container_t *c = *container;
+      for (int64_t i = 0; i < nchunks; i++)
+        if (special[i]) { /* special chunks are made here and owned by the container */
+          c->chunks[i] = special[i];
+          c->csizes[i] = csz[i];
+          special[i] = NULL;
+        }
+      c->lazy.map = map;
+      c->lazy.map_len = (size_t)size;
+      c->lazy.hsize = hsize;
+      c->lazy.opos = opos;
+      c->lazy.off = loff;
+      c->lazy.pending = true;
+      loff = NULL;
+    }
+  } else if (INA_SUCCEED(rc)) {
+    rc = iarray_from_chunks(ctx, &dts, &st, nchunks, ptrs, csz, container);
+  }
   if (INA_SUCCEED(rc)) (*container)->frame = fp;
   for (int64_t i = 0; i < nchunks; i++) free(special[i]);
   free(special);
   free(ptrs);
   free(csz);
+  free(loff);
   return rc;
+}
+
+/* iarray_container_open of a contiguous frame file: the file is mapped
+ * read-only and the container reads its chunks from the mapping on first use
+ * (the reference's Caterva opens the frame and reads chunks lazily); the
+ * mapping lives until the container is freed. */
+static ina_rc_t open_mapped(iarray_context_t *ctx, const char *urlpath, iarray_container_t **container) {
+  const int fd = open(urlpath, O_RDONLY);
+  if (fd < 0) return iax_set_error(INA_ERROR(IARRAY_ERR_CATERVA_FAILED), "cannot open %s", urlpath);
+  struct stat sb;
+  void *m = MAP_FAILED;
+  if (fstat(fd, &sb) == 0 && sb.st_size > 0) m = mmap(NULL, (size_t)sb.st_size, PROT_READ, MAP_PRIVATE, fd, 0);
+  close(fd);
+  if (m == MAP_FAILED) return iax_set_error(INA_ERROR(IARRAY_ERR_CATERVA_FAILED), "cannot map %s", urlpath);
+  const ina_rc_t rc = container_from_frame(ctx, (const uint8_t *)m, (int64_t)sb.st_size, NULL, (uint8_t *)m, container);
+  if (INA_FAILED(rc)) {
+    munmap(m, (size_t)sb.st_size);
+    return rc;
+  }
+  (*container)->on_disk = true;
+  return INA_SUCCESS;
 }
 
 static ina_rc_t load(iarray_context_t *ctx, const char *urlpath, bool on_disk, iarray_container_t **container) {
@@ -368,6 +428,7 @@ static ina_rc_t load(iarray_context_t *ctx, const char *urlpath, bool on_disk, i
   if (access(urlpath, F_OK) != 0) return iax_set_error(INA_ERROR(INA_ERR_FILE_OPEN), "%s does not exist", urlpath);
   struct stat sb;
   const bool sparse = stat(urlpath, &sb) == 0 && S_ISDIR(sb.st_mode);
+  if (on_disk && !sparse) return open_mapped(ctx, urlpath, container);
   char ipath[4096];
   if (sparse && snprintf(ipath, sizeof(ipath), "%s/chunks.b2frame", urlpath) >= (int)sizeof(ipath))
     return iax_set_error(INA_ERROR(INA_ERR_FILE_OPEN), "path too long");
@@ -383,7 +444,7 @@ static ina_rc_t load(iarray_context_t *ctx, const char *urlpath, bool on_disk, i
   }
   fclose(f);
   if (!d) return iax_set_error(INA_ERROR(IARRAY_ERR_CATERVA_FAILED), "cannot read %s", urlpath);
-  const ina_rc_t rc = container_from_frame(ctx, d, size, sparse ? urlpath : NULL, container);
+  const ina_rc_t rc = container_from_frame(ctx, d, size, sparse ? urlpath : NULL, NULL, container);
   free(d);
   if (INA_SUCCEED(rc)) (*container)->on_disk = on_disk;
   return rc;
@@ -393,9 +454,9 @@ INA_API(ina_rc_t) iarray_container_load(iarray_context_t *ctx, char *urlpath, ia
   return load(ctx, urlpath, false, container);
 }
 
-/* The reference keeps an opened frame on disk and reads chunks lazily; here
- * the chunks are read at open, as for load, and the container counts as
- * stored on disk (it cannot be saved). */
+/* The reference keeps an opened frame on disk and reads chunks lazily; so
+ * does this one for a frame file (a frame directory's chunk files are read at
+ * open), and the container counts as stored on disk (it cannot be saved). */
 INA_API(ina_rc_t) iarray_container_open(iarray_context_t *ctx, char *urlpath, iarray_container_t **container) {
   return load(ctx, urlpath, true, container);
 }
@@ -404,7 +465,7 @@ INA_API(ina_rc_t) iarray_from_cframe(iarray_context_t *ctx, uint8_t *frame, int6
                                      iarray_container_t **container) {
   (void)copy; /* the chunks are always copied into the container */
   if (!ctx || !frame || !container) return INA_ERROR(INA_ERR_NULL_POINTER);
-  return container_from_frame(ctx, frame, frame_len, NULL, container);
+  return container_from_frame(ctx, frame, frame_len, NULL, NULL, container);
 }
 
 /* ---- writing a frame ------------------------------------------------------------
@@ -660,7 +721,9 @@ ina_rc_t iax_persist(iarray_container_t *c) {
 INA_API(ina_rc_t) iarray_to_cframe(iarray_context_t *ctx, iarray_container_t *src, uint8_t **frame,
                                    int64_t *frame_len, bool *needs_free) {
   if (!ctx || !src || !frame || !frame_len || !needs_free) return INA_ERROR(INA_ERR_NULL_POINTER);
-  const ina_rc_t rc = frame_write(src, false, frame, frame_len);
+  ina_rc_t rc = iax_materialize(src);
+  if (INA_FAILED(rc)) return rc;
+  rc = frame_write(src, false, frame, frame_len);
   *needs_free = INA_SUCCEED(rc);
   return rc;
 }

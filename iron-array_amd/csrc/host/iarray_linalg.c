@@ -340,7 +340,9 @@ static ina_rc_t run_gemm(iarray_container_t *a, iarray_container_t *b, iarray_co
   int e = iamx_gemm_prepare_b(dt, prec, slices, N, K, dB, N);
   if (e) return iax_dev_error(e, "gemm");
   rows_ctx_t rx = {(const uint8_t *)dA, (uint8_t *)dC, K, N, a->itemsize};
+  iax_write_form = 1 + prec * 16 + slices;
   rc = iax_write_from_dense_rows(c, dC, panel_rows(M, N, prec), gemm_rows, &rx);
+  iax_write_form = 0;
   if (INA_FAILED(rc)) return rc;
   return check_status();
 }
@@ -354,23 +356,19 @@ static ina_rc_t run_gemv(iarray_container_t *a, iarray_container_t *x, iarray_co
   if (INA_FAILED(rc)) return rc;
   void *dy = iamx_workspace(4, (size_t)(M * a->itemsize) + 16);
   if (!dy) return iax_dev_error(IAMX_ENOMEM, "y buffer");
-  int e;
+  int e = 0;
   if (!a->viewed && a->block[1] <= 1024) {
-    rc = iax_ensure_plan(a);
-    if (INA_FAILED(rc)) return rc;
-    const iax_plan_t *p = &a->plan;
-    void *scratch = iamx_workspace(0, (size_t)p->scratch_bytes + 16);
-    const size_t pbytes = (size_t)(p->nbcols * M * a->itemsize);
-    iax_operand_forget(6);
-    void *partial = iamx_workspace(6, pbytes + 16);
-    if (!scratch || !partial) return iax_dev_error(IAMX_ENOMEM, "gemv scratch");
-    e = iamx_memset(partial, 0, pbytes);
+    const int streamed = iax_gemv_should_stream(a);
+    if (!streamed) {
+      rc = iax_ensure_plan(a);
+      if (INA_FAILED(rc)) return rc;
+    }
     uint8_t *d_xzero = NULL;
-    if (!e && opt == OPT_GEMV && p->n_blocks > 0) {
+    if (opt == OPT_GEMV) {
       /* zero flags of x's blocks (its chunk test and block test), one per
        * block column of A: the two tilings agree along K */
       const int64_t nbx = x->nchunks_ax[1] * x->nblocks_ax[1];
-      if (nbx != p->nbcols)
+      if (nbx != a->nchunks_ax[1] * a->nblocks_ax[1])
         return iax_set_error(INA_ERROR(IARRAY_ERR_INVALID_BLOCKSHAPE), "x's blocks differ from A's block columns");
       uint8_t *hz = (uint8_t *)malloc((size_t)nbx);
       d_xzero = (uint8_t *)iamx_workspace(15, (size_t)nbx + 16);
@@ -383,14 +381,28 @@ static ina_rc_t run_gemv(iarray_container_t *a, iarray_container_t *x, iarray_co
       e = iamx_h2d(d_xzero, hz, (size_t)nbx);
       free(hz);
     }
-    if (!e && p->n_blocks > 0 && d_xzero)
-      e = iamx_opt_gemv_fused(dt, p->d_lz4, p->n_lz4, p->d_blocks, p->d_blk_rc, p->n_blocks, (int)p->nbcols,
-                              (int)a->block[0], (int)a->block[1], dx, dy, M, scratch, partial, d_xzero);
-    else if (!e && p->n_blocks > 0)
-      e = iamx_gemv_fused(dt, p->d_lz4, p->n_lz4, p->d_blocks, p->d_blk_rc, p->n_blocks, (int)p->nbcols,
-                          (int)a->block[0], (int)a->block[1], dx, dy, M, scratch, partial);
-    else if (!e)
-      e = iamx_memset(dy, 0, (size_t)(M * a->itemsize));
+    if (streamed) {
+      if (!e) {
+        rc = iax_gemv_streamed(a, dt, dx, dy, d_xzero);
+        if (INA_FAILED(rc)) return rc;
+      }
+    } else {
+      const iax_plan_t *p = &a->plan;
+      void *scratch = iamx_workspace(0, (size_t)p->scratch_bytes + 16);
+      const size_t pbytes = (size_t)(p->nbcols * M * a->itemsize);
+      iax_operand_forget(6);
+      void *partial = iamx_workspace(6, pbytes + 16);
+      if (!scratch || !partial) return iax_dev_error(IAMX_ENOMEM, "gemv scratch");
+      if (!e) e = iamx_memset(partial, 0, pbytes);
+      if (!e && p->n_blocks > 0 && d_xzero)
+        e = iamx_opt_gemv_fused(dt, p->d_lz4, p->n_lz4, p->d_blocks, p->d_blk_rc, p->n_blocks, (int)p->nbcols,
+                                (int)a->block[0], (int)a->block[1], dx, dy, M, scratch, partial, d_xzero);
+      else if (!e && p->n_blocks > 0)
+        e = iamx_gemv_fused(dt, p->d_lz4, p->n_lz4, p->d_blocks, p->d_blk_rc, p->n_blocks, (int)p->nbcols,
+                            (int)a->block[0], (int)a->block[1], dx, dy, M, scratch, partial);
+      else if (!e)
+        e = iamx_memset(dy, 0, (size_t)(M * a->itemsize));
+    }
   } else {
     void *dA = NULL;
     rc = iax_operand(a, 2, 6, &dA);

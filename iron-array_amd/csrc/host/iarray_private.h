@@ -96,6 +96,19 @@ struct iarray_container_s {
    * loaded) containers count as stored on disk */
   iax_frame_params_t frame;
   bool on_disk;
+  /* a contiguous frame opened with iarray_container_open: the file is mapped
+   * read-only, and the data chunks are located in the mapping (their pages
+   * read from the file) only when something first needs them
+   * (iax_materialize); until then chunks[i] is NULL and off[i] >= 0 is the
+   * chunk's offset in the data section [hsize, opos) of the file.  Chunks in
+   * the mapping are borrowed, not owned. */
+  struct {
+    uint8_t *map;
+    size_t map_len;
+    int64_t hsize, opos;
+    int64_t *off;
+    bool pending;
+  } lazy;
   /* Ozaki slice count [0] and CRT modulus count [1] picked for this
    * container as A against one B: valid while both operands read the same
    * elements (iax_operand_key) */
@@ -169,6 +182,14 @@ ina_rc_t iax_check_decode_status(void);
 uint8_t *iax_special_chunk(int kind, int32_t nbytes, int32_t blocksize, int typesize);
 /* writes an on-disk container (storage.urlpath) through to its frame */
 ina_rc_t iax_persist(iarray_container_t *c);
+/* Locate the chunks of a lazily opened frame (a no-op for every other
+ * container); every reader of chunks[] or csizes[] calls it first. */
+ina_rc_t iax_materialize(iarray_container_t *c);
+/* GEMV y = A x with A streamed through the device in panels of chunk rows
+ * (A does not fit the device budget; see iax_gemv_should_stream).  dx, dy:
+ * device x and y; d_xzero: opt_gemv's zero flags of x's blocks, or NULL. */
+int iax_gemv_should_stream(iarray_container_t *a);
+ina_rc_t iax_gemv_streamed(iarray_container_t *a, int dt, const void *dx, void *dy, const uint8_t *d_xzero);
 /* Write C from a dense device buffer (logical shape, ld = shape1): GPU pack +
  * host LZ4 encode; fills c->chunks. */
 ina_rc_t iax_write_from_dense(iarray_container_t *c, const void *d_dense);
@@ -184,6 +205,9 @@ ina_rc_t iax_set_error(ina_rc_t rc, const char *fmt, ...);
 ina_rc_t iax_dev_error(int code, const char *what);
 
 extern double iax_last_encode_s;
+/* product form of the output being written (keys the adaptive GPU share of
+ * the C write); 0 outside a product */
+extern int iax_write_form;
 extern double iax_last_device_ms;
 extern int64_t iax_last_gpu_enc_in, iax_last_gpu_enc_out; /* bytes the GPU encoder read and wrote in the last write */
 extern int iax_last_slices;

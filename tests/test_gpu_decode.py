@@ -110,6 +110,10 @@ CASES = [
     (np.float64, (777,), (256,), (100,), ia.COMP_BITSHUFFLE, 3, ia.AUTO_SPLIT),
     (np.float32, (500, 500), (250, 250), (250, 250), ia.COMP_BITSHUFFLE, 5, ia.AUTO_SPLIT),
     (np.float64, (300, 260), (128, 128), (64, 64), ia.COMP_SHUFFLE, 5, ia.AUTO_SPLIT),
+    # rows of a multiple of 4 elements with ragged edges, planes in one stream
+    # or one stream per plane
+    (np.float32, (90, 130), (40, 60), (20, 20), ia.COMP_SHUFFLE, 5, ia.NEVER_SPLIT),
+    (np.float64, (90, 130), (40, 60), (20, 12), ia.COMP_SHUFFLE, 5, ia.ALWAYS_SPLIT),
 ]
 
 

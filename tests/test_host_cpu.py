@@ -14,6 +14,7 @@ import ctypes
 import glob
 import os
 import re
+import struct
 
 import numpy as np
 import pytest
@@ -536,3 +537,32 @@ def test_device_paths_fail_loudly_without_gpu():
     a = ia.Container.arange(ctx, np.float64, (20, 20), (10, 10), (5, 5))
     assert _err(lambda: a.to_numpy()) == ia.ERR_NO_DEVICE
     assert _err(lambda: ia.matmul(a, a, (10, 10), (5, 5))) == ia.ERR_NO_DEVICE
+
+
+@pytest.mark.parametrize("path", FRAMES[:3], ids=os.path.basename)
+def test_open_reads_chunks_lazily(tmp_path, path):
+    """iarray_container_open maps the frame file and locates a chunk only when
+    it is first needed (the reference's Caterva reads an opened frame's chunks
+    lazily, src/iarray_container.c:79-218): the chunks and sizes equal those
+    of iarray_container_load, and a chunk whose header is damaged is reported
+    on first use, not at open."""
+    ctx = _ctx()
+    full = ia.Container.load(ctx, path)
+    lazy = ia.Container.load(ctx, path, lazy=True)
+    assert lazy.shape == full.shape and lazy.dtype == full.dtype
+    assert lazy.info() == full.info()
+    assert lazy.chunks() == full.chunks()
+    # damage the size field of the last chunk of a copy: open succeeds, the
+    # first reader of the chunks fails
+    raw = bytearray(open(path, "rb").read())
+    chunks = full.chunks()
+    last = bytes(chunks[-1])
+    pos = raw.rfind(last)
+    assert pos > 0
+    raw[pos + 12:pos + 16] = struct.pack("<i", len(raw))  # ends past the data section
+    bad = tmp_path / "damaged.iarray"
+    bad.write_bytes(bytes(raw))
+    opened = ia.Container.load(ctx, str(bad), lazy=True)
+    assert _err(lambda: opened.chunks()) == ia.ERR_BLOSC_FAILED
+    assert _err(lambda: opened.info()) == ia.ERR_BLOSC_FAILED
+    assert _err(lambda: ia.Container.load(ctx, str(bad))) != 0
