@@ -116,6 +116,12 @@ INA_API(void) iarray_last_gpu_encode_bytes(int64_t *stream_bytes, int64_t *chunk
  * while neither container changes. */
 INA_API(int) iarray_last_ozaki_slices(double *sampled_err);
 
+/* Panels the last GEMV streamed its matrix in (0: the matrix was resident).
+ * A matrix whose compressed chunks plus decoded streams exceed the device
+ * budget (IARRAY_DEVICE_BUDGET bytes, environment, else half the device's
+ * free memory) travels through the device in panels of chunk rows. */
+INA_API(int) iarray_last_gemv_panels(void);
+
 /* Device description (name, CU count, gfx arch string). */
 INA_API(ina_rc_t) iarray_device_info(char *name, int namelen, int *cus, char *arch, int archlen);
 

@@ -676,6 +676,7 @@ ina_rc_t iax_gemv_streamed(iarray_container_t *a, int dt, const void *dx, void *
     if (db > max_scratch) max_scratch = db;
   }
   pstart[np] = ncr;
+  iax_last_gemv_panels = np;
   const int64_t nbc = ncc * a->nblocks_ax[1];
   /* the stage's device buffer: compressed chunks, then the job tables (a
    * panel's blocks: at most its chunks' blocks; LZ4 jobs: at most 8 a block) */

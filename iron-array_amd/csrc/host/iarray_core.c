@@ -116,6 +116,9 @@ INA_API(bool) iarray_prof_entry(int i, const char **name, int64_t *launches, dou
   return iamx_prof_entry(i, name, launches, total_ms) != 0;
 }
 
+int iax_last_gemv_panels = 0;
+INA_API(int) iarray_last_gemv_panels(void) { return iax_last_gemv_panels; }
+
 INA_API(int) iarray_last_ozaki_slices(double *sampled_err) {
   if (sampled_err) *sampled_err = iax_last_slice_err;
   return iax_last_slices;

@@ -357,6 +357,7 @@ static ina_rc_t run_gemv(iarray_container_t *a, iarray_container_t *x, iarray_co
   void *dy = iamx_workspace(4, (size_t)(M * a->itemsize) + 16);
   if (!dy) return iax_dev_error(IAMX_ENOMEM, "y buffer");
   int e = 0;
+  iax_last_gemv_panels = 0;
   if (!a->viewed && a->block[1] <= 1024) {
     const int streamed = iax_gemv_should_stream(a);
     if (!streamed) {

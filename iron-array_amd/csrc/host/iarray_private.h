@@ -189,6 +189,7 @@ ina_rc_t iax_materialize(iarray_container_t *c);
  * (A does not fit the device budget; see iax_gemv_should_stream).  dx, dy:
  * device x and y; d_xzero: opt_gemv's zero flags of x's blocks, or NULL. */
 int iax_gemv_should_stream(iarray_container_t *a);
+extern int iax_last_gemv_panels; /* panels of the last GEMV's streamed matrix, 0 if resident */
 ina_rc_t iax_gemv_streamed(iarray_container_t *a, int dt, const void *dx, void *dy, const uint8_t *d_xzero);
 /* Write C from a dense device buffer (logical shape, ld = shape1): GPU pack +
  * host LZ4 encode; fills c->chunks. */

@@ -129,6 +129,7 @@ def lib():
         "iarray_last_matmul_times": ([P(ctypes.c_double), P(ctypes.c_double)], None),
         "iarray_last_gpu_encode_bytes": ([P(i64), P(i64)], None),
         "iarray_last_ozaki_slices": ([P(ctypes.c_double)], ctypes.c_int),
+        "iarray_last_gemv_panels": ([], ctypes.c_int),
         "iarray_device_info": ([ctypes.c_char_p, ctypes.c_int, P(ctypes.c_int), ctypes.c_char_p, ctypes.c_int], rc_t),
         "iarray_container_device_plan": ([vp, vp], rc_t),
     }
@@ -459,6 +460,11 @@ def prof_entries():
         out[name.value.decode()] = (n.value, ms.value)
         i += 1
     return out
+
+
+def last_gemv_panels():
+    """Panels the last GEMV streamed its matrix in (0: resident)."""
+    return lib().iarray_last_gemv_panels()
 
 
 def last_ozaki_slices():

@@ -52,8 +52,10 @@ def test_streamed_gemv_equals_resident(orc, dtype, filt):
     resident = ia.Container.from_numpy(ctx, A, cs, bs)
     streamed = ia.Container.from_numpy(ctx, A, cs, bs)
     y0 = ia.matmul(resident, cx, (cs[0],), (bs[0],)).to_numpy()
+    assert ia.last_gemv_panels() == 0
     budget = _budget_for(streamed, (cs, bs))
     y1 = _with_budget(budget, lambda: ia.matmul(streamed, cx, (cs[0],), (bs[0],)).to_numpy())
+    assert ia.last_gemv_panels() == 5  # one panel per chunk row
     assert y1.tobytes() == y0.tobytes()
     # and again (the upload stages are reused)
     y2 = _with_budget(budget, lambda: ia.matmul(streamed, cx, (cs[0],), (bs[0],)).to_numpy())
@@ -77,6 +79,7 @@ def test_streamed_opt_gemv_skips_zero_blocks(orc):
     y0 = ia.opt_gemv(resident, cx, (cs[0],), (bs[0],)).to_numpy()
     y1 = _with_budget(_budget_for(streamed, (cs, bs)),
                       lambda: ia.opt_gemv(streamed, cx, (cs[0],), (bs[0],)).to_numpy())
+    assert ia.last_gemv_panels() == 5
     assert y1.tobytes() == y0.tobytes()
     want = orc.opt_gemv(orc.Array(streamed.chunks(), A.shape, cs, bs, 8),
                         orc.Array(cx.chunks(), x.shape, (cs[1],), (bs[1],), 8), (cs[0],), (bs[0],))
@@ -94,4 +97,5 @@ def test_streamed_gemv_from_opened_frame(tmp_path):
     src.save(path)
     opened = ia.Container.load(ctx, path, lazy=True)
     y1 = _with_budget(_budget_for(opened, (cs, bs)), lambda: ia.matmul(opened, cx, (cs[0],), (bs[0],)).to_numpy())
+    assert ia.last_gemv_panels() == 5
     assert y1.tobytes() == y0.tobytes()
