@@ -107,11 +107,16 @@ typedef struct iamx_lz4_job_s {
   int64_t dst_off; /* byte offset in scratch of the decoded stream */
   int32_t csize;   /* compressed bytes */
   int32_t nbytes;  /* decoded bytes (the stream's neblock) */
-  int32_t codec;   /* IAMX_CODEC_LZ4 (LZ4 and LZ4HC streams) or IAMX_CODEC_BLOSCLZ */
+  int32_t codec;   /* IAMX_CODEC_LZ4 (LZ4 and LZ4HC streams), IAMX_CODEC_BLOSCLZ or IAMX_CODEC_ZSTD */
   int32_t reserved;
 } iamx_lz4_job_t;
-/* The stream formats of a job (a chunk's codec, bits 5-7 of its flags, 1 or 0). */
-enum { IAMX_CODEC_LZ4 = 0, IAMX_CODEC_BLOSCLZ = 1 };
+/* The stream formats of a job (a chunk's codec, bits 5-7 of its flags: 1, 0
+ * or 4).  Zstd streams are decoded by a pass of their own that the decode
+ * entry points run once iamx_enable_zstd(1) has been called (the host library
+ * calls it when it plans a zstd stream); before that a zstd job flags
+ * IAMX_STATUS_CODEC. */
+enum { IAMX_CODEC_LZ4 = 0, IAMX_CODEC_BLOSCLZ = 1, IAMX_CODEC_ZSTD = 2 };
+IAMX_API void iamx_enable_zstd(int on);
 
 #define IAMX_MAX_STREAMS 8
 enum { IAMX_SRC_CONST = 0, IAMX_SRC_ADDR = 1, IAMX_SRC_SCRATCH = 2 };
@@ -340,6 +345,10 @@ IAMX_API int iamx_decode_status(void);
 #define IAMX_STATUS_RANGE (1 << 8)
 /* Status bit of a malformed BloscLZ stream. */
 #define IAMX_STATUS_BLOSCLZ (1 << 9)
+/* Status bit of a malformed zstd stream. */
+#define IAMX_STATUS_ZSTD (1 << 10)
+/* Status bit of a zstd job decoded while zstd decoding is off. */
+#define IAMX_STATUS_CODEC (1 << 11)
 
 #ifdef __cplusplus
 }

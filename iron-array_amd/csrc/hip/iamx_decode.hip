@@ -536,7 +536,8 @@ __device__ __forceinline__ void blosclz_stream(const iamx_lz4_job_t job, uint8_t
 
 template <int RING>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RING <= 4096 ? 8 : 1, 8))) void k_lz4_streams(
-    const iamx_lz4_job_t *__restrict__ jobs, int njobs, uint8_t *__restrict__ scratch, int64_t cap, int *status) {
+    const iamx_lz4_job_t *__restrict__ jobs, int njobs, uint8_t *__restrict__ scratch, int64_t cap, int *status,
+    int zstd_on) {
   __shared__ __align__(16) uint8_t ring[RING];
   const int j = blockIdx.x;
   if (j >= njobs) return;
@@ -548,6 +549,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RING <= 4096
   }
   if (job.codec == IAMX_CODEC_BLOSCLZ) {
     blosclz_stream<RING>(job, scratch, status, ring);
+    return;
+  }
+  if (job.codec == IAMX_CODEC_ZSTD) {  // decoded by the zstd pass
+    if (!zstd_on && threadIdx.x == 0) atomicOr(status, IAMX_STATUS_CODEC);
     return;
   }
   lz4_stream<RING>(job, scratch, status, ring);
@@ -897,20 +902,22 @@ static int lz4_ring_kib() {
 }
 
 int launch_lz4(const char *name, const iamx_lz4_job_t *d_lz4, int n_lz4, void *d_scratch, int64_t cap) {
+  const int z = zstd_enabled() ? 1 : 0;
   switch (lz4_ring_kib()) {
     case 8:
       IAMX_LAUNCH(name, k_lz4_streams<8192>, dim3(n_lz4), dim3(64), 0, d_lz4, n_lz4, (uint8_t *)d_scratch, cap,
-                  status_word());
+                  status_word(), z);
       break;
     case 16:
       IAMX_LAUNCH(name, k_lz4_streams<16384>, dim3(n_lz4), dim3(64), 0, d_lz4, n_lz4, (uint8_t *)d_scratch, cap,
-                  status_word());
+                  status_word(), z);
       break;
     default:
       IAMX_LAUNCH(name, k_lz4_streams<4096>, dim3(n_lz4), dim3(64), 0, d_lz4, n_lz4, (uint8_t *)d_scratch, cap,
-                  status_word());
+                  status_word(), z);
       break;
   }
+  if (z) return launch_zstd("zstd_streams", d_lz4, n_lz4, d_scratch, cap);
   return IAMX_OK;
 }
 

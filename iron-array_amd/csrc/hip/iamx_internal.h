@@ -35,6 +35,11 @@ int *status_word();
 int launch_lz4(const char *name, const iamx_lz4_job_t *d_lz4, int n_lz4, void *d_scratch,
                int64_t cap = INT64_MAX);
 
+// Zstd streams: their decode pass (jobs of other codecs are skipped) and
+// whether it runs (switched on once a zstd stream has been planned).
+int launch_zstd(const char *name, const iamx_lz4_job_t *d_jobs, int njobs, void *d_scratch, int64_t cap);
+bool zstd_enabled();
+
 // Resolves this device's copy of the no-allocation scratch array (once per
 // device, with the device current); called by the library's init.
 int noalloc_setup();

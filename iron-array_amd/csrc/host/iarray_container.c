@@ -479,11 +479,12 @@ static ina_rc_t block_streams(const uint8_t *ch, int32_t csize, uint64_t dev, co
       p += cs;
     } else {
       if (p + cs > csize) return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "truncated LZ4 stream");
-      /* codec formats 1 (LZ4, LZ4HC) and 0 (BloscLZ) decode on the GPU */
-      if (h->compformat != 1 && h->compformat != 0)
+      /* codec formats 1 (LZ4, LZ4HC), 0 (BloscLZ) and 4 (Zstd) decode on the GPU */
+      if (h->compformat != 1 && h->compformat != 0 && h->compformat != 4)
         return iax_set_error(INA_ERROR(IARRAY_ERR_BLOSC_FAILED), "codec format %d has no GPU decoder", h->compformat);
-      iamx_lz4_job_t lj = {dev + (uint64_t)p, jobs->scratch, cs, neblock,
-                           h->compformat == 0 ? IAMX_CODEC_BLOSCLZ : IAMX_CODEC_LZ4, 0};
+      const int codec = h->compformat == 0 ? IAMX_CODEC_BLOSCLZ : h->compformat == 4 ? IAMX_CODEC_ZSTD : IAMX_CODEC_LZ4;
+      if (codec == IAMX_CODEC_ZSTD) iamx_enable_zstd(1);
+      iamx_lz4_job_t lj = {dev + (uint64_t)p, jobs->scratch, cs, neblock, codec, 0};
       if (push_lz4(jobs, lj)) return INA_ERROR(INA_ERR_OUT_OF_MEMORY);
       J->kind[s] = IAMX_SRC_SCRATCH;
       J->src[s] = (uint64_t)jobs->scratch;

@@ -17,6 +17,12 @@
  *                              far-distance escape, split and unsplit streams; c-blosc2's
  *                              own BloscLZ output is not in the image (parity unpinned
  *                              beyond the shared stream format).
+ *   - orc_zstd_decode ....... libzstd 1.4.8 ZSTD_compress outputs (black box) at levels
+ *                              -5..22 over random, low-entropy, float, run and structured
+ *                              data (raw / RLE / compressed blocks, multi-block frames,
+ *                              1- and 4-stream Huffman literals, all FSE table modes,
+ *                              repeat offsets), and c-blosc 1.21.0 chunks written with
+ *                              compressor "zstd" (black box).
  *   - orc_chunk_* / getitem .. the reference's tests/data/<name>.iarray frames (c-blosc2
  *                              writes them; SHA-256 of every decoded array is fixed),
  *                              c-blosc 1.21.0 chunks (blosc_compress, black box).
@@ -56,6 +62,9 @@ extern "C" {
 int orc_lz4_decode(const uint8_t *src, int srclen, uint8_t *dst, int dstcap);
 /* ---- BloscLZ stream (blosclz_block.c) ------------------------------------ */
 int orc_blosclz_decode(const uint8_t *src, int srclen, uint8_t *dst, int dstcap);
+
+/* ---- Zstd frames (zstd_block.c) -------------------------------------------- */
+int orc_zstd_decode(const uint8_t *src, int srclen, uint8_t *dst, int dstcap);
 
 /* ---- Blosc chunk (blosc2 extended header v3..v5, and blosc1 v2) -------- */
 typedef struct {

@@ -414,3 +414,72 @@ def test_mixed_codec_operands_matmul(orc, dt):
     o = orc.gemm_blosc(orc.Array(ach, a.shape, (40, 50), (40, 50), isz),
                        orc.Array(wch, w.shape, (50, 40), (50, 40), isz), (100, 90), (50, 45))
     assert np.array_equal(z, o) and np.array_equal(z, a @ w)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_zstd_chunks_decode_on_gpu(ctx, orc, seed):
+    """Zstd streams (codec 4) written by c-blosc 1.21.0 (black box, libzstd
+    inside) decode on the GPU to their input, bit for bit, and to the
+    oracle's decode: structured data with far matches and long lengths
+    (compressed blocks, repeat offsets), random bytes (raw blocks or raw
+    streams), smooth floats, runs (RLE blocks), unfiltered, shuffled and
+    bitshuffled, clevels 1/5/9, forced block sizes."""
+    b = _libblosc()
+    if b is None:
+        pytest.skip("c-blosc 1.21 not present")
+    rng = np.random.default_rng(500 + seed)
+    n = 160000
+    datas = [_lz77_structured(rng, n), rng.integers(0, 256, n, dtype=np.uint8).tobytes(),
+             (np.sin(np.arange(n // 8) * 1e-3) + rng.normal(0, 1e-4, n // 8)).tobytes(),
+             np.repeat(rng.integers(0, 256, n // 4000, dtype=np.uint8), 4000).tobytes()]
+    for di, raw in enumerate(datas):
+        for clevel in (1, 5, 9):
+            for ts, shuf in ((1, 0), (4, 1), (8, 1), (4, 2)):
+                bs = int(rng.choice([0, 16384, 65536]))
+                print(f"case: zstd seed={seed} data={di} clevel={clevel} ts={ts} shuf={shuf} bs={bs}", flush=True)
+                chunk = _cblosc_compress(b, b"zstd", raw, clevel, shuf, ts, bs)
+                assert chunk[2] >> 5 == 4
+                assert orc.decompress_chunk(chunk, len(raw)) == raw
+                assert ia.decode_chunks_flat(ctx, [chunk], len(raw)) == raw
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_three_codec_operands_gemm_and_gemv(orc, dt):
+    """Operands whose chunks cycle through Zstd, BloscLZ and LZ4 (as BTune
+    may leave them, src/iarray.c:477-498), re-compressed by c-blosc 1.21:
+    the GEMM and the fused GEMV on the GPU equal the oracle's products on
+    the same chunks (integer data: exact)."""
+    b = _libblosc()
+    if b is None:
+        pytest.skip("c-blosc 1.21 not present")
+    ctx = ia.Context(clevel=5, filters=ia.COMP_SHUFFLE)
+    rng = np.random.default_rng(19)
+    isz = np.dtype(dt).itemsize
+    comps = (b"zstd", b"blosclz", b"lz4")
+
+    def mixed(x, cs, bs, shuf=1):
+        c = ia.Container.from_numpy(ctx, x, cs, bs)
+        chunks = []
+        for i, ch in enumerate(c.chunks()):
+            raw = orc.decompress_chunk(ch, struct.unpack("<i", ch[4:8])[0])
+            cc = _cblosc_compress(b, comps[i % 3], raw, 5, shuf, isz, 0)
+            assert struct.unpack("<i", cc[8:12])[0] == (bs[0] * bs[1] if len(bs) == 2 else bs[0]) * isz
+            chunks.append(cc)
+        return ia.Container.from_chunks(ctx, dt, x.shape, cs, bs, chunks), chunks
+
+    a = rng.integers(-20, 21, (300, 260)).astype(dt)
+    w = rng.integers(-20, 21, (260, 180)).astype(dt)
+    A, ach = mixed(a, (40, 50), (40, 50))
+    W, wch = mixed(w, (50, 40), (50, 40))
+    assert {c[2] >> 5 for c in ach} == {0, 1, 4}
+    assert np.array_equal(A.to_numpy(), a)
+    z = ia.matmul(A, W, (100, 90), (50, 45)).to_numpy()
+    o = orc.gemm_blosc(orc.Array(ach, a.shape, (40, 50), (40, 50), isz),
+                       orc.Array(wch, w.shape, (50, 40), (50, 40), isz), (100, 90), (50, 45))
+    assert np.array_equal(z, o) and np.array_equal(z, a @ w)
+    # GEMV with bit-shuffled Zstd / BloscLZ / LZ4 chunks of A
+    x = rng.integers(-5, 6, 260).astype(dt)
+    A2, a2ch = mixed(a, (40, 50), (40, 50), shuf=2)
+    X = ia.Container.from_numpy(ctx, x, (50,), (50,))
+    y = ia.matmul(A2, X, (40,), (40,)).to_numpy()
+    assert np.array_equal(y, a @ x)

@@ -197,3 +197,102 @@ def test_shuffle_is_byte_transpose(orc, ts):
     got = orc.shuffle(ts, x.tobytes())
     assert got == want
     assert orc.unshuffle(ts, got) == x.tobytes()
+
+
+def _libzstd():
+    for name in ("/usr/lib/x86_64-linux-gnu/libzstd.so.1", "/opt/conda/lib/libzstd.so.1"):
+        try:
+            z = ctypes.CDLL(name)
+        except OSError:
+            continue
+        z.ZSTD_compress.restype = ctypes.c_size_t
+        z.ZSTD_compress.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        z.ZSTD_compressBound.restype = ctypes.c_size_t
+        z.ZSTD_compressBound.argtypes = [ctypes.c_size_t]
+        z.ZSTD_isError.argtypes = [ctypes.c_size_t]
+        return z
+    return None
+
+
+def _zstd(z, data, level):
+    cap = z.ZSTD_compressBound(len(data))
+    dst = ctypes.create_string_buffer(cap)
+    n = z.ZSTD_compress(dst, cap, data, len(data), level)
+    assert not z.ZSTD_isError(n)
+    return dst.raw[:n]
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_zstd_against_libzstd(orc, seed):
+    """Zstd frames written by libzstd 1.4.8's ZSTD_compress (black box) decode
+    through the oracle to their input: raw, RLE and compressed blocks,
+    frames of several 128 KiB blocks (repeat tables and treeless literals),
+    single- and four-stream Huffman literals, predefined / RLE / described FSE
+    tables, repeat offsets; levels from the fast -5 to 22."""
+    z = _libzstd()
+    if z is None:
+        pytest.skip("libzstd not present")
+    rng = np.random.default_rng(300 + seed)
+    size = int(rng.integers(1, 330000))
+    kind = seed % 6
+    if kind == 0:
+        d = rng.integers(0, 256, size, dtype=np.uint8).tobytes()
+    elif kind == 1:
+        d = rng.integers(0, 1 + seed, size, dtype=np.uint8).tobytes()
+    elif kind == 2:
+        d = (rng.standard_normal(size // 4 + 1).astype(np.float32) * 1000).tobytes()[:size]
+    elif kind == 3:
+        d = np.repeat(rng.integers(0, 256, size // 50 + 1, dtype=np.uint8), 50).tobytes()[:size]
+    elif kind == 4:
+        d = _lz77_structured(rng, size)
+    else:
+        d = bytes(size)
+    for level in (-5, -1, 1, 3, 6, 9, 13, 17, 19, 22):
+        assert orc.zstd_decode(_zstd(z, d, level), len(d)) == d, f"level {level}"
+
+
+def test_zstd_rejects_malformed(orc):
+    z = _libzstd()
+    if z is None:
+        pytest.skip("libzstd not present")
+    d = _lz77_structured(np.random.default_rng(5), 50000)
+    c = bytearray(_zstd(z, d, 5))
+    with pytest.raises(ValueError):
+        orc.zstd_decode(bytes(c[:len(c) // 2]), len(d))     # truncated
+    with pytest.raises(ValueError):
+        orc.zstd_decode(b"\x00" * 4 + bytes(c[4:]), len(d))  # bad magic
+    with pytest.raises(ValueError):
+        orc.zstd_decode(bytes(c), len(d) - 1)                 # output larger than the slot
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_zstd_chunks_against_cblosc1(orc, seed):
+    """Zstd streams (codec 4) written by c-blosc 1.21.0's blosc_compress with
+    compressor "zstd" (black box) decode through the oracle's chunk decoder:
+    unfiltered, shuffled and bitshuffled, clevels 1/5/9, forced block sizes."""
+    b = _libblosc()
+    if b is None:
+        pytest.skip("c-blosc 1.21 not present")
+    b.blosc_set_compressor(b"zstd")
+    try:
+        rng = np.random.default_rng(400 + seed)
+        n = int(rng.integers(4000, 200000)) // 8 * 8
+        datas = [np.frombuffer(_lz77_structured(rng, n), np.uint8), rng.integers(0, 256, n, dtype=np.uint8),
+                 (np.sin(np.arange(n // 8) * 1e-3) + rng.normal(0, 1e-4, n // 8))]
+        seen = 0
+        for x in datas:
+            for clevel in (1, 5, 9):
+                for ts, shuf in ((1, 0), (4, 1), (8, 1), (4, 2)):
+                    b.blosc_set_blocksize(int(rng.choice([0, 16384, 65536])))
+                    raw = x.tobytes()
+                    dst = ctypes.create_string_buffer(len(raw) + 64)
+                    cs = b.blosc_compress(clevel, shuf, ts, len(raw), raw, dst, len(raw) + 64)
+                    assert cs > 0
+                    chunk = dst.raw[:cs]
+                    assert chunk[2] >> 5 == 4  # codec 4: Zstd
+                    seen += not (chunk[2] & 0x2)
+                    assert orc.decompress_chunk(chunk, len(raw)) == raw
+        assert seen > 0
+    finally:
+        b.blosc_set_compressor(b"lz4")
+        b.blosc_set_blocksize(0)
