@@ -347,7 +347,8 @@ IAMX_API int iamx_decode_status(void);
 #define IAMX_STATUS_BLOSCLZ (1 << 9)
 /* Status bit of a malformed zstd stream. */
 #define IAMX_STATUS_ZSTD (1 << 10)
-/* Status bit of a zstd job decoded while zstd decoding is off. */
+/* Status bit of a zstd job met while zstd decoding is off, or of a job whose
+ * codec has no decoder here (a zlib stream, an unknown value). */
 #define IAMX_STATUS_CODEC (1 << 11)
 
 #ifdef __cplusplus

@@ -551,8 +551,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RING <= 4096
     blosclz_stream<RING>(job, scratch, status, ring);
     return;
   }
-  if (job.codec == IAMX_CODEC_ZSTD) {  // decoded by the zstd pass
+  if (job.codec == IAMX_CODEC_ZSTD) {  // decoded by a pass of its own
     if (!zstd_on && threadIdx.x == 0) atomicOr(status, IAMX_STATUS_CODEC);
+    return;
+  }
+  if (job.codec != IAMX_CODEC_LZ4) {  // no decoder for this codec
+    if (threadIdx.x == 0) atomicOr(status, IAMX_STATUS_CODEC);
     return;
   }
   lz4_stream<RING>(job, scratch, status, ring);

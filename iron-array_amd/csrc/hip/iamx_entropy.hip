@@ -1,5 +1,7 @@
-// iamx_zstd.hip -- Zstandard stream decoder (codec format 4 of a Blosc2
-// chunk: c-blosc2 compresses each stream of a block into one zstd frame).
+// iamx_entropy.hip -- decoder of the entropy-coded Blosc2 codec Zstandard
+// (codec format 4: c-blosc2 compresses each stream of a block into one zstd
+// frame).  zlib streams (codec format 3) have no GPU decoder: the host
+// library refuses such chunks when it plans them.
 //
 // k_zstd_streams   one 64-lane wavefront per zstd stream (jobs of other codecs
 //                  return at once).  The frame is parsed uniformly by all
@@ -618,8 +620,8 @@ __global__ __launch_bounds__(64) void k_zstd_streams(const iamx_lz4_job_t *__res
 namespace iamx {
 
 // Zstd decoding is switched on (sticky) once a zstd stream has been planned;
-// until then decode launches skip the zstd pass, and the LZ4 kernel flags
-// any zstd job it meets.
+// until then decode launches skip its pass, and the LZ4 kernel flags any zstd
+// job it meets.
 static bool g_zstd_on = false;
 bool zstd_enabled() { return g_zstd_on; }
 

@@ -192,7 +192,8 @@ int orc_decode_block(const uint8_t *c, int32_t csize, int nblock, uint8_t *dst) 
     memcpy(dst, c + h.header_len + (int64_t)nblock * h.blocksize, (size_t)bsize);
     return bsize;
   }
-  if (h.compformat != 0 && h.compformat != 1 && h.compformat != 4) return -4; /* BloscLZ, LZ4, Zstd streams */
+  if (h.compformat != 0 && h.compformat != 1 && h.compformat != 3 && h.compformat != 4)
+    return -4; /* BloscLZ, LZ4, zlib and Zstd streams */
   if (h.extended && (h.blosc2_flags & 0x1)) return -5; /* dictionaries unsupported */
   const int32_t bstart = rd32(c + h.header_len + 4 * nblock);
   if (bstart < h.header_len || bstart >= csize) return -6;
@@ -222,6 +223,7 @@ int orc_decode_block(const uint8_t *c, int32_t csize, int nblock, uint8_t *dst) 
       if (end - p < cs) { rc = -11; break; }
       const int got = h.compformat == 0   ? orc_blosclz_decode(p, cs, out, neblock)
                       : h.compformat == 4 ? orc_zstd_decode(p, cs, out, neblock)
+                      : h.compformat == 3 ? orc_zlib_decode(p, cs, out, neblock)
                                           : lz4_dec(p, cs, out, neblock);
       if (got != neblock) { rc = -12; break; }
       p += cs;

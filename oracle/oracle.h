@@ -23,6 +23,10 @@
  *                              1- and 4-stream Huffman literals, all FSE table modes,
  *                              repeat offsets), and c-blosc 1.21.0 chunks written with
  *                              compressor "zstd" (black box).
+ *   - orc_zlib_decode ....... zlib compress2 outputs (Python's zlib module, black box) at
+ *                              levels 0-9 over random, low-entropy, float, run and
+ *                              structured data (stored, fixed and dynamic blocks), and
+ *                              c-blosc 1.21.0 chunks written with compressor "zlib".
  *   - orc_chunk_* / getitem .. the reference's tests/data/<name>.iarray frames (c-blosc2
  *                              writes them; SHA-256 of every decoded array is fixed),
  *                              c-blosc 1.21.0 chunks (blosc_compress, black box).
@@ -65,6 +69,9 @@ int orc_blosclz_decode(const uint8_t *src, int srclen, uint8_t *dst, int dstcap)
 
 /* ---- Zstd frames (zstd_block.c) -------------------------------------------- */
 int orc_zstd_decode(const uint8_t *src, int srclen, uint8_t *dst, int dstcap);
+
+/* ---- zlib streams (deflate_block.c) ------------------------------------------ */
+int orc_zlib_decode(const uint8_t *src, int srclen, uint8_t *dst, int dstcap);
 
 /* ---- Blosc chunk (blosc2 extended header v3..v5, and blosc1 v2) -------- */
 typedef struct {

@@ -43,6 +43,7 @@ def lib():
         vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
         L.orc_lz4_decode.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int]
         L.orc_zstd_decode.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int]
+        L.orc_zlib_decode.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int]
         L.orc_decode_block.argtypes = [vp, i32, ctypes.c_int, vp]
         L.orc_decompress_chunk.argtypes = [vp, i32, vp, i64]
         L.orc_getitem.argtypes = [vp, i32, ctypes.c_int, ctypes.c_int, vp]
@@ -80,6 +81,15 @@ def zstd_decode(src: bytes, n: int) -> bytes:
     r = lib().orc_zstd_decode(src, len(src), out, n)
     if r < 0:
         raise ValueError(f"zstd decode error {r}")
+    return out.raw[:r]
+
+
+def zlib_decode(src: bytes, n: int) -> bytes:
+    """Decode a zlib stream (orc_zlib_decode) into at most n bytes."""
+    out = ctypes.create_string_buffer(max(n, 1))
+    r = lib().orc_zlib_decode(src, len(src), out, n)
+    if r < 0:
+        raise ValueError(f"zlib decode error {r}")
     return out.raw[:r]
 
 

@@ -483,3 +483,95 @@ def test_three_codec_operands_gemm_and_gemv(orc, dt):
     X = ia.Container.from_numpy(ctx, x, (50,), (50,))
     y = ia.matmul(A2, X, (40,), (40,)).to_numpy()
     assert np.array_equal(y, a @ x)
+
+
+# A GPU zlib decoder (one wavefront per stream, canonical Huffman codes read
+# bit by bit) stopped with a memory aperture violation on its first run over
+# c-blosc 1.21 zlib chunks; a CPU emulation of the kernel over the same
+# streams stays in bounds and decodes them exactly, so the cause was not
+# found and the decoder was removed.  These cases stay here, skipped.
+ZLIB_SKIP = ("no GPU zlib decoder: the one written faulted on the GPU with its cause not found, "
+             "and zlib chunks are refused")
+
+
+@pytest.mark.skip(reason=ZLIB_SKIP)
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_four_codec_operands_gemm_and_gemv(orc, dt):
+    """Operands whose chunks cycle through Zstd, BloscLZ, zlib and LZ4 (as
+    BTune may leave them, src/iarray.c:477-498), re-compressed by c-blosc
+    1.21: the GEMM and the fused GEMV on the GPU equal the oracle's products
+    on the same chunks (integer data: exact)."""
+    b = _libblosc()
+    if b is None:
+        pytest.skip("c-blosc 1.21 not present")
+    ctx = ia.Context(clevel=5, filters=ia.COMP_SHUFFLE)
+    rng = np.random.default_rng(19)
+    isz = np.dtype(dt).itemsize
+    comps = (b"zstd", b"blosclz", b"zlib", b"lz4")
+
+    def mixed(x, cs, bs, shuf=1):
+        c = ia.Container.from_numpy(ctx, x, cs, bs)
+        chunks = []
+        for i, ch in enumerate(c.chunks()):
+            raw = orc.decompress_chunk(ch, struct.unpack("<i", ch[4:8])[0])
+            cc = _cblosc_compress(b, comps[i % 4], raw, 5, shuf, isz, 0)
+            assert struct.unpack("<i", cc[8:12])[0] == (bs[0] * bs[1] if len(bs) == 2 else bs[0]) * isz
+            chunks.append(cc)
+        return ia.Container.from_chunks(ctx, dt, x.shape, cs, bs, chunks), chunks
+
+    a = rng.integers(-20, 21, (300, 260)).astype(dt)
+    w = rng.integers(-20, 21, (260, 180)).astype(dt)
+    A, ach = mixed(a, (40, 50), (40, 50))
+    W, wch = mixed(w, (50, 40), (50, 40))
+    assert {c[2] >> 5 for c in ach} == {0, 1, 3, 4}
+    assert np.array_equal(A.to_numpy(), a)
+    z = ia.matmul(A, W, (100, 90), (50, 45)).to_numpy()
+    o = orc.gemm_blosc(orc.Array(ach, a.shape, (40, 50), (40, 50), isz),
+                       orc.Array(wch, w.shape, (50, 40), (50, 40), isz), (100, 90), (50, 45))
+    assert np.array_equal(z, o) and np.array_equal(z, a @ w)
+    # GEMV with bit-shuffled Zstd / BloscLZ / LZ4 chunks of A
+    x = rng.integers(-5, 6, 260).astype(dt)
+    A2, a2ch = mixed(a, (40, 50), (40, 50), shuf=2)
+    X = ia.Container.from_numpy(ctx, x, (50,), (50,))
+    y = ia.matmul(A2, X, (40,), (40,)).to_numpy()
+    assert np.array_equal(y, a @ x)
+
+
+@pytest.mark.skip(reason=ZLIB_SKIP)
+@pytest.mark.parametrize("seed", range(2))
+def test_zlib_chunks_decode_on_gpu(ctx, orc, seed):
+    """zlib streams (codec 3) written by c-blosc 1.21.0 (black box) decode on
+    the GPU to their input, bit for bit, and to the oracle's decode: stored,
+    fixed and dynamic blocks from structured, random, smooth and run data,
+    unfiltered, shuffled and bitshuffled, clevels 1/5/9."""
+    b = _libblosc()
+    if b is None:
+        pytest.skip("c-blosc 1.21 not present")
+    rng = np.random.default_rng(800 + seed)
+    n = 120000
+    datas = [_lz77_structured(rng, n), rng.integers(0, 256, n, dtype=np.uint8).tobytes(),
+             (np.sin(np.arange(n // 8) * 1e-3) + rng.normal(0, 1e-4, n // 8)).tobytes(),
+             np.repeat(rng.integers(0, 256, n // 3000, dtype=np.uint8), 3000).tobytes()]
+    for di, raw in enumerate(datas):
+        for clevel in (1, 5, 9):
+            for ts, shuf in ((1, 0), (4, 1), (8, 2)):
+                bs = int(rng.choice([0, 16384]))
+                print(f"case: zlib seed={seed} data={di} clevel={clevel} ts={ts} shuf={shuf} bs={bs}", flush=True)
+                chunk = _cblosc_compress(b, b"zlib", raw, clevel, shuf, ts, bs)
+                assert chunk[2] >> 5 == 3
+                assert orc.decompress_chunk(chunk, len(raw)) == raw
+                assert ia.decode_chunks_flat(ctx, [chunk], len(raw)) == raw
+
+
+def test_zlib_chunks_are_refused(ctx):
+    """zlib streams (codec 3) have no GPU decoder: a chunk holding them is
+    refused when its decode is planned, with an error that names the codec,
+    and nothing is decoded from it."""
+    b = _libblosc()
+    if b is None:
+        pytest.skip("c-blosc 1.21 not present")
+    raw = np.arange(20000, dtype=np.float32).tobytes()
+    chunk = _cblosc_compress(b, b"zlib", raw, 5, 1, 4, 0)
+    assert chunk[2] >> 5 == 3
+    with pytest.raises(ia.IArrayError, match="codec format 3 has no GPU decoder"):
+        ia.decode_chunks_flat(ctx, [chunk], len(raw))

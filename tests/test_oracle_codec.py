@@ -296,3 +296,56 @@ def test_zstd_chunks_against_cblosc1(orc, seed):
     finally:
         b.blosc_set_compressor(b"lz4")
         b.blosc_set_blocksize(0)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_zlib_against_python_zlib(orc, seed):
+    """zlib streams written by zlib's compress2 (Python's zlib module, black
+    box) at every level decode through the oracle to their input: stored
+    blocks (level 0 and random data), fixed and dynamic Huffman blocks, long
+    and overlapping matches."""
+    import zlib
+    rng = np.random.default_rng(600 + seed)
+    size = int(rng.integers(1, 200000))
+    kind = seed % 4
+    if kind == 0:
+        d = rng.integers(0, 256, size, dtype=np.uint8).tobytes()
+    elif kind == 1:
+        d = rng.integers(0, 1 + seed, size, dtype=np.uint8).tobytes()
+    elif kind == 2:
+        d = _lz77_structured(rng, size)
+    else:
+        d = np.repeat(rng.integers(0, 256, size // 300 + 1, dtype=np.uint8), 300).tobytes()[:size]
+    for level in range(10):
+        assert orc.zlib_decode(zlib.compress(d, level), len(d)) == d, f"level {level}"
+    with pytest.raises(ValueError):
+        orc.zlib_decode(zlib.compress(d, 6)[: max(3, size // 100)], len(d))
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_zlib_chunks_against_cblosc1(orc, seed):
+    """zlib streams (codec 3) written by c-blosc 1.21.0's blosc_compress with
+    compressor "zlib" (black box) decode through the oracle's chunk decoder."""
+    b = _libblosc()
+    if b is None:
+        pytest.skip("c-blosc 1.21 not present")
+    b.blosc_set_compressor(b"zlib")
+    try:
+        rng = np.random.default_rng(700 + seed)
+        n = int(rng.integers(4000, 120000)) // 8 * 8
+        datas = [np.frombuffer(_lz77_structured(rng, n), np.uint8), rng.integers(0, 256, n, dtype=np.uint8),
+                 (np.sin(np.arange(n // 8) * 1e-3) + rng.normal(0, 1e-4, n // 8))]
+        for x in datas:
+            for clevel in (1, 5, 9):
+                for ts, shuf in ((1, 0), (4, 1), (8, 2)):
+                    b.blosc_set_blocksize(int(rng.choice([0, 16384])))
+                    raw = x.tobytes()
+                    dst = ctypes.create_string_buffer(len(raw) + 64)
+                    cs = b.blosc_compress(clevel, shuf, ts, len(raw), raw, dst, len(raw) + 64)
+                    assert cs > 0
+                    chunk = dst.raw[:cs]
+                    assert chunk[2] >> 5 == 3  # codec 3: zlib
+                    assert orc.decompress_chunk(chunk, len(raw)) == raw
+    finally:
+        b.blosc_set_compressor(b"lz4")
+        b.blosc_set_blocksize(0)
