@@ -85,8 +85,9 @@ IAMX_API int iamx_mem_info(size_t *free_bytes, size_t *total_bytes);
 /* slots 0..7 and 12..14 belong to the host library, 8..9 to the Ozaki slice
  * tuner, 1 and 10..11 to the mixed-precision products (B planes of one-shot
  * products, B planes of the row-panel interface, planes of the current A
- * panel) */
-#define IAMX_WORKSPACES 17 /* slot 16: the plain GEMM's split-K partial tiles */
+ * panel), 16 to the plain GEMM's split-K partial tiles, 17 to hipBLASLt */
+#define IAMX_WORKSPACES 18
+#define IAMX_BLT_WORKSPACE 17
 IAMX_API void *iamx_workspace(int slot, size_t bytes);
 IAMX_API void iamx_shutdown(void);
 

@@ -338,55 +338,69 @@ IAMX_API int iamx_gemm(int dtype, int precision, int slices, int64_t M, int64_t 
   }
   if (precision != 0) return iamx::set_error(IAMX_EINVAL, "iamx_gemm: precision %d not available", precision);
   if (dtype != 0 && dtype != 1) return iamx::set_error(IAMX_EINVAL, "iamx_gemm: dtype %d", dtype);
-  const int64_t tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
-  if (tiles_m * tiles_n > INT32_MAX / 8) return iamx::set_error(IAMX_EINVAL, "iamx_gemm: too many tiles");
-  // Tiles run as rounds of `slots` workgroups (two per CU).  The tiles past
-  // the last full round would leave most CUs idle for a whole round, so they
-  // are split into K parts (enough to fill a round, at most 8) whose partial
-  // tiles are summed in order by a second kernel; a product with fewer tiles
-  // than one round is split the same way.  IAMX_GEMM_SPLITK=0 turns this off.
-  static const bool splitk = [] {
-    const char *e = getenv("IAMX_GEMM_SPLITK");
-    return !(e && atoi(e) == 0);
-  }();
-  const int nt = (int)(tiles_m * tiles_n);
-  const int slots = 2 * (iamx::g_num_cus > 0 ? iamx::g_num_cus : 256);
-  const int full = nt / slots * slots, rem = nt - full;
-  const int64_t bk = dtype == 0 ? Slab<double>::BK : Slab<float>::BK;
-  int S = rem > 0 ? slots / rem : 1;
-  if (S > 8) S = 8;
-  int64_t kc = (K + bk - 1) / bk;  // K slabs
-  kc = (kc + S - 1) / S * bk;       // slabs per part, as elements
-  if (kc < 4 * bk) kc = 4 * bk;     // at least four slabs a part
-  S = kc > 0 ? (int)((K + kc - 1) / kc) : 1;
-  const bool split = splitk && rem > 0 && S >= 2 && rem * 4 <= slots * 3;
-  const int head = split ? full : nt;
-  const int64_t isz = dtype == 0 ? 8 : 4;
-  if (head > 0) {
-    const dim3 grid((unsigned)head);
-    if (dtype == 0)
-      IAMX_LAUNCH("gemm_f64_mfma", k_gemm<double>, grid, dim3(256), 0, M, N, K, (const double *)A, lda,
-                  (const double *)B, ldb, (double *)C, ldc, (int)tiles_n, 0, head, 1, K, (double *)nullptr);
-    else
-      IAMX_LAUNCH("gemm_f32_mfma", k_gemm<float>, grid, dim3(256), 0, M, N, K, (const float *)A, lda,
-                  (const float *)B, ldb, (float *)C, ldc, (int)tiles_n, 0, head, 1, K, (float *)nullptr);
-  }
-  if (!split) return IAMX_OK;
-  void *part = iamx_workspace(16, (size_t)S * rem * BM * BN * isz + 256);
-  if (!part) return iamx::set_error(IAMX_ENOMEM, "iamx_gemm: split-K partial tiles");
-  const dim3 pgrid((unsigned)(rem * S)), rgrid((unsigned)(((int64_t)rem * BM * BN + 255) / 256));
-  if (dtype == 0) {
-    IAMX_LAUNCH("gemm_f64_mfma", k_gemm<double>, pgrid, dim3(256), 0, M, N, K, (const double *)A, lda,
-                (const double *)B, ldb, (double *)C, ldc, (int)tiles_n, full, rem, S, kc, (double *)part);
-    IAMX_LAUNCH("gemm_sum_parts", k_sum_parts<double>, rgrid, dim3(256), 0, (const double *)part, S, rem, full,
-                (int)tiles_n, M, N, (double *)C, ldc);
-  } else {
-    IAMX_LAUNCH("gemm_f32_mfma", k_gemm<float>, pgrid, dim3(256), 0, M, N, K, (const float *)A, lda,
-                (const float *)B, ldb, (float *)C, ldc, (int)tiles_n, full, rem, S, kc, (float *)part);
-    IAMX_LAUNCH("gemm_sum_parts", k_sum_parts<float>, rgrid, dim3(256), 0, (const float *)part, S, rem, full,
-                (int)tiles_n, M, N, (float *)C, ldc);
-  }
-  return IAMX_OK;
+  // the hand-written MFMA kernel
+  auto own = [&](int) -> int {
+    const int64_t tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
+    if (tiles_m * tiles_n > INT32_MAX / 8) return iamx::set_error(IAMX_EINVAL, "iamx_gemm: too many tiles");
+    // Tiles run as rounds of `slots` workgroups (two per CU).  The tiles past
+    // the last full round would leave most CUs idle for a whole round, so they
+    // are split into K parts (enough to fill a round, at most 8) whose partial
+    // tiles are summed in order by a second kernel; a product with fewer tiles
+    // than one round is split the same way.  IAMX_GEMM_SPLITK=0 turns this off.
+    static const bool splitk = [] {
+      const char *e = getenv("IAMX_GEMM_SPLITK");
+      return !(e && atoi(e) == 0);
+    }();
+    const int nt = (int)(tiles_m * tiles_n);
+    const int slots = 2 * (iamx::g_num_cus > 0 ? iamx::g_num_cus : 256);
+    const int full = nt / slots * slots, rem = nt - full;
+    const int64_t bk = dtype == 0 ? Slab<double>::BK : Slab<float>::BK;
+    int S = rem > 0 ? slots / rem : 1;
+    if (S > 8) S = 8;
+    int64_t kc = (K + bk - 1) / bk;  // K slabs
+    kc = (kc + S - 1) / S * bk;       // slabs per part, as elements
+    if (kc < 4 * bk) kc = 4 * bk;     // at least four slabs a part
+    S = kc > 0 ? (int)((K + kc - 1) / kc) : 1;
+    const bool split = splitk && rem > 0 && S >= 2 && rem * 4 <= slots * 3;
+    const int head = split ? full : nt;
+    const int64_t isz = dtype == 0 ? 8 : 4;
+    if (head > 0) {
+      const dim3 grid((unsigned)head);
+      if (dtype == 0)
+        IAMX_LAUNCH("gemm_f64_mfma", k_gemm<double>, grid, dim3(256), 0, M, N, K, (const double *)A, lda,
+                    (const double *)B, ldb, (double *)C, ldc, (int)tiles_n, 0, head, 1, K, (double *)nullptr);
+      else
+        IAMX_LAUNCH("gemm_f32_mfma", k_gemm<float>, grid, dim3(256), 0, M, N, K, (const float *)A, lda,
+                    (const float *)B, ldb, (float *)C, ldc, (int)tiles_n, 0, head, 1, K, (float *)nullptr);
+    }
+    if (!split) return IAMX_OK;
+    void *part = iamx_workspace(16, (size_t)S * rem * BM * BN * isz + 256);
+    if (!part) return iamx::set_error(IAMX_ENOMEM, "iamx_gemm: split-K partial tiles");
+    const dim3 pgrid((unsigned)(rem * S)), rgrid((unsigned)(((int64_t)rem * BM * BN + 255) / 256));
+    if (dtype == 0) {
+      IAMX_LAUNCH("gemm_f64_mfma", k_gemm<double>, pgrid, dim3(256), 0, M, N, K, (const double *)A, lda,
+                  (const double *)B, ldb, (double *)C, ldc, (int)tiles_n, full, rem, S, kc, (double *)part);
+      IAMX_LAUNCH("gemm_sum_parts", k_sum_parts<double>, rgrid, dim3(256), 0, (const double *)part, S, rem, full,
+                  (int)tiles_n, M, N, (double *)C, ldc);
+    } else {
+      IAMX_LAUNCH("gemm_f32_mfma", k_gemm<float>, pgrid, dim3(256), 0, M, N, K, (const float *)A, lda,
+                  (const float *)B, ldb, (float *)C, ldc, (int)tiles_n, full, rem, S, kc, (float *)part);
+      IAMX_LAUNCH("gemm_sum_parts", k_sum_parts<float>, rgrid, dim3(256), 0, (const float *)part, S, rem, full,
+                  (int)tiles_n, M, N, (float *)C, ldc);
+    }
+    return IAMX_OK;
+  };
+  if (K == 0 || !iamx::blt_usable()) return own(256);
+  // the autotuner weighs it against hipBLASLt once per shape class
+  auto lib = [&]() -> int {
+    return iamx::blt_gemm(dtype == 0 ? iamx::BLT_F64 : iamx::BLT_F32, 0, M, N, K, A, lda, B, ldb, C, ldc);
+  };
+  auto run = [&](int c) -> int { return c == 0 ? lib() : own(c); };
+  const int cands[2] = {256, 0};
+  int prc = IAMX_OK;
+  iamx::tune_run(dtype == 0 ? 3 : 4, M, N, K, cands, 2, [](int c, void *x) { return (*(decltype(run) *)x)(c); },
+                 &run, &prc);
+  return prc;
 }
 
 // B operand fixed by iamx_gemm_prepare_b for the iamx_gemm_rows calls that follow

@@ -35,6 +35,25 @@ int *status_word();
 int launch_lz4(const char *name, const iamx_lz4_job_t *d_lz4, int n_lz4, void *d_scratch,
                int64_t cap = INT64_MAX);
 
+// Plain dense products through hipBLASLt (iamx_blaslt.hip).  Row-major
+// C[b] (rows x n, ldc) = A[b] (rows x k, lda) B[b], B given as k x n (ldb;
+// bt = 0) or as its n x k transpose (bt = 1); batch products with element
+// strides sa, sb, sc.  blt_usable() is false during a graph capture or when
+// the library cannot start; the autotuners then keep their own kernels.
+enum { BLT_F64 = 0, BLT_F32 = 1, BLT_BF16_F32 = 2, BLT_I8_I32 = 3 };
+bool blt_usable();
+int blt_gemm(int type, int bt, int64_t rows, int64_t n, int64_t k, const void *A, int64_t lda, const void *B,
+             int64_t ldb, void *C, int64_t ldc, int batch = 1, int64_t sa = 0, int64_t sb = 0, int64_t sc = 0);
+void blt_shutdown();
+
+// Autotuner of the dense products (iamx_mixed.hip): the first product of a
+// (form, M, N, K) class runs every candidate in cands (the hand-written
+// kernel's tile, 128 or 256, or 0 for hipBLASLt) through run(candidate,
+// ctx), timed, and the fastest is used for that class from then on.
+// Returns the candidate used; *rc is the status of the product.
+int tune_run(int form, int64_t M, int64_t N, int64_t K, const int *cands, int ncand, int (*run)(int, void *),
+             void *ctx, int *rc);
+
 // Zstd streams: their decode pass (jobs of other codecs are skipped) and
 // whether it runs (switched on once a zstd stream has been planned).
 int launch_zstd(const char *name, const iamx_lz4_job_t *d_jobs, int njobs, void *d_scratch, int64_t cap);

@@ -37,6 +37,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <initializer_list>
+
 #include "iamx_internal.h"
 
 namespace {
@@ -85,11 +87,15 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// Writes hi/lo planes (rows_p x cols_p, zero padded).  trans = 0: plane[r][c] =
-// X[r][c]; trans = 1: plane[r][c] = X[c][r] (B stored transposed).
+// Writes hi/lo planes (rows_p x cols_p, zero padded, row stride ld), hi a
+// second time into hi2.  trans = 0: plane[r][c] = X[r][c]; trans = 1:
+// plane[r][c] = X[c][r] (B stored transposed).  The three planes sit side by
+// side in rows of 3 cols_p, so the three products of the split are one
+// product over K stacked three times.
 __global__ __launch_bounds__(256) void k_split_bf16(const float *__restrict__ X, int64_t xr, int64_t xc, int64_t ldx,
                                                     int trans, uint16_t *__restrict__ hi, uint16_t *__restrict__ lo,
-                                                    int64_t rows_p, int64_t cols_p) {
+                                                    uint16_t *__restrict__ hi2, int64_t rows_p, int64_t cols_p,
+                                                    int64_t ld) {
   __shared__ float t[32][33];
   const int64_t r0 = (int64_t)blockIdx.y * 32, c0 = (int64_t)blockIdx.x * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
@@ -105,8 +111,9 @@ __global__ __launch_bounds__(256) void k_split_bf16(const float *__restrict__ X,
       if (r < rows_p && c < cols_p) {
         const float v = t[tx][i];
         const uint16_t h = f2bf(v);
-        hi[r * cols_p + c] = h;
-        lo[r * cols_p + c] = f2bf(v - bf2f(h));
+        hi[r * ld + c] = h;
+        hi2[r * ld + c] = h;
+        lo[r * ld + c] = f2bf(v - bf2f(h));
       }
     }
   } else {
@@ -115,8 +122,9 @@ __global__ __launch_bounds__(256) void k_split_bf16(const float *__restrict__ X,
       if (r < rows_p && c < cols_p) {
         const float v = (r < xr && c < xc) ? X[r * ldx + c] : 0.f;
         const uint16_t h = f2bf(v);
-        hi[r * cols_p + c] = h;
-        lo[r * cols_p + c] = f2bf(v - bf2f(h));
+        hi[r * ld + c] = h;
+        hi2[r * ld + c] = h;
+        lo[r * ld + c] = f2bf(v - bf2f(h));
       }
     }
   }
@@ -257,6 +265,7 @@ __global__ __launch_bounds__(256) void k_residues_i8(const double *__restrict__ 
 
 struct CrtCombine {
   int32_t n;
+  int32_t m[kMaxMods];               // the moduli
   uint32_t w[kMaxMods][kLimbs - 1];  // W_l in 20-bit limbs, least significant first
   uint32_t mlimb[kLimbs];            // M
   uint32_t half[kLimbs];             // ceil(M / 2)
@@ -275,16 +284,24 @@ __device__ __forceinline__ bool limbs_ge(const uint32_t *r, const uint32_t *s) {
 // C[i][j] = 2^(e_i + f_j - p - q - 2) * C', C' the representative in
 // [-M/2, M/2) of sum_l r_l W_l (mod M), r_l the residue byte of plane l.
 // Everything up to the conversion to FP64 is exact integer arithmetic.
-__global__ __launch_bounds__(256) void k_crt_combine(const uint8_t *__restrict__ R, int64_t plane, int64_t ldr,
+// r_l: a residue byte in [0, m_l), or an exact INT32 product sum reduced here
+__device__ __forceinline__ uint32_t residue(uint8_t v, int32_t) { return v; }
+__device__ __forceinline__ uint32_t residue(int32_t v, int32_t m) {
+  const int32_t r = v % m;
+  return (uint32_t)(r < 0 ? r + m : r);
+}
+
+template <typename RT>
+__global__ __launch_bounds__(256) void k_crt_combine(const RT *__restrict__ R, int64_t plane, int64_t ldr,
                                                      int64_t rows, int64_t cols, const int32_t *__restrict__ ea,
                                                      const int32_t *__restrict__ fb, const CrtCombine P,
                                                      double *__restrict__ C, int64_t ldc) {
   const int64_t i = blockIdx.y, j = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= rows || j >= cols) return;
   uint32_t acc[kLimbs] = {0, 0, 0, 0, 0, 0, 0};
-  const uint8_t *rp = R + i * ldr + j;
+  const RT *rp = R + i * ldr + j;
   for (int l = 0; l < P.n; l++) {
-    const uint32_t r = rp[(int64_t)l * plane];
+    const uint32_t r = residue(rp[(int64_t)l * plane], P.m[l]);
 #pragma unroll
     for (int k = 0; k < kLimbs - 1; k++) acc[k] += __umul24(r, P.w[l][k]);  // < 15 * 255 * 2^20 < 2^32
   }
@@ -755,18 +772,22 @@ __global__ __launch_bounds__(256) void k_zero_rows(T *__restrict__ C, int64_t M,
 
 int64_t rup(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
 
-// ---- tile autotuner ------------------------------------------------------------------------
+// ---- autotuner ------------------------------------------------------------------------------
 // The first product of a given (form, M, N, K) class (each dimension rounded
-// up to a power of two) runs with both tile shapes, timed with HIP events on
-// the library stream; the faster one is kept for that class and, so that
-// every call of a class returns the same bits, re-run once if it was not the
-// last one tried.  IAMX_TILE=128 or 256 pins the choice.
+// up to a power of two) runs with every candidate -- the hand-written MFMA
+// kernel with 128x128 or 256x256 tiles, or hipBLASLt (kLib) -- once untimed,
+// then once timed with HIP events on the library stream.  The fastest is
+// kept for that class and, so that every call of a class returns the same
+// bits, re-run once if it was not the last one timed.  A candidate that
+// fails to run (the library offering no algorithm for the shape) drops out.
+// IAMX_TILE=128, 256 or lib pins the choice where it is a candidate.
+constexpr int kLib = 0;
 struct TuneEntry {
   int form;
   int64_t m, n, k;
-  int tile;
+  int pick;
 };
-static TuneEntry g_tune[64];
+static TuneEntry g_tune[128];
 static int g_ntune = 0;
 
 static int64_t pow2_up(int64_t v) {
@@ -775,61 +796,102 @@ static int64_t pow2_up(int64_t v) {
   return p;
 }
 
-static int forced_tile() {
-  static const int t = [] {
-    const char *e = getenv("IAMX_TILE");
-    const int v = e ? atoi(e) : 0;
-    return (v == 128 || v == 256) ? v : 0;
-  }();
-  return t;
+// read on every product, so a caller can pin a candidate for some products only
+static int forced_pick() {
+  const char *e = getenv("IAMX_TILE");
+  if (!e) return -1;
+  if (!strcmp(e, "lib")) return kLib;
+  const int v = atoi(e);
+  return (v == 128 || v == 256) ? v : -1;
 }
 
-// run(tile) enqueues the product with the given tile shape and returns an
-// IAMX status; returns the tile used.
-template <typename F>
-static int tuned(int form, int64_t M, int64_t N, int64_t K, F run, int *rc) {
-  if (forced_tile()) {
-    *rc = run(forced_tile());
-    return forced_tile();
+}  // namespace
+
+int iamx::tune_run(int form, int64_t M, int64_t N, int64_t K, const int *cands, int ncand, int (*run)(int, void *),
+                   void *ctx, int *rc) {
+  int c[4], nc = 0;
+  for (int i = 0; i < ncand && nc < 4; i++)
+    if (cands[i] != kLib || blt_usable()) c[nc++] = cands[i];
+  if (nc == 0) {
+    *rc = set_error(IAMX_EINVAL, "autotuner: no candidate");
+    return -1;
   }
-  if (M < BM || N < BN) {  // a 256-tile launch would be mostly padding
-    *rc = run(128);
-    return 128;
+  const int f = forced_pick();
+  for (int i = 0; i < nc; i++)
+    if (c[i] == f) {
+      *rc = run(f, ctx);
+      return f;
+    }
+  if (nc == 1) {
+    *rc = run(c[0], ctx);
+    return c[0];
   }
   const int64_t m = pow2_up(M), n = pow2_up(N), k = pow2_up(K);
   for (int i = 0; i < g_ntune; i++)
     if (g_tune[i].form == form && g_tune[i].m == m && g_tune[i].n == n && g_tune[i].k == k) {
-      *rc = run(g_tune[i].tile);
-      return g_tune[i].tile;
+      *rc = run(g_tune[i].pick, ctx);
+      return g_tune[i].pick;
     }
-  if (iamx::capturing()) {  // no timing inside a graph capture
-    *rc = run(256);
-    return 256;
+  if (capturing()) {  // no timing inside a graph capture: the first hand-written candidate
+    for (int i = 0; i < nc; i++)
+      if (c[i] != kLib) {
+        *rc = run(c[i], ctx);
+        return c[i];
+      }
+    *rc = run(c[0], ctx);
+    return c[0];
   }
-  hipEvent_t ev[3];
-  for (auto &e : ev)
-    if (hipEventCreate(&e) != hipSuccess) {
-      *rc = run(256);
-      return 256;
+  hipEvent_t ev[5];
+  int made = 0;
+  for (; made < nc + 1; made++)
+    if (hipEventCreate(&ev[made]) != hipSuccess) break;
+  if (made < nc + 1) {
+    for (int i = 0; i < made; i++) (void)hipEventDestroy(ev[i]);
+    *rc = run(c[0], ctx);
+    return c[0];
+  }
+  // one untimed run each first: the library loads its kernels and picks its
+  // algorithm on its first call of a shape, which must not count
+  for (int i = 0; i < nc; i++) (void)run(c[i], ctx);
+  int ok[4] = {0, 0, 0, 0}, last = -1;
+  (void)hipEventRecord(ev[0], g_cur);
+  for (int i = 0; i < nc; i++) {
+    ok[i] = run(c[i], ctx) == IAMX_OK;
+    if (ok[i]) last = i;
+    (void)hipEventRecord(ev[i + 1], g_cur);
+  }
+  int best = -1;
+  float best_ms = 0.f;
+  if (last >= 0 && hipEventSynchronize(ev[nc]) == hipSuccess) {
+    for (int i = 0; i < nc; i++) {
+      float ms = 0.f;
+      if (!ok[i] || hipEventElapsedTime(&ms, ev[i], ev[i + 1]) != hipSuccess) continue;
+      if (best < 0 || ms < best_ms) {
+        best = i;
+        best_ms = ms;
+      }
     }
-  const int tiles[2] = {128, 256};
-  float ms[2] = {0, 0};
-  *rc = IAMX_OK;
-  (void)hipEventRecord(ev[0], iamx::g_cur);
-  for (int t = 0; t < 2 && !*rc; t++) {
-    *rc = run(tiles[t]);
-    (void)hipEventRecord(ev[t + 1], iamx::g_cur);
   }
-  int best = 256;
-  if (!*rc && hipEventSynchronize(ev[2]) == hipSuccess) {
-    (void)hipEventElapsedTime(&ms[0], ev[0], ev[1]);
-    (void)hipEventElapsedTime(&ms[1], ev[1], ev[2]);
-    best = ms[0] < ms[1] ? 128 : 256;
-    if (best != 256) *rc = run(best);
+  for (int i = 0; i < nc + 1; i++) (void)hipEventDestroy(ev[i]);
+  if (best < 0) {
+    *rc = last < 0 ? run(c[0], ctx) : IAMX_OK;  // the error of the first candidate, or the last that ran
+    return last < 0 ? c[0] : c[last];
   }
-  for (auto &e : ev) (void)hipEventDestroy(e);
-  if (!*rc && g_ntune < 64) g_tune[g_ntune++] = TuneEntry{form, m, n, k, best};
-  return best;
+  *rc = best != last ? run(c[best], ctx) : IAMX_OK;
+  if (!*rc && g_ntune < 128) g_tune[g_ntune++] = TuneEntry{form, m, n, k, c[best]};
+  return c[best];
+}
+
+namespace {
+
+// run(candidate) enqueues the product and returns an IAMX status; returns
+// the candidate used.
+template <typename F>
+static int tuned(int form, int64_t M, int64_t N, int64_t K, std::initializer_list<int> cands, F run, int *rc) {
+  const int list[4] = {cands.size() > 0 ? cands.begin()[0] : 0, cands.size() > 1 ? cands.begin()[1] : 0,
+                       cands.size() > 2 ? cands.begin()[2] : 0, cands.size() > 3 ? cands.begin()[3] : 0};
+  return iamx::tune_run(form, M, N, K, list, (int)cands.size(), [](int c, void *x) { return (*(F *)x)(c); },
+                        &run, rc);
 }
 
 // ---- prepared B operand and row panels -------------------------------------------------------
@@ -904,6 +966,7 @@ static CrtCombine crt_combine_consts(int n, int p, int q) {
   to_limbs(M, c.mlimb, kLimbs);
   to_limbs(M / 2, c.half, kLimbs);
   c.inv_m = 1.0 / (double)M;
+  for (int l = 0; l < n; l++) c.m[l] = kMods[l];
   c.p1 = p + 1;
   c.q1 = q + 1;
   return c;
@@ -916,10 +979,12 @@ static int mixed_prepare_b(PrepB &pb, int slot, int form, int64_t N, int64_t K, 
   if (form == 1) {
     const int64_t Kp = rup(K, KB / 2);
     const size_t b_bytes = (size_t)(Np * Kp * 2);
-    uint8_t *ws = (uint8_t *)iamx_workspace(slot, 2 * b_bytes + 256);
+    uint8_t *ws = (uint8_t *)iamx_workspace(slot, 3 * b_bytes + 256);
     if (!ws) return iamx::set_error(IAMX_ENOMEM, "split-bf16 workspace");
+    // rows of B^T: [hi | lo | hi]
+    uint16_t *w16 = (uint16_t *)ws;
     IAMX_LAUNCH("split_bf16_B", k_split_bf16, dim3((unsigned)(Kp / 32), (unsigned)(Np / 32)), dim3(256), 0,
-                (const float *)B, K, N, ldb, 1, (uint16_t *)ws, (uint16_t *)(ws + b_bytes), Np, Kp);
+                (const float *)B, K, N, ldb, 1, w16, w16 + Kp, w16 + 2 * Kp, Np, Kp, 3 * Kp);
     pb = PrepB{1, N, K, Np, Kp, 0, ws, nullptr, true, 0, 0};
     return IAMX_OK;
   }
@@ -986,21 +1051,28 @@ static int mixed_rows(const PrepB &pb, int form, int64_t M, const void *A, int64
   P.ldc = ldc;
   int rc = IAMX_OK;
   if (form == 1) {
-    const size_t a_bytes = (size_t)(Mp * Kp * 2), b_bytes = (size_t)(Np * Kp * 2);
-    uint8_t *ws = (uint8_t *)iamx_workspace(11, 2 * a_bytes + 256);
+    const size_t a_bytes = (size_t)(Mp * Kp * 2);
+    uint8_t *ws = (uint8_t *)iamx_workspace(11, 3 * a_bytes + 256);
     if (!ws) return iamx::set_error(IAMX_ENOMEM, "split-bf16 workspace");
+    // rows of A: [lo | hi | hi] against B^T's [hi | lo | hi]: the three
+    // products, small ones first (lo*hi, hi*lo, then hi*hi), are one product
+    // over K stacked three times
+    uint16_t *w16 = (uint16_t *)ws;
     IAMX_LAUNCH("split_bf16_A", k_split_bf16, dim3((unsigned)(Kp / 32), (unsigned)(Mp / 32)), dim3(256), 0,
-                (const float *)A, M, K, lda, 0, (uint16_t *)ws, (uint16_t *)(ws + a_bytes), Mp, Kp);
+                (const float *)A, M, K, lda, 0, w16 + Kp, w16, w16 + 2 * Kp, Mp, Kp, 3 * Kp);
     P.A = ws;
-    P.lda = Kp * 2;
-    P.ldb = Kp * 2;
+    P.lda = 3 * Kp * 2;
+    P.ldb = 3 * Kp * 2;
     P.Kb = Kp * 2;
-    P.nseg = 3;  // small products first: lo*hi, hi*lo, then hi*hi
-    P.seg[0] = Seg{(int64_t)a_bytes, 0, 0, 0};
-    P.seg[1] = Seg{0, (int64_t)b_bytes, 0, 0};
-    P.seg[2] = Seg{0, 0, 0, 0};
-    tuned(0, M, N, K,
+    P.nseg = 3;
+    P.seg[0] = Seg{0, 0, 0, 0};
+    P.seg[1] = Seg{Kp * 2, Kp * 2, 0, 0};
+    P.seg[2] = Seg{2 * Kp * 2, 2 * Kp * 2, 0, 0};
+    const bool small = M < BM || N < BN;  // a 256-tile launch would be mostly padding
+    tuned(0, M, N, K, small ? std::initializer_list<int>{128, kLib} : std::initializer_list<int>{256, 128, kLib},
           [&](int tile) -> int {
+            if (tile == kLib)
+              return iamx::blt_gemm(iamx::BLT_BF16_F32, 1, M, N, 3 * Kp, ws, 3 * Kp, pb.planes, 3 * Kp, C, ldc);
             if (tile == 256) {
               P.tiles_n = (int)(Np / BN);
               IAMX_LAUNCH("gemm_split_bf16_mfma", k_seg_gemm_big<0>, dim3((unsigned)((Mp / BM) * (Np / BN))),
@@ -1017,7 +1089,8 @@ static int mixed_rows(const PrepB &pb, int form, int64_t M, const void *A, int64
   }
   if (form == 3) {
     const int n = pb.S;
-    const size_t a_bytes = (size_t)(n * Mp * Kp), r_bytes = (size_t)(n * Mp * Np);
+    // residue planes: bytes from the hand-written kernel, INT32 sums from hipBLASLt
+    const size_t a_bytes = (size_t)(n * Mp * Kp), r_bytes = (size_t)(n * Mp * Np) * (iamx::blt_usable() ? 4 : 1);
     uint8_t *ws = (uint8_t *)iamx_workspace(11, a_bytes + (size_t)Mp * 4 + r_bytes + 512);
     if (!ws) return iamx::set_error(IAMX_ENOMEM, "ozaki-crt workspace");
     int32_t *ea = (int32_t *)(ws + a_bytes);
@@ -1039,12 +1112,27 @@ static int mixed_rows(const PrepB &pb, int form, int64_t M, const void *A, int64
     P.C = R;
     P.ldc = Np;
     P.tiles_n = (int)(Np / BN);
-    IAMX_LAUNCH("gemm_ozaki_crt_i8_mfma", k_seg_gemm_big<2>, dim3((unsigned)((Mp / BM) * (Np / BN) * n)), dim3(256),
-                0, P, 0, 0, (int)(Mp / BM));
     const CrtCombine comb = crt_combine_consts(n, pb.p, pb.q);
-    IAMX_LAUNCH("ozaki_crt_combine", k_crt_combine, dim3((unsigned)((N + 255) / 256), (unsigned)M), dim3(256), 0,
-                (const uint8_t *)R, Mp * Np, Np, M, N, (const int32_t *)ea, pb.fb, comb, (double *)C, ldc);
-    return IAMX_OK;
+    const dim3 cgrid((unsigned)((N + 255) / 256), (unsigned)M);
+    // both candidates give the exact residues, so the same bits
+    tuned(2, M, N, K * n, {256, kLib},
+          [&](int tile) -> int {
+            if (tile == kLib) {
+              const int e = iamx::blt_gemm(iamx::BLT_I8_I32, 1, M, N, Kp, ws, Kp, pb.planes, Kp, R, Np, n, Mp * Kp,
+                                           Np * Kp, Mp * Np);
+              if (e) return e;
+              IAMX_LAUNCH("ozaki_crt_combine", k_crt_combine<int32_t>, cgrid, dim3(256), 0, (const int32_t *)R,
+                          Mp * Np, Np, M, N, (const int32_t *)ea, pb.fb, comb, (double *)C, ldc);
+              return IAMX_OK;
+            }
+            IAMX_LAUNCH("gemm_ozaki_crt_i8_mfma", k_seg_gemm_big<2>, dim3((unsigned)((Mp / BM) * (Np / BN) * n)),
+                        dim3(256), 0, P, 0, 0, (int)(Mp / BM));
+            IAMX_LAUNCH("ozaki_crt_combine", k_crt_combine<uint8_t>, cgrid, dim3(256), 0, (const uint8_t *)R, Mp * Np,
+                        Np, M, N, (const int32_t *)ea, pb.fb, comb, (double *)C, ldc);
+            return IAMX_OK;
+          },
+          &rc);
+    return rc;
   }
   const int S = pb.S;
   const size_t a_bytes = (size_t)(S * Mp * Kp);
@@ -1063,7 +1151,8 @@ static int mixed_rows(const PrepB &pb, int form, int64_t M, const void *A, int64
   P.fb = pb.fb;
   // levels are added from the least significant (S+1) up to the most
   // significant (2), so small contributions are not lost to FP64 rounding first
-  tuned(1, M, N, K * S,
+  const bool small = M < BM || N < BN;
+  tuned(1, M, N, K * S, small ? std::initializer_list<int>{128} : std::initializer_list<int>{256, 128},
         [&](int tile) -> int {
           if (tile == 256) {
             P.tiles_n = (int)(Np / BN);

@@ -485,6 +485,7 @@ IAMX_API void *iamx_workspace(int slot, size_t bytes) {
 IAMX_API void iamx_shutdown(void) {
   if (g_init_state != 1) return;
   sync_streams();
+  blt_shutdown();
   for (auto &w : g_ws) {
     if (w.ptr) (void)hipFree(w.ptr);
     w = Workspace{};
