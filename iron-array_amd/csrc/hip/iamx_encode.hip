@@ -3,11 +3,11 @@
 // for every C chunk of the gemm branch (src/linalg/matmul/gemm.c:216-243) and
 // of iarray_opt_gemm (src/linalg/iarray_gemm.c:399-425).
 //
-// k_lz4_encode_sub one wavefront per 4 KiB sub-block of a stream, staged in
+// k_lz4_encode_sub one wavefront per 2 KiB sub-block of a stream, staged in
 //                  LDS: a greedy LZ77 parse whose matches start and end inside
 //                  the sub-block.  Candidates are found 64 positions at a
 //                  time: each lane hashes the 4 bytes at its position and
-//                  reads the hash table (LDS, 2^10 entries holding a position
+//                  reads the hash table (LDS, 2^9 entries holding a position
 //                  + 1); when the 4 bytes there are equal to its own it also
 //                  compares the next 16 bytes at both places.
 //                  Then every lane inserts its position with an atomic max,
@@ -93,8 +93,12 @@ __device__ __forceinline__ int32_t ext_bytes(int32_t v) { return v / 255 + 1; }
 // written, where its trailing literals start, whether any byte is nonzero,
 // and overflow.  The merge joins each sub-block's leading literals with the
 // previous one's trailing literals into one run.
-constexpr int kSub = 4096;
-constexpr int kSubHlog = 10;
+// 2 KiB sub-blocks: stage and hash table take 4.1 KB of LDS, so 8 wavefronts
+// per SIMD are resident (4 KiB ones with 2^10 entries took 8.3 KB, 5 per
+// SIMD, and each launch twice as long; their C was 1% smaller for float32
+// products, 0.1% for float64)
+constexpr int kSub = 2048;
+constexpr int kSubHlog = 9;
 constexpr int kSubSlot = kSub + 256;  // a sub-block's worst output, rounded
 constexpr int kInfo = 8;              // int32 per info record
 constexpr int kGiveUp = 1024;         // bytes scanned without a match before a sub-block stays literal
