@@ -122,7 +122,7 @@ BltPlan *plan_for(const BltKey &key) {
 bool blt_usable() { return !capturing() && ready(); }
 
 int blt_gemm(int type, int bt, int64_t rows, int64_t n, int64_t k, const void *A, int64_t lda, const void *B,
-             int64_t ldb, void *C, int64_t ldc, int batch, int64_t sa, int64_t sb, int64_t sc) {
+             int64_t ldb, void *C, int64_t ldc, int batch, int64_t sa, int64_t sb, int64_t sc, const char *name) {
   if (!blt_usable()) return set_error(IAMX_EINVAL, "hipBLASLt product unavailable");
   if (rows <= 0 || n <= 0 || k <= 0 || batch < 1) return set_error(IAMX_EINVAL, "hipBLASLt product: bad shape");
   void *ws = iamx_workspace(IAMX_BLT_WORKSPACE, kWsBytes);
@@ -138,8 +138,11 @@ int blt_gemm(int type, int bt, int64_t rows, int64_t n, int64_t k, const void *A
   const void *beta = type == BLT_F64 ? (const void *)&zero_d : type == BLT_I8_I32 ? (const void *)&zero_i
                                                                                   : (const void *)&zero_f;
   ++g_launch_seq;
-  ProfScope ps(type == BLT_F64 ? "gemm_f64_hipblaslt" : type == BLT_F32 ? "gemm_f32_hipblaslt"
-               : type == BLT_BF16_F32 ? "gemm_split_bf16_hipblaslt" : "gemm_ozaki_crt_i8_hipblaslt");
+  ProfScope ps(name ? name
+               : type == BLT_F64 ? "gemm_f64_hipblaslt"
+               : type == BLT_F32 ? "gemm_f32_hipblaslt"
+               : type == BLT_BF16_F32 ? "gemm_split_bf16_hipblaslt"
+                                      : "gemm_i8_hipblaslt");
   const hipblasStatus_t s = hipblasLtMatmul(g_h, p->op, alpha, B, p->la, A, p->lb, beta, C, p->lc, C, p->lc,
                                             &p->algo, ws, kWsBytes, g_cur);
   if (s != HIPBLAS_STATUS_SUCCESS) return set_error(IAMX_EHIP, "hipblasLtMatmul failed (%d)", (int)s);

@@ -42,8 +42,10 @@ int launch_lz4(const char *name, const iamx_lz4_job_t *d_lz4, int n_lz4, void *d
 // the library cannot start; the autotuners then keep their own kernels.
 enum { BLT_F64 = 0, BLT_F32 = 1, BLT_BF16_F32 = 2, BLT_I8_I32 = 3 };
 bool blt_usable();
+// `name` labels the launch in the profile (default: by type).
 int blt_gemm(int type, int bt, int64_t rows, int64_t n, int64_t k, const void *A, int64_t lda, const void *B,
-             int64_t ldb, void *C, int64_t ldc, int batch = 1, int64_t sa = 0, int64_t sb = 0, int64_t sc = 0);
+             int64_t ldb, void *C, int64_t ldc, int batch = 1, int64_t sa = 0, int64_t sb = 0, int64_t sc = 0,
+             const char *name = nullptr);
 void blt_shutdown();
 
 // Autotuner of the dense products (iamx_mixed.hip): the first product of a

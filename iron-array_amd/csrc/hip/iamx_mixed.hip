@@ -180,11 +180,14 @@ __global__ __launch_bounds__(256) void k_col_exp_zero(int32_t *__restrict__ e, i
   if (i < n && e[i] == INT32_MIN) e[i] = 0;
 }
 
-// S digit planes, each rows_p x cols_p int8.  trans as in k_split_bf16; the
-// scale exponent is per plane row (e[r]).
+// S digit planes of rows_p x cols_p int8, side by side in rows of S cols_p:
+// digit s (0 = most significant) at column block s, or S-1-s when rev (B,
+// so that the slice pairs (s, L-s) of a level L sit at equal offsets of two
+// contiguous column ranges: one product over K stacked n_L times).  trans as
+// in k_split_bf16; the scale exponent is per plane row (e[r]).
 __global__ __launch_bounds__(256) void k_slice_i8(const double *__restrict__ X, int64_t xr, int64_t xc, int64_t ldx,
                                                   int trans, const int32_t *__restrict__ e, int S,
-                                                  int8_t *__restrict__ planes, int64_t rows_p, int64_t cols_p) {
+                                                  int8_t *__restrict__ planes, int64_t rows_p, int64_t cols_p, int rev) {
   __shared__ double t[32][33];
   const int64_t r0 = (int64_t)blockIdx.y * 32, c0 = (int64_t)blockIdx.x * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
@@ -207,9 +210,23 @@ __global__ __launch_bounds__(256) void k_slice_i8(const double *__restrict__ X, 
       q *= 128.0;                // exact
       const double d = rint(q);  // |d| <= 64
       q -= d;                    // exact, |q| <= 1/2
-      planes[(int64_t)s * rows_p * cols_p + r * cols_p + c] = (int8_t)d;
+      planes[r * S * cols_p + (int64_t)(rev ? S - 1 - s : s) * cols_p + c] = (int8_t)d;
     }
   }
+}
+
+// One Ozaki level from its exact INT32 sums (the library's product): the
+// epilogue of k_seg_gemm_big<1>, C (+)= 2^(e_i + f_j - 7 L) acc, so the same
+// bits in the same level order.
+__global__ __launch_bounds__(256) void k_ozaki_level(const int32_t *__restrict__ acc, int64_t lda, int64_t rows,
+                                                     int64_t cols, const int32_t *__restrict__ ea,
+                                                     const int32_t *__restrict__ fb, int level, int accumulate,
+                                                     double *__restrict__ C, int64_t ldc) {
+  const int64_t i = blockIdx.y, j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= rows || j >= cols) return;
+  const double t = ldexp((double)acc[i * lda + j], ea[i] + fb[j] - 7 * level);
+  double *c = C + i * ldc + j;
+  *c = accumulate ? *c + t : t;
 }
 
 // ---- Ozaki CRT prep and combine ------------------------------------------------------------
@@ -829,8 +846,13 @@ int iamx::tune_run(int form, int64_t M, int64_t N, int64_t K, const int *cands, 
   const int64_t m = pow2_up(M), n = pow2_up(N), k = pow2_up(K);
   for (int i = 0; i < g_ntune; i++)
     if (g_tune[i].form == form && g_tune[i].m == m && g_tune[i].n == n && g_tune[i].k == k) {
-      *rc = run(g_tune[i].pick, ctx);
-      return g_tune[i].pick;
+      // the class's pick among the candidates of this call (the library is
+      // none under a capture; a class may hold picks of two candidate lists)
+      for (int j = 0; j < nc; j++)
+        if (c[j] == g_tune[i].pick) {
+          *rc = run(c[j], ctx);
+          return c[j];
+        }
     }
   if (capturing()) {  // no timing inside a graph capture: the first hand-written candidate
     for (int i = 0; i < nc; i++)
@@ -1023,7 +1045,7 @@ static int mixed_prepare_b(PrepB &pb, int slot, int form, int64_t N, int64_t K, 
               dim3(256), 0, Bd, K, N, ldb, fb);
   IAMX_LAUNCH("ozaki_col_exp", k_col_exp_zero, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, fb, N);
   IAMX_LAUNCH("ozaki_slice_B", k_slice_i8, dim3((unsigned)(Kp / 32), (unsigned)(Np / 32)), dim3(256), 0, Bd, K, N,
-              ldb, 1, fb, S, (int8_t *)ws, Np, Kp);
+              ldb, 1, fb, S, (int8_t *)ws, Np, Kp, 1);
   pb = PrepB{2, N, K, Np, Kp, S, ws, fb, true, 0, 0};
   return IAMX_OK;
 }
@@ -1119,7 +1141,7 @@ static int mixed_rows(const PrepB &pb, int form, int64_t M, const void *A, int64
           [&](int tile) -> int {
             if (tile == kLib) {
               const int e = iamx::blt_gemm(iamx::BLT_I8_I32, 1, M, N, Kp, ws, Kp, pb.planes, Kp, R, Np, n, Mp * Kp,
-                                           Np * Kp, Mp * Np);
+                                           Np * Kp, Mp * Np, "gemm_ozaki_crt_i8_hipblaslt");
               if (e) return e;
               IAMX_LAUNCH("ozaki_crt_combine", k_crt_combine<int32_t>, cgrid, dim3(256), 0, (const int32_t *)R,
                           Mp * Np, Np, M, N, (const int32_t *)ea, pb.fb, comb, (double *)C, ldc);
@@ -1136,24 +1158,43 @@ static int mixed_rows(const PrepB &pb, int form, int64_t M, const void *A, int64
   }
   const int S = pb.S;
   const size_t a_bytes = (size_t)(S * Mp * Kp);
-  uint8_t *ws = (uint8_t *)iamx_workspace(11, a_bytes + (size_t)Mp * 4 + 256);
+  // a level's INT32 sums when the library runs it
+  const size_t l_bytes = iamx::blt_usable() ? (size_t)M * (size_t)N * 4 : 0;
+  uint8_t *ws = (uint8_t *)iamx_workspace(11, a_bytes + (size_t)Mp * 4 + 256 + l_bytes + 256);
   if (!ws) return iamx::set_error(IAMX_ENOMEM, "ozaki workspace");
   int32_t *ea = (int32_t *)(ws + a_bytes);
+  int32_t *lacc = (int32_t *)(ws + rup((int64_t)(a_bytes + (size_t)Mp * 4), 256));
   const double *Ad = (const double *)A;
   IAMX_LAUNCH("ozaki_row_exp", k_row_exp, dim3((unsigned)((M + 3) / 4)), dim3(256), 0, Ad, M, K, lda, ea);
   IAMX_LAUNCH("ozaki_slice_A", k_slice_i8, dim3((unsigned)(Kp / 32), (unsigned)(Mp / 32)), dim3(256), 0, Ad, M, K,
-              lda, 0, ea, S, (int8_t *)ws, Mp, Kp);
+              lda, 0, ea, S, (int8_t *)ws, Mp, Kp, 0);
+  // rows of A hold digits 1..S, rows of B^T digits S..1: the pair (s, t) of
+  // a level sits at A column block s-1 and B column block S-t
   P.A = ws;
-  P.lda = Kp;
-  P.ldb = Kp;
+  P.lda = S * Kp;
+  P.ldb = S * Kp;
   P.Kb = Kp;
   P.ea = ea;
   P.fb = pb.fb;
   // levels are added from the least significant (S+1) up to the most
   // significant (2), so small contributions are not lost to FP64 rounding first
   const bool small = M < BM || N < BN;
-  tuned(1, M, N, K * S, small ? std::initializer_list<int>{128} : std::initializer_list<int>{256, 128},
+  const dim3 lgrid((unsigned)((N + 255) / 256), (unsigned)M);
+  tuned(1, M, N, K * S, small ? std::initializer_list<int>{128, kLib} : std::initializer_list<int>{256, 128, kLib},
         [&](int tile) -> int {
+          if (tile == kLib) {
+            for (int L = S + 1; L >= 2; L--) {
+              const int s_lo = L - S > 1 ? L - S : 1, s_hi = L - 1 < S ? L - 1 : S;
+              const int e = iamx::blt_gemm(iamx::BLT_I8_I32, 1, M, N, (int64_t)(s_hi - s_lo + 1) * Kp,
+                                           ws + (int64_t)(s_lo - 1) * Kp, S * Kp,
+                                           pb.planes + (int64_t)(S - (L - s_lo)) * Kp, S * Kp, lacc, N, 1, 0, 0, 0,
+                                           "gemm_ozaki_i8_hipblaslt");
+              if (e) return e;
+              IAMX_LAUNCH("ozaki_level", k_ozaki_level, lgrid, dim3(256), 0, (const int32_t *)lacc, N, M, N,
+                          (const int32_t *)ea, pb.fb, L, L != S + 1 ? 1 : 0, (double *)C, ldc);
+            }
+            return IAMX_OK;
+          }
           if (tile == 256) {
             P.tiles_n = (int)(Np / BN);
             for (int L = S + 1; L >= 2; L--) {
@@ -1161,7 +1202,7 @@ static int mixed_rows(const PrepB &pb, int form, int64_t M, const void *A, int64
               for (int s = 1; s <= S; s++) {
                 const int t = L - s;
                 if (t < 1 || t > S) continue;
-                P.seg[n++] = Seg{(int64_t)(s - 1) * Mp * Kp, (int64_t)(t - 1) * Np * Kp, L, 0};
+                P.seg[n++] = Seg{(int64_t)(s - 1) * Kp, (int64_t)(S - t) * Kp, L, 0};
               }
               P.nseg = n;
               IAMX_LAUNCH("gemm_ozaki_i8_mfma", k_seg_gemm_big<1>, dim3((unsigned)((Mp / BM) * (Np / BN))),
@@ -1174,7 +1215,7 @@ static int mixed_rows(const PrepB &pb, int form, int64_t M, const void *A, int64
               for (int s = 1; s <= S; s++) {
                 const int t = L - s;
                 if (t < 1 || t > S) continue;
-                P.seg[n++] = Seg{(int64_t)(s - 1) * Mp * Kp, (int64_t)(t - 1) * Np * Kp, L, 0};
+                P.seg[n++] = Seg{(int64_t)(s - 1) * Kp, (int64_t)(S - t) * Kp, L, 0};
               }
               P.seg[n - 1].level_end = 1;
             }

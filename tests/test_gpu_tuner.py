@@ -5,8 +5,9 @@ pins one.  Every candidate of every form is checked here against the FP64
 double-double oracle (oracle/tiling_linalg.c orc_dd_gemm_sample), and the
 profile names the path that ran.  Bars (componentwise max |C - C*| / (|A||B|)):
 plain float32 1e-5, split-BF16 2e-6, plain float64 and Ozaki-II (14 moduli)
-1e-13.  The Ozaki-II residue products are exact integers, so the hand-written
-kernel and the library give the same bits."""
+1e-13, Ozaki-I (7 slices) 1e-12.  The Ozaki products are exact integers (the
+residues, or each level's digit products) recombined the same way, so the
+hand-written kernels and the library give the same bits."""
 import os
 
 import numpy as np
@@ -21,6 +22,7 @@ FORMS = {
     "split_bf16": (np.float32, ia.SPLIT_BF16, 0, 2e-6, ["256", "128", "lib"]),
     "f64": (np.float64, ia.PLAIN, 0, 1e-13, ["256", "lib"]),
     "ozaki_crt": (np.float64, ia.OZAKI_CRT, 14, 1e-13, ["256", "lib"]),
+    "ozaki_int8": (np.float64, ia.OZAKI_INT8, 7, 1e-12, ["256", "128", "lib"]),
 }
 
 
@@ -56,7 +58,7 @@ def test_every_candidate_matches_the_oracle(orc, form):
     dt, _, _, bar, picks = FORMS[form]
     rng = np.random.default_rng(len(form))
     M, K, N = 600, 900, 520
-    if dt == np.float64 and form == "ozaki_crt":
+    if form.startswith("ozaki"):
         a = np.where(rng.random((M, K)) < 0.5, -1.0, 1.0) * 10.0 ** rng.uniform(-3, 3, (M, K))
         b = np.where(rng.random((K, N)) < 0.5, -1.0, 1.0) * 10.0 ** rng.uniform(-3, 3, (K, N))
     else:
@@ -72,8 +74,9 @@ def test_every_candidate_matches_the_oracle(orc, form):
         print(f"{form} {pick}: componentwise {cw:.3e}", flush=True)
         assert cw <= bar
         out[pick] = c
-    if form == "ozaki_crt":
-        assert out["lib"].tobytes() == out["256"].tobytes()
+    if form.startswith("ozaki"):
+        for pick in picks:
+            assert out[pick].tobytes() == out["256"].tobytes(), pick
 
 
 def test_library_product_repeats_its_bits():
