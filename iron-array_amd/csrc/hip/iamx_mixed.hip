@@ -188,29 +188,40 @@ __global__ __launch_bounds__(256) void k_col_exp_zero(int32_t *__restrict__ e, i
 __global__ __launch_bounds__(256) void k_slice_i8(const double *__restrict__ X, int64_t xr, int64_t xc, int64_t ldx,
                                                   int trans, const int32_t *__restrict__ e, int S,
                                                   int8_t *__restrict__ planes, int64_t rows_p, int64_t cols_p, int rev) {
-  __shared__ double t[32][33];
-  const int64_t r0 = (int64_t)blockIdx.y * 32, c0 = (int64_t)blockIdx.x * 32;
+  // a workgroup makes 32 plane rows x 128 plane columns (rows_p a multiple
+  // of 32, cols_p of 128); a thread 4 consecutive columns of 4 rows, stored
+  // as one dword per digit
+  __shared__ double t[128][33];
+  const int64_t r0 = (int64_t)blockIdx.y * 32, c0 = (int64_t)blockIdx.x * 128;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
   if (trans) {
-    for (int i = ty; i < 32; i += 8) {
+    for (int i = ty; i < 128; i += 8) {
       const int64_t xrow = c0 + i, xcol = r0 + tx;
       t[i][tx] = (xrow < xr && xcol < xc) ? X[xrow * ldx + xcol] : 0.0;
     }
     __syncthreads();
   }
+  const int64_t c = c0 + 4 * tx;
   for (int i = ty; i < 32; i += 8) {
-    const int64_t r = r0 + i, c = c0 + tx;
+    const int64_t r = r0 + i;
     if (r >= rows_p || c >= cols_p) continue;
-    double v;
-    if (trans) v = t[tx][i];
-    else v = (r < xr && c < xc) ? X[r * ldx + c] : 0.0;
     const int er = r < (trans ? xc : xr) ? e[r] : 0;
-    double q = ldexp(v, -er);  // |q| < 1/2, exact
+    double q[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const double v = trans ? t[4 * tx + k][i] : ((r < xr && c + k < xc) ? X[r * ldx + c + k] : 0.0);
+      q[k] = ldexp(v, -er);  // |q| < 1/2, exact
+    }
     for (int s = 0; s < S; s++) {
-      q *= 128.0;                // exact
-      const double d = rint(q);  // |d| <= 64
-      q -= d;                    // exact, |q| <= 1/2
-      planes[r * S * cols_p + (int64_t)(rev ? S - 1 - s : s) * cols_p + c] = (int8_t)d;
+      uint32_t word = 0;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        q[k] *= 128.0;                // exact
+        const double d = rint(q[k]);  // |d| <= 64
+        q[k] -= d;                    // exact, |q| <= 1/2
+        word |= (uint32_t)(uint8_t)(int8_t)(int)d << (8 * k);
+      }
+      *reinterpret_cast<uint32_t *>(planes + r * S * cols_p + (int64_t)(rev ? S - 1 - s : s) * cols_p + c) = word;
     }
   }
 }
@@ -250,32 +261,43 @@ __global__ __launch_bounds__(256) void k_residues_i8(const double *__restrict__ 
                                                      int64_t ldx, int trans, const int32_t *__restrict__ e, int scale,
                                                      const CrtMods mods, int8_t *__restrict__ planes, int64_t rows_p,
                                                      int64_t cols_p) {
-  __shared__ double t[32][33];
-  const int64_t r0 = (int64_t)blockIdx.y * 32, c0 = (int64_t)blockIdx.x * 32;
+  // a workgroup makes 32 plane rows x 128 plane columns (rows_p a multiple
+  // of 32, cols_p of 128); a thread 4 consecutive columns of 4 rows, stored
+  // as one dword per modulus
+  __shared__ double t[128][33];
+  const int64_t r0 = (int64_t)blockIdx.y * 32, c0 = (int64_t)blockIdx.x * 128;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
   if (trans) {
-    for (int i = ty; i < 32; i += 8) {
+    for (int i = ty; i < 128; i += 8) {
       const int64_t xrow = c0 + i, xcol = r0 + tx;
       t[i][tx] = (xrow < xr && xcol < xc) ? X[xrow * ldx + xcol] : 0.0;
     }
     __syncthreads();
   }
+  const int64_t plane = rows_p * cols_p;
+  const int64_t c = c0 + 4 * tx;
   for (int i = ty; i < 32; i += 8) {
-    const int64_t r = r0 + i, c = c0 + tx;
+    const int64_t r = r0 + i;
     if (r >= rows_p || c >= cols_p) continue;
-    double v;
-    if (trans) v = t[tx][i];
-    else v = (r < xr && c < xc) ? X[r * ldx + c] : 0.0;
     const int er = r < (trans ? xc : xr) ? e[r] : 0;
-    const double a = rint(ldexp(v, scale - er));
-    const int64_t plane = rows_p * cols_p;
+    double a[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const double v = trans ? t[4 * tx + k][i] : ((r < xr && c + k < xc) ? X[r * ldx + c + k] : 0.0);
+      a[k] = rint(ldexp(v, scale - er));
+    }
     for (int l = 0; l < mods.n; l++) {
       const double m = (double)mods.m[l];
-      double res = fma(-m, rint(a * mods.inv[l]), a);
       const double hi = (double)((mods.m[l] - 1) / 2), lo = -(double)(mods.m[l] / 2);
-      res -= res > hi ? m : 0.0;
-      res += res < lo ? m : 0.0;
-      planes[(int64_t)l * plane + r * cols_p + c] = (int8_t)(int)res;
+      uint32_t word = 0;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        double res = fma(-m, rint(a[k] * mods.inv[l]), a[k]);
+        res -= res > hi ? m : 0.0;
+        res += res < lo ? m : 0.0;
+        word |= (uint32_t)(uint8_t)(int8_t)(int)res << (8 * k);
+      }
+      *reinterpret_cast<uint32_t *>(planes + (int64_t)l * plane + r * cols_p + c) = word;
     }
   }
 }
@@ -1026,7 +1048,7 @@ static int mixed_prepare_b(PrepB &pb, int slot, int form, int64_t N, int64_t K, 
                 dim3((unsigned)((N + 63) / 64), (unsigned)((K + kColSlab - 1) / kColSlab)), dim3(256), 0, Bd, K, N,
                 ldb, fb);
     IAMX_LAUNCH("ozaki_col_exp", k_col_exp_zero, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, fb, N);
-    IAMX_LAUNCH("ozaki_crt_residues_B", k_residues_i8, dim3((unsigned)(Kp / 32), (unsigned)(Np / 32)), dim3(256), 0,
+    IAMX_LAUNCH("ozaki_crt_residues_B", k_residues_i8, dim3((unsigned)(Kp / 128), (unsigned)(Np / 32)), dim3(256), 0,
                 Bd, K, N, ldb, 1, fb, q + 1, crt_mods(S), (int8_t *)ws, Np, Kp);
     pb = PrepB{3, N, K, Np, Kp, S, ws, fb, true, p, q};
     return IAMX_OK;
@@ -1044,7 +1066,7 @@ static int mixed_prepare_b(PrepB &pb, int slot, int form, int64_t N, int64_t K, 
   IAMX_LAUNCH("ozaki_col_exp", k_col_exp, dim3((unsigned)((N + 63) / 64), (unsigned)((K + kColSlab - 1) / kColSlab)),
               dim3(256), 0, Bd, K, N, ldb, fb);
   IAMX_LAUNCH("ozaki_col_exp", k_col_exp_zero, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, fb, N);
-  IAMX_LAUNCH("ozaki_slice_B", k_slice_i8, dim3((unsigned)(Kp / 32), (unsigned)(Np / 32)), dim3(256), 0, Bd, K, N,
+  IAMX_LAUNCH("ozaki_slice_B", k_slice_i8, dim3((unsigned)(Kp / 128), (unsigned)(Np / 32)), dim3(256), 0, Bd, K, N,
               ldb, 1, fb, S, (int8_t *)ws, Np, Kp, 1);
   pb = PrepB{2, N, K, Np, Kp, S, ws, fb, true, 0, 0};
   return IAMX_OK;
@@ -1119,7 +1141,7 @@ static int mixed_rows(const PrepB &pb, int form, int64_t M, const void *A, int64
     uint8_t *R = ws + rup((int64_t)(a_bytes + (size_t)Mp * 4), 256);
     const double *Ad = (const double *)A;
     IAMX_LAUNCH("ozaki_row_exp", k_row_exp, dim3((unsigned)((M + 3) / 4)), dim3(256), 0, Ad, M, K, lda, ea);
-    IAMX_LAUNCH("ozaki_crt_residues_A", k_residues_i8, dim3((unsigned)(Kp / 32), (unsigned)(Mp / 32)), dim3(256), 0,
+    IAMX_LAUNCH("ozaki_crt_residues_A", k_residues_i8, dim3((unsigned)(Kp / 128), (unsigned)(Mp / 32)), dim3(256), 0,
                 Ad, M, K, lda, 0, ea, pb.p + 1, crt_mods(n), (int8_t *)ws, Mp, Kp);
     P.A = ws;
     P.lda = Kp;
@@ -1166,7 +1188,7 @@ static int mixed_rows(const PrepB &pb, int form, int64_t M, const void *A, int64
   int32_t *lacc = (int32_t *)(ws + rup((int64_t)(a_bytes + (size_t)Mp * 4), 256));
   const double *Ad = (const double *)A;
   IAMX_LAUNCH("ozaki_row_exp", k_row_exp, dim3((unsigned)((M + 3) / 4)), dim3(256), 0, Ad, M, K, lda, ea);
-  IAMX_LAUNCH("ozaki_slice_A", k_slice_i8, dim3((unsigned)(Kp / 32), (unsigned)(Mp / 32)), dim3(256), 0, Ad, M, K,
+  IAMX_LAUNCH("ozaki_slice_A", k_slice_i8, dim3((unsigned)(Kp / 128), (unsigned)(Mp / 32)), dim3(256), 0, Ad, M, K,
               lda, 0, ea, S, (int8_t *)ws, Mp, Kp, 0);
   // rows of A hold digits 1..S, rows of B^T digits S..1: the pair (s, t) of
   // a level sits at A column block s-1 and B column block S-t
