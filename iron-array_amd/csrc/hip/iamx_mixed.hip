@@ -305,6 +305,7 @@ __global__ __launch_bounds__(256) void k_residues_i8(const double *__restrict__ 
 struct CrtCombine {
   int32_t n;
   int32_t m[kMaxMods];               // the moduli
+  double minv[kMaxMods];             // 1 / m_l, rounded
   uint32_t w[kMaxMods][kLimbs - 1];  // W_l in 20-bit limbs, least significant first
   uint32_t mlimb[kLimbs];            // M
   uint32_t half[kLimbs];             // ceil(M / 2)
@@ -323,11 +324,16 @@ __device__ __forceinline__ bool limbs_ge(const uint32_t *r, const uint32_t *s) {
 // C[i][j] = 2^(e_i + f_j - p - q - 2) * C', C' the representative in
 // [-M/2, M/2) of sum_l r_l W_l (mod M), r_l the residue byte of plane l.
 // Everything up to the conversion to FP64 is exact integer arithmetic.
-// r_l: a residue byte in [0, m_l), or an exact INT32 product sum reduced here
-__device__ __forceinline__ uint32_t residue(uint8_t v, int32_t) { return v; }
-__device__ __forceinline__ uint32_t residue(int32_t v, int32_t m) {
-  const int32_t r = v % m;
-  return (uint32_t)(r < 0 ? r + m : r);
+// r_l: a residue byte in [0, m_l), or an exact INT32 product sum reduced
+// here: the quotient from the FP64 product with 1/m (exact v, so off by at
+// most one) and one correction each way
+__device__ __forceinline__ uint32_t residue(uint8_t v, int32_t, double) { return v; }
+__device__ __forceinline__ uint32_t residue(int32_t v, int32_t m, double minv) {
+  // (wrapping unsigned arithmetic: the true difference lies in [-m, 2m))
+  int32_t r = (int32_t)((uint32_t)v - (uint32_t)(int32_t)floor((double)v * minv) * (uint32_t)m);
+  r += r < 0 ? m : 0;
+  r -= r >= m ? m : 0;
+  return (uint32_t)r;
 }
 
 template <typename RT>
@@ -340,7 +346,7 @@ __global__ __launch_bounds__(256) void k_crt_combine(const RT *__restrict__ R, i
   uint32_t acc[kLimbs] = {0, 0, 0, 0, 0, 0, 0};
   const RT *rp = R + i * ldr + j;
   for (int l = 0; l < P.n; l++) {
-    const uint32_t r = residue(rp[(int64_t)l * plane], P.m[l]);
+    const uint32_t r = residue(rp[(int64_t)l * plane], P.m[l], P.minv[l]);
 #pragma unroll
     for (int k = 0; k < kLimbs - 1; k++) acc[k] += __umul24(r, P.w[l][k]);  // < 15 * 255 * 2^20 < 2^32
   }
@@ -1010,7 +1016,10 @@ static CrtCombine crt_combine_consts(int n, int p, int q) {
   to_limbs(M, c.mlimb, kLimbs);
   to_limbs(M / 2, c.half, kLimbs);
   c.inv_m = 1.0 / (double)M;
-  for (int l = 0; l < n; l++) c.m[l] = kMods[l];
+  for (int l = 0; l < n; l++) {
+    c.m[l] = kMods[l];
+    c.minv[l] = 1.0 / (double)kMods[l];
+  }
   c.p1 = p + 1;
   c.q1 = q + 1;
   return c;
