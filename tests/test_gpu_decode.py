@@ -309,6 +309,14 @@ LZ4_PLANS = {
     "huge-lengths": ([(0, 3), (17000, 20000)], [(1, 300), (5000, 9000)], [(20000, 60000), (4, 30)]),
     "mixed": ([(0, 0), (1, 14), (15, 59), (60, 700)], [(1, 7), (8, 63), (64, 4031), (4032, 65535)],
               [(4, 18), (19, 273), (274, 3000)]),
+    # the kernel's limits, each value exactly: literals around one length byte
+    # and the walk's 59, offsets around 64 (periodic copies), the 4 KiB ring
+    # (4032 = RING - 64) and its size, match lengths around 64-byte steps and
+    # the walk's 273
+    "edges": ([(v, v) for v in (0, 1, 14, 15, 16, 58, 59, 60, 63, 64, 65)],
+              [(v, v) for v in (1, 2, 3, 4, 31, 32, 33, 63, 64, 65, 255, 256, 4031, 4032, 4033, 4095, 4096, 4097,
+                                8191, 8192, 65535)],
+              [(v, v) for v in (4, 5, 18, 19, 20, 63, 64, 65, 128, 272, 273, 274, 275)]),
 }
 
 
